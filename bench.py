@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""MPSFT recovery on one MI355X: microseconds per full solve (Demo1::Run to
+its stale stop) at N=2^22, S=1000 modes with log-uniform magnitudes in
+[0.1, 10] and noise sigma=1e-7.  The timed form is MIXED (complex64 input,
+thresholds calibrated to sigma_eff = sqrt(sigma^2 + sigma_q^2)); the FP64
+form on the same complex64-rounded input, and the plain FP64 form on the
+complex128 input with sigma, are timed beside it.
+
+A step is one solve; inputs are resident in device memory before timing.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N, K, SIGMA, DELTA = 1 << 22, 1000, 1e-7, 1e-6
+SIGNAL_SEED, SOLVE_SEED = 2, 11
+HBM_PEAK_GBS = 8000.0
+METRIC = "µs per MPSFT solve N=2^22 S=1000: mixed vs FP64 GPU at 1e-6 coef err"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_solves(M, solver, xd, steps, warmup):
+    for _ in range(warmup):
+        solver.solve(xd, SOLVE_SEED)
+    M.synchronize()
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(steps):
+        iters = solver.solve(xd, SOLVE_SEED)
+    M.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e6, iters
+
+
+def gather_bytes(M, solver, elem_bytes):
+    """Algorithmic bytes of every k_bin_time launch of the last solve: 64 B
+    per distinct 64-B line of x its gathers touch, plus the fold it writes
+    (rows x B double2).  Also the reference's logical gather bytes."""
+    bins_log, _ = solver.log()
+    tfs, qs = solver.transforms()
+    line_bytes = logical = 0
+    for it, B in enumerate(bins_log):
+        B = int(B)
+        P = M.lib().mpsft_window_p(N, B, DELTA)
+        p2 = (P - 1) // 2
+        bits = M.num_bits(N, B)
+        a, c = int(tfs[it, 0]), int(tfs[it, 2])
+        for q in qs[it]:
+            offs = [0] + [s * B * (1 << k) for k in range(bits) for s in (1, -1)]
+            u = np.unique(np.concatenate([np.arange(o - p2, o + p2 + 1, dtype=np.int64) + int(q)
+                                          for o in offs]))
+            j = (a * (u % N) + c) % N
+            lines = np.unique(j * elem_bytes // 64).size
+            rows = 1 + 2 * bits
+            line_bytes += 64 * lines + rows * B * 16
+            logical += rows * P * elem_bytes
+    return line_bytes, logical, len(bins_log)
+
+
+def cpu_baseline(x, sample_solves):
+    """The FP64 CPU restatement of the reference (single thread, V4-order
+    binning) on the plain FP64 workload."""
+    from oracle import pyoracle as O
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(sample_solves):
+        O.seed(SOLVE_SEED)
+        _, bl, _ = O.demo1_run(x, K, SIGMA)
+        iters = len(bl)
+    dt = (time.perf_counter() - t0) / sample_solves
+    O.set_threads(0)
+    return dt * 1e6, iters
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-solves", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (one signal's solve per device); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import mpsft_amd as M
+    from mpsft_amd.signals import make_signal, quantization_sigma
+
+    log("generating the N=2^22, S=1000 signal")
+    x, truth = make_signal(N, K, SIGMA, seed=SIGNAL_SEED, magnitude="loguniform")
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(SIGMA, quantization_sigma(x, x64)))
+
+    mixed = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_MIXED)
+    fp64q = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
+    fp64 = M.Solver(N, M.solve_options(K, SIGMA, window_delta=DELTA), M.MPSFT_FP64)
+    xd64 = mixed.upload(x64)
+    xd128q = fp64q.upload(x64.astype(np.complex128))
+    xd128 = fp64.upload(x)
+    M.synchronize()
+
+    log("timing")
+    us_mixed, it_mixed = time_solves(M, mixed, xd64, args.steps, args.warmup)
+    us_fp64q, it_fp64q = time_solves(M, fp64q, xd128q, args.steps, args.warmup)
+    us_fp64, it_fp64 = time_solves(M, fp64, xd128, args.steps, args.warmup)
+
+    # Accuracy of the timed result: MIXED vs FP64 on the same input.
+    gm, gq = mixed.modes_dict(), fp64q.modes_dict()
+    same_support = set(gm) == set(gq)
+    rel = [abs(gm[k] - gq[k]) / abs(gq[k]) for k in gq if k in gm]
+    max_rel = max(rel) if rel else None
+    hits = sum(1 for k in truth if k in gm)
+    truth_err = max(abs(gm[k] - v) / abs(v) for k, v in truth.items() if k in gm)
+
+    line = {
+        "metric": METRIC,
+        "value": round(us_mixed, 2),
+        "unit": "us/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(us_mixed / 1000.0, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 input; f64 phase/FFT/pencil",
+        "data": "synthetic",
+        "config": {"workload": "MPSFT solve N=2^22 S=1000 |c| log-uniform [0.1,10] sigma=1e-7",
+                   "n": N, "k": K, "sigma": SIGMA, "sigma_eff": sig_eff, "window_delta": DELTA,
+                   "solve_seed": SOLVE_SEED, "signal_seed": SIGNAL_SEED},
+        "solves_per_sec": round(1e6 / us_mixed, 1),
+        "iterations": it_mixed,
+        "fp64_same_input_us": round(us_fp64q, 2),
+        "fp64_same_input_iterations": it_fp64q,
+        "mixed_speedup_vs_fp64": round(us_fp64q / us_mixed, 3),
+        "fp64_c128_sigma_us": round(us_fp64, 2),
+        "fp64_c128_sigma_iterations": it_fp64,
+        "support_identical_mixed_vs_fp64": same_support,
+        "max_rel_coef_diff_mixed_vs_fp64": max_rel,
+        "true_modes_recovered": hits,
+        "max_rel_coef_err_vs_truth": truth_err,
+    }
+
+    if not args.no_roofline:
+        log("profiled solve for the roofline")
+        it_p, ms = mixed.solve_profiled(xd64, SOLVE_SEED)
+        line_bytes, logical, n_launch = gather_bytes(M, mixed, 8)
+        t_bt = ms[0] / 1000.0
+        achieved = line_bytes / t_bt / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_bin_time<float2>", "launches": n_launch,
+            "avg_launch_us": round(ms[0] * 1000.0 / n_launch, 2),
+            "algorithmic_bytes_per_launch": int(line_bytes / n_launch),
+            "logical_gather_GBs": round(logical / t_bt / 1e9, 1),
+        }
+        line["kernel_ms_per_solve"] = {"k_bin_time": round(ms[0], 4), "k_bin_fft": round(ms[1], 4),
+                                       "k_identify": round(ms[2], 4)}
+
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: {args.cpu_solves} single-thread oracle solves")
+        us_cpu, it_cpu = cpu_baseline(x, args.cpu_solves)
+        line["cpu_baseline"] = {
+            "value": round(us_cpu, 1), "unit": "us/solve", "cores": 1, "kind": "port",
+            "sample": f"{args.cpu_solves} full FP64 solves of the same N=2^22 S=1000 signal "
+                      f"(complex128, sigma=1e-7, {it_cpu} iterations each)",
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

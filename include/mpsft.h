@@ -1,0 +1,198 @@
+/*
+ * mpsft.h -- C-ABI of the MI355X MPSFT recovery loop (libmpsft_hip.so).
+ *
+ * Plain pointers and sizes only.  Every device pointer is a HIP device
+ * address owned by the caller; every call is ordered on the HIP stream it
+ * is given (NULL = the default stream).  Complex arrays are interleaved
+ * (re, im): double2 for MPSFT_FP64, float2 for the MPSFT_MIXED input x.
+ * No call aborts: errors come back as an mpsft_status.
+ *
+ * Each entry point replaces a reference (jiaweichiu/mpsft) interface; the
+ * file:line of that interface is given beside it (paths under src/).
+ */
+#ifndef MPSFT_H
+#define MPSFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPSFT_VERSION_MAJOR 0
+#define MPSFT_VERSION_MINOR 1
+
+/* Status codes.  The reference CHECK-aborts (glog) where these are
+ * returned: even bins (src/window.cc:33), wrong row count
+ * (src/binner.cc:83,410), unknown binner type (src/binner.cc:74,533),
+ * window_threshold <= 0 (src/iterate.cc:84), even trials
+ * (src/iterate.cc:86). */
+typedef enum mpsft_status {
+  MPSFT_OK = 0,
+  MPSFT_ERR_INVALID_ARG = 1, /* even bins / trials, bad n, bad threshold */
+  MPSFT_ERR_SHAPE = 2,       /* rows / sizes do not match the plan */
+  MPSFT_ERR_UNSUPPORTED = 3, /* unknown binner type or precision, B too big */
+  MPSFT_ERR_HIP = 4,         /* a HIP runtime call failed */
+  MPSFT_ERR_NOMEM = 5,       /* device allocation failed */
+  MPSFT_ERR_CAPACITY = 6,    /* found-mode list overflowed its capacity */
+  MPSFT_ERR_NOT_CONVERGED = 7 /* solve hit its iteration cap */
+} mpsft_status;
+
+/* Arithmetic forms.  FP64: every stage in double, input complex128, equal
+ * to the reference.  MIXED: input complex64; gathers, window products and
+ * the fold in FP32 with compensated accumulation; phases, bin FFT, pencil
+ * and rounding in FP64. */
+typedef enum mpsft_precision {
+  MPSFT_FP64 = 0,
+  MPSFT_MIXED = 1
+} mpsft_precision;
+
+/* Transform{a, b, c, a_inv} (src/integer.h:75-82): y_t = x_{a t + c}
+ * e^{2 pi i b t / n}.  Values may be unreduced or negative, as the
+ * reference allows (src/binner_test.cc:124); they are reduced mod n. */
+typedef struct mpsft_transform {
+  int64_t a, b, c, a_inv;
+} mpsft_transform;
+
+const char *mpsft_status_string(int status);
+int mpsft_version(void);
+
+/* ---- device memory / stream helpers (for callers without a runtime) --- */
+int mpsft_device_count(int *count);
+int mpsft_malloc(void **ptr, size_t bytes);
+int mpsft_free(void *ptr);
+int mpsft_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mpsft_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mpsft_memset(void *dst, int value, size_t bytes);
+int mpsft_stream_create(void **stream);
+int mpsft_stream_destroy(void *stream);
+int mpsft_stream_synchronize(void *stream);
+int mpsft_device_synchronize(void);
+
+/* ---- integer helpers ----
+ * mpsft_transform_make: Transform(n, a, b, c) (src/integer.cc:51-56) with
+ * the inverse found by extended Euclid (equal to the Fermat inverse of
+ * src/integer.cc:39-40 for prime n, and valid for odd a when n = 2^k). */
+int mpsft_transform_make(int64_t n, int64_t a, int64_t b, int64_t c,
+                         mpsft_transform *tf);
+/* NumBits (src/iterate.cc:30-42). */
+int32_t mpsft_num_bits(int64_t n, int32_t bins);
+/* Window::p() (src/window.cc:40-50); 0 on invalid arguments. */
+int32_t mpsft_window_p(int64_t n, int32_t bins, double delta);
+
+/* ---- plan: one window + binner pair ----
+ * Replaces Window(n, bins, delta) (src/window.cc:28) together with
+ * BinInTime::Create / BinInFreq::Create (src/binner.cc:58-76, 526-535).
+ * bits gives 1 + 2*bits rows (src/binner.h:27-28). */
+typedef struct mpsft_plan mpsft_plan;
+int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits,
+                      int precision, mpsft_plan **plan);
+int mpsft_plan_destroy(mpsft_plan *plan);
+
+/* BinInTime::Run (src/binner.h:33-34; V0 definition src/binner.cc:81-111,
+ * default V4 src/binner.cc:394-521).  x_dev: n complex (double2 for FP64,
+ * float2 for MIXED).  out_dev: (1+2*bits) x bins complex double, row-major,
+ * overwritten. */
+int mpsft_bin_in_time(mpsft_plan *plan, const void *x_dev, int64_t n,
+                      const mpsft_transform *tf, int64_t q, void *out_dev,
+                      void *stream);
+
+/* BinInFreq::Run (src/binner.h:126-127; default V1 src/binner.cc:565-591).
+ * Subtracts the window response of the m modes (k0_dev: int64 locations,
+ * coef_dev: double2 coefficients) from out_dev in place. */
+int mpsft_bin_in_freq(mpsft_plan *plan, const int64_t *k0_dev,
+                      const void *coef_dev, int64_t m,
+                      const mpsft_transform *tf, int64_t q, void *out_dev,
+                      void *stream);
+
+/* MatPencil (src/freqid.cc:27-48), batched on the device: for i < m,
+ * u_dev[3i..3i+2] = (u0, u1, u2) double2; writes sigma_dev[2i..2i+1] and
+ * vote_dev[i] (1 = second half). */
+int mpsft_mat_pencil(const void *u_dev, int64_t m, double *sigma_dev,
+                     uint8_t *vote_dev, void *stream);
+
+/* ---- solver: Iterate and Demo1::Run on the device ---- */
+
+/* IterateOptions (src/iterate.h:26-59).  Types: bin_in_time_type 5/6 and
+ * bin_in_freq_type 5/6 select the HIP FP64 / MIXED forms; the reference's
+ * CPU types 0-4 / 0-1 are not provided by this library. */
+typedef struct mpsft_iterate_options {
+  int32_t bins;
+  double window_delta;
+  int32_t trials;
+  double bin_threshold;
+  double window_threshold;
+  double sv_threshold;
+  int32_t bin_in_time_type;
+  int32_t bin_in_freq_type;
+} mpsft_iterate_options;
+
+/* Demo1Options (src/demo1.h:25-33) plus the mode count k and noise sigma
+ * of Demo1(opt, n, k, sigma) (src/demo1.cc:27). */
+typedef struct mpsft_solve_options {
+  int32_t trials;
+  int32_t min_bins;
+  double window_delta;
+  double window_threshold;
+  int32_t max_stale_iter;
+  int32_t k;
+  double sigma;
+  int32_t max_iter; /* safety cap on iterations; 0 = 100000 */
+} mpsft_solve_options;
+
+typedef struct mpsft_solver mpsft_solver;
+
+/* A solver owns the device state of one signal length n: found-mode
+ * storage (a dense n-entry coefficient array plus a key list), per-B
+ * window tables and the iteration graph.  capacity bounds the number of
+ * distinct found modes (0 = 8*k + 4096). */
+int mpsft_solver_create(int64_t n, const mpsft_solve_options *opt,
+                        int precision, int64_t capacity, mpsft_solver **s);
+int mpsft_solver_destroy(mpsft_solver *s);
+
+/* Demo1::Run (src/demo1.cc:36-59) on x_dev.  The transforms and q's are
+ * drawn on the host from a std::mt19937 seeded with seed, in the order of
+ * Transform(n) (src/integer.cc:44-49) then RandomInt32() % n per trial
+ * (src/iterate.cc:103); for n = 2^k the multiplier is drawn odd.
+ * Returns the iteration count in *iters. */
+int mpsft_solve(mpsft_solver *s, const void *x_dev, uint32_t seed,
+                int32_t *iters, void *stream);
+
+/* Instrumentation, no reference counterpart: the same solve with each
+ * kernel bracketed by HIP events on `stream` (no graph), returning the
+ * summed milliseconds of the gather-fold, bin-FFT and identify kernels in
+ * kernel_ms[0..2].  Results are identical to mpsft_solve. */
+int mpsft_solve_profiled(mpsft_solver *s, const void *x_dev, uint32_t seed,
+                         int32_t *iters, double *kernel_ms, void *stream);
+
+/* Iterate (src/iterate.cc:80-177) once with explicit options, transform
+ * and q's (qs: trials entries, host memory).  The solver's current found
+ * modes are the ModeMap; *found is the return value of Iterate. */
+int mpsft_iterate(mpsft_solver *s, const void *x_dev,
+                  const mpsft_iterate_options *opt, const mpsft_transform *tf,
+                  const int64_t *qs, int32_t *found, void *stream);
+
+/* Found-mode map access (host memory).  set replaces the map. */
+int mpsft_solver_modes_count(mpsft_solver *s, int64_t *count);
+int mpsft_solver_modes_get(mpsft_solver *s, int64_t *keys, double *coefs,
+                           int64_t cap, int64_t *count);
+int mpsft_solver_modes_set(mpsft_solver *s, const int64_t *keys,
+                           const double *coefs, int64_t m);
+/* After an mpsft_iterate: copies the last iteration's bin matrices
+ * (trials x rows x bins double2) into host memory. */
+int mpsft_solver_bins_get(mpsft_solver *s, double *out, int64_t cap_doubles,
+                          int32_t *trials, int32_t *rows, int32_t *bins);
+/* Per-iteration log of the last solve: bins and found count. */
+int mpsft_solver_log_get(mpsft_solver *s, int32_t *bins, int32_t *found,
+                         int32_t cap, int32_t *n_iter);
+/* The transforms (reduced mod n) and q's (trials per iteration) the last
+ * mpsft_solve drew, for its first min(cap, iterations) iterations. */
+int mpsft_solver_transforms_get(mpsft_solver *s, mpsft_transform *tfs,
+                                int64_t *qs, int32_t cap, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSFT_H */

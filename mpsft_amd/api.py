@@ -1,0 +1,338 @@
+"""Python mirror of the reference's binner / window / freqid / iterate /
+demo1 interfaces, over the HIP library's C-ABI.
+
+Reference interfaces (jiaweichiu/mpsft, paths under src/):
+  Window(n, bins, delta)                      window.h:25-55
+  Transform(n, a, b, c)                       integer.h:75-82
+  BinInTime::Create(type, win, bits)->Run     binner.h:30-37, binner.cc:58-76
+  BinInFreq::Create(type, win, bits)->Run     binner.h:123-131, binner.cc:526-535
+  MatPencil(u0, u1, u2, sigma)                freqid.h:28
+  IterateOptions / Iterate(x, opt, mm)        iterate.h:26-65
+  Demo1Options / Demo1::Run                   demo1.h:25-53, demo1.cc:36-59
+The HIP forms are new binner types: 5 = FP64, 6 = MIXED (complex64 input).
+Types 0-4 / 0-1 are the reference's CPU binners and are rejected here, as
+an unknown type is by the reference (binner.cc:74,533), but with an error
+instead of an abort.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MPSFT_FP64, MPSFT_MIXED, IterateOptions, MpsftError, SolveOptions,
+                   Transform, check, lib)
+
+BIN_TYPE_FP64 = 5
+BIN_TYPE_MIXED = 6
+_TYPE_PRECISION = {BIN_TYPE_FP64: MPSFT_FP64, BIN_TYPE_MIXED: MPSFT_MIXED}
+
+
+def device_count():
+    n = C.c_int(0)
+    st = lib().mpsft_device_count(C.byref(n))
+    return n.value if st == 0 else 0
+
+
+class DeviceArray:
+    """A device allocation made through the C-ABI (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(lib().mpsft_malloc(C.byref(self.ptr), max(1, self.nbytes)), "mpsft_malloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().mpsft_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes), "mpsft_memcpy_htod")
+        d.dtype, d.shape = a.dtype, a.shape
+        return d
+
+    def to_numpy(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if out.nbytes:
+            check(lib().mpsft_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes),
+                  "mpsft_memcpy_dtoh")
+        return out
+
+    def zero(self):
+        check(lib().mpsft_memset(self.ptr, 0, self.nbytes), "mpsft_memset")
+
+    def free(self):
+        if self.ptr:
+            lib().mpsft_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib().mpsft_device_synchronize(), "mpsft_device_synchronize")
+
+
+def transform_make(n, a, b, c):
+    """Transform(n, a, b, c) (src/integer.cc:51-56)."""
+    tf = Transform()
+    check(lib().mpsft_transform_make(n, a, b, c, C.byref(tf)), "mpsft_transform_make")
+    return tf
+
+
+def num_bits(n, bins):
+    """NumBits (src/iterate.cc:30-42)."""
+    return lib().mpsft_num_bits(n, bins)
+
+
+class Window:
+    """Window(n, bins, delta) (src/window.cc:28-81): host-side parameters.
+    The taps themselves are computed on the device inside BinInTime."""
+
+    def __init__(self, n, bins, delta):
+        if bins <= 0 or bins % 2 != 1:
+            raise MpsftError("Window", 1)
+        self.n, self.bins_, self.delta = int(n), int(bins), float(delta)
+        self.p_ = lib().mpsft_window_p(n, bins, delta)
+        if self.p_ <= 0:
+            raise MpsftError("Window", 1)
+
+    def bins(self):
+        return self.bins_
+
+    def p(self):
+        return self.p_
+
+
+def _as_device(x, precision):
+    """numpy complex array -> (DeviceArray, keepalive)."""
+    if isinstance(x, DeviceArray):
+        return x
+    dt = np.complex128 if precision == MPSFT_FP64 else np.complex64
+    return DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=dt))
+
+
+class Plan:
+    """mpsft_plan: one window + binner pair on the device."""
+
+    def __init__(self, n, bins, delta, bits, precision=MPSFT_FP64):
+        self.n, self.bins, self.delta, self.bits = int(n), int(bins), float(delta), int(bits)
+        self.rows = 1 + 2 * self.bits
+        self.precision = precision
+        self.h = C.c_void_p()
+        check(lib().mpsft_plan_create(n, bins, delta, bits, precision, C.byref(self.h)),
+              "mpsft_plan_create")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().mpsft_plan_destroy(self.h)
+        except Exception:
+            pass
+
+    def bin_in_time_dev(self, x_dev, tf, q, out_dev, stream=None):
+        check(lib().mpsft_bin_in_time(self.h, x_dev.ptr, self.n, C.byref(tf), q, out_dev.ptr,
+                                      stream), "mpsft_bin_in_time")
+
+    def bin_in_freq_dev(self, k0_dev, coef_dev, m, tf, q, out_dev, stream=None):
+        check(lib().mpsft_bin_in_freq(self.h, k0_dev.ptr if m else None,
+                                      coef_dev.ptr if m else None, m, C.byref(tf), q,
+                                      out_dev.ptr, stream), "mpsft_bin_in_freq")
+
+    def bin_in_time(self, x, tf, q):
+        """BinInTime::Run on host data: returns (rows, bins) complex128."""
+        xd = _as_device(x, self.precision)
+        out = DeviceArray(16 * self.rows * self.bins)
+        self.bin_in_time_dev(xd, tf, q, out)
+        synchronize()
+        return out.to_numpy(np.complex128, (self.rows, self.bins))
+
+    def bin_in_freq(self, keys, coefs, tf, q, out):
+        """BinInFreq::Run on host data: subtracts from out (rows, bins)."""
+        keys = np.ascontiguousarray(keys, dtype=np.int64)
+        coefs = np.ascontiguousarray(coefs, dtype=np.complex128)
+        od = DeviceArray.from_numpy(np.ascontiguousarray(out, dtype=np.complex128))
+        kd = DeviceArray.from_numpy(keys) if keys.size else None
+        cd = DeviceArray.from_numpy(coefs) if keys.size else None
+        self.bin_in_freq_dev(kd, cd, keys.size, tf, q, od)
+        synchronize()
+        out[...] = od.to_numpy(np.complex128, out.shape)
+        return out
+
+
+class BinInTime:
+    """BinInTime::Create(type, win, bits) (src/binner.cc:58-76)."""
+
+    @staticmethod
+    def create(binner_type, win, bits):
+        if binner_type not in _TYPE_PRECISION:
+            raise MpsftError(f"BinInTime::Create(type={binner_type})", 3)
+        return _BinInTimeHip(Plan(win.n, win.bins_, win.delta, bits,
+                                  _TYPE_PRECISION[binner_type]))
+
+
+class _BinInTimeHip:
+    def __init__(self, plan):
+        self.plan = plan
+
+    def run(self, x, tf, q, out=None):
+        """Overwrites and returns the (1+2*bits, bins) bin matrix."""
+        r = self.plan.bin_in_time(x, tf, q)
+        if out is not None:
+            if out.shape != r.shape:
+                raise MpsftError("BinInTime::Run rows", 2)
+            out[...] = r
+            return out
+        return r
+
+
+class BinInFreq:
+    """BinInFreq::Create(type, win, bits) (src/binner.cc:526-535)."""
+
+    @staticmethod
+    def create(binner_type, win, bits):
+        if binner_type not in _TYPE_PRECISION:
+            raise MpsftError(f"BinInFreq::Create(type={binner_type})", 3)
+        return _BinInFreqHip(Plan(win.n, win.bins_, win.delta, bits, MPSFT_FP64))
+
+
+class _BinInFreqHip:
+    def __init__(self, plan):
+        self.plan = plan
+
+    def run(self, mm, tf, q, out):
+        """Subtracts the modes of mm ({k0: coef}) from out in place."""
+        if out.shape != (self.plan.rows, self.plan.bins):
+            raise MpsftError("BinInFreq::Run rows", 2)
+        keys = np.array(list(mm.keys()), dtype=np.int64)
+        coefs = np.array([complex(v) for v in mm.values()], dtype=np.complex128)
+        return self.plan.bin_in_freq(keys, coefs, tf, q, out)
+
+
+def mat_pencil(u0, u1, u2):
+    """MatPencil (src/freqid.cc:27-48), batched on the device.  Returns
+    (votes bool[m], sigma float64[m, 2])."""
+    u0, u1, u2 = (np.atleast_1d(np.asarray(v, dtype=np.complex128)) for v in (u0, u1, u2))
+    m = u0.size
+    u = np.stack([u0, u1, u2], axis=1).reshape(-1)
+    ud = DeviceArray.from_numpy(u)
+    sd = DeviceArray(16 * m)
+    vd = DeviceArray(m)
+    check(lib().mpsft_mat_pencil(ud.ptr, m, sd.ptr, vd.ptr, None), "mpsft_mat_pencil")
+    synchronize()
+    return vd.to_numpy(np.uint8, (m,)).astype(bool), sd.to_numpy(np.float64, (m, 2))
+
+
+def iterate_options(bins, window_delta=1e-6, trials=1, bin_threshold=0.0, window_threshold=0.0,
+                    sv_threshold=0.0, precision=MPSFT_FP64):
+    """IterateOptions (src/iterate.h:26-59) with the HIP binner types."""
+    t = BIN_TYPE_FP64 if precision == MPSFT_FP64 else BIN_TYPE_MIXED
+    return IterateOptions(bins, window_delta, trials, bin_threshold, window_threshold,
+                          sv_threshold, t, t)
+
+
+def solve_options(k, sigma, trials=1, min_bins=101, window_delta=1e-6, window_threshold=0.1,
+                  max_stale_iter=3, max_iter=0):
+    """Demo1Options (src/demo1.h:25-33) plus Demo1's k and sigma."""
+    return SolveOptions(trials, min_bins, window_delta, window_threshold, max_stale_iter, k,
+                        sigma, max_iter)
+
+
+class Solver:
+    """Device state for Iterate / Demo1::Run on signals of length n."""
+
+    def __init__(self, n, opt, precision=MPSFT_FP64, capacity=0):
+        self.n, self.opt, self.precision = int(n), opt, precision
+        self.h = C.c_void_p()
+        check(lib().mpsft_solver_create(n, C.byref(opt), precision, capacity, C.byref(self.h)),
+              "mpsft_solver_create")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().mpsft_solver_destroy(self.h)
+        except Exception:
+            pass
+
+    def upload(self, x):
+        return _as_device(x, self.precision)
+
+    def solve(self, x_dev, seed, stream=None):
+        """Demo1::Run; returns the iteration count."""
+        it = C.c_int32(0)
+        check(lib().mpsft_solve(self.h, x_dev.ptr, seed, C.byref(it), stream), "mpsft_solve")
+        return it.value
+
+    def solve_profiled(self, x_dev, seed, stream=None):
+        """The same solve with per-kernel HIP event timing; returns
+        (iterations, [ms gather-fold, ms bin FFT, ms identify])."""
+        it = C.c_int32(0)
+        ms = (C.c_double * 3)()
+        check(lib().mpsft_solve_profiled(self.h, x_dev.ptr, seed, C.byref(it), ms, stream),
+              "mpsft_solve_profiled")
+        return it.value, list(ms)
+
+    def iterate(self, x_dev, iopt, tf, qs, stream=None):
+        """Iterate with explicit transform and q's; returns Iterate's bool."""
+        q = (C.c_int64 * len(qs))(*[int(v) for v in qs])
+        found = C.c_int32(0)
+        check(lib().mpsft_iterate(self.h, x_dev.ptr, C.byref(iopt), C.byref(tf), q,
+                                  C.byref(found), stream), "mpsft_iterate")
+        return bool(found.value)
+
+    def modes(self):
+        """Found-mode map as (keys int64[m], coefs complex128[m])."""
+        cnt = C.c_int64(0)
+        check(lib().mpsft_solver_modes_count(self.h, C.byref(cnt)), "modes_count")
+        m = cnt.value
+        keys = np.zeros(max(m, 1), np.int64)
+        coefs = np.zeros(2 * max(m, 1), np.float64)
+        check(lib().mpsft_solver_modes_get(
+            self.h, keys.ctypes.data_as(C.POINTER(C.c_int64)),
+            coefs.ctypes.data_as(C.POINTER(C.c_double)), m, C.byref(cnt)), "modes_get")
+        return keys[:m], coefs.view(np.complex128)[:m]
+
+    def modes_dict(self):
+        k, c = self.modes()
+        return {int(a): complex(b) for a, b in zip(k, c)}
+
+    def set_modes(self, d):
+        keys = np.array(list(d.keys()), np.int64)
+        coefs = np.array([complex(v) for v in d.values()], np.complex128).view(np.float64)
+        check(lib().mpsft_solver_modes_set(
+            self.h, keys.ctypes.data_as(C.POINTER(C.c_int64)),
+            coefs.ctypes.data_as(C.POINTER(C.c_double)), keys.size), "modes_set")
+
+    def bins_matrix(self, trials, rows, bins):
+        out = np.zeros(2 * trials * rows * bins, np.float64)
+        t, r, b = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().mpsft_solver_bins_get(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                          out.size, C.byref(t), C.byref(r), C.byref(b)),
+              "bins_get")
+        return out.view(np.complex128).reshape(t.value, r.value, b.value)
+
+    def log(self):
+        cap = 1 << 16
+        bins = np.zeros(cap, np.int32)
+        found = np.zeros(cap, np.int32)
+        n_it = C.c_int32()
+        check(lib().mpsft_solver_log_get(self.h, bins.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         found.ctypes.data_as(C.POINTER(C.c_int32)), cap,
+                                         C.byref(n_it)), "log_get")
+        return bins[:n_it.value].copy(), found[:n_it.value].copy()
+
+    def transforms(self):
+        """(a, b, c, a_inv) rows and q's of the last solve's iterations."""
+        cap = 1 << 16
+        tfs = (Transform * cap)()
+        qs = np.zeros(cap * self.opt.trials, np.int64)
+        cnt = C.c_int32()
+        check(lib().mpsft_solver_transforms_get(self.h, tfs, qs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                cap, C.byref(cnt)), "transforms_get")
+        m = cnt.value
+        tf = np.array([(t.a, t.b, t.c, t.a_inv) for t in tfs[:m]], np.int64).reshape(m, 4)
+        return tf, qs[:m * self.opt.trials].reshape(m, self.opt.trials)

@@ -1,0 +1,65 @@
+"""Builds libmpsft_hip.so in-tree with hipcc for gfx950.
+
+Run as ``python -m mpsft_amd.build`` (or through ``__graft_entry__.build``).
+The library is always rebuilt from source.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmpsft_hip.so")
+SOURCES = ["kernels.hip", "engine.cpp", "capi.cpp"]
+ARCH = os.environ.get("MPSFT_OFFLOAD_ARCH", "gfx950")
+
+# Packed-FP32 instructions stay off (they are produced by hipcc -O3 from
+# ordinary float code otherwise).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def hipcc():
+    h = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(h):
+        raise RuntimeError("hipcc not found")
+    return h
+
+
+def build(verbose=False):
+    """Compile every source and link the shared library.  Returns its path."""
+    objs = []
+    tmp = tempfile.mkdtemp(prefix="mpsft_build_")
+    try:
+        for src in SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            lang = ["-x", "hip"] if src.endswith(".hip") else []
+            cmd = [hipcc()] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            err = "\n".join(l for l in r.stderr.splitlines()
+                            if "packed-fp32-ops" not in l and l.strip())
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc failed on {src}:\n{err}")
+            if err and verbose:
+                print(err, file=sys.stderr)
+            objs.append(obj)
+        out_tmp = LIB + ".tmp"
+        cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", out_tmp] + objs
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr}")
+        os.replace(out_tmp, LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

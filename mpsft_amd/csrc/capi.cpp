@@ -1,0 +1,257 @@
+// capi.cpp -- the extern "C" boundary declared in include/mpsft.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "../../include/mpsft.h"
+#include "engine.h"
+#include "kernels.h"
+
+using mpsft::DevTf;
+using mpsft::Engine;
+
+struct mpsft_plan {
+  Engine *eng = nullptr;
+  int64_t n = 0;
+  int32_t bins = 0, bits = 0;
+  double delta = 0;
+  mpsft::BParams bp{};
+};
+
+struct mpsft_solver {
+  Engine *eng = nullptr;
+  mpsft_solve_options opt{};
+};
+
+namespace {
+
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int Hip(hipError_t e) { return e == hipSuccess ? MPSFT_OK : MPSFT_ERR_HIP; }
+
+bool TfOk(int64_t n, const mpsft_transform *tf) {
+  return tf && n >= 2 && mpsft::InvMod(tf->a, n) != 0;
+}
+
+DevTf Reduce(int64_t n, const mpsft_transform *tf) {
+  return mpsft::ReduceTf(n, tf->a, tf->b, tf->c, tf->a_inv);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpsft_status_string(int status) {
+  switch (status) {
+    case MPSFT_OK: return "ok";
+    case MPSFT_ERR_INVALID_ARG: return "invalid argument";
+    case MPSFT_ERR_SHAPE: return "shape mismatch";
+    case MPSFT_ERR_UNSUPPORTED: return "unsupported";
+    case MPSFT_ERR_HIP: return "HIP runtime error";
+    case MPSFT_ERR_NOMEM: return "device allocation failed";
+    case MPSFT_ERR_CAPACITY: return "found-mode capacity exceeded";
+    case MPSFT_ERR_NOT_CONVERGED: return "iteration cap reached";
+    default: return "unknown status";
+  }
+}
+
+int mpsft_version(void) { return MPSFT_VERSION_MAJOR * 100 + MPSFT_VERSION_MINOR; }
+
+int mpsft_device_count(int *count) {
+  if (!count) return MPSFT_ERR_INVALID_ARG;
+  *count = 0;
+  return Hip(hipGetDeviceCount(count));
+}
+int mpsft_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return MPSFT_ERR_INVALID_ARG;
+  return hipMalloc(ptr, bytes) == hipSuccess ? MPSFT_OK : MPSFT_ERR_NOMEM;
+}
+int mpsft_free(void *ptr) { return Hip(hipFree(ptr)); }
+int mpsft_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  return Hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+int mpsft_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  return Hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+int mpsft_memset(void *dst, int value, size_t bytes) {
+  return Hip(hipMemset(dst, value, bytes));
+}
+int mpsft_stream_create(void **stream) {
+  if (!stream) return MPSFT_ERR_INVALID_ARG;
+  hipStream_t s;
+  const int rc = Hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = rc == MPSFT_OK ? reinterpret_cast<void *>(s) : nullptr;
+  return rc;
+}
+int mpsft_stream_destroy(void *stream) { return Hip(hipStreamDestroy(S(stream))); }
+int mpsft_stream_synchronize(void *stream) { return Hip(hipStreamSynchronize(S(stream))); }
+int mpsft_device_synchronize(void) { return Hip(hipDeviceSynchronize()); }
+
+int mpsft_transform_make(int64_t n, int64_t a, int64_t b, int64_t c, mpsft_transform *tf) {
+  if (!tf || n < 2) return MPSFT_ERR_INVALID_ARG;
+  const int64_t inv = mpsft::InvMod(a, n);
+  if (inv == 0) return MPSFT_ERR_INVALID_ARG;
+  tf->a = a;
+  tf->b = b;
+  tf->c = c;
+  tf->a_inv = inv;
+  return MPSFT_OK;
+}
+
+int32_t mpsft_num_bits(int64_t n, int32_t bins) { return mpsft::NumBits(n, bins); }
+
+int32_t mpsft_window_p(int64_t n, int32_t bins, double delta) {
+  mpsft::BParams bp;
+  return mpsft::WindowParams(n, bins, delta, &bp) ? bp.P : 0;
+}
+
+int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits, int precision,
+                      mpsft_plan **plan) {
+  if (!plan) return MPSFT_ERR_INVALID_ARG;
+  *plan = nullptr;
+  mpsft::BParams bp;
+  if (!mpsft::WindowParams(n, bins, delta, &bp) || bits < 1 || bits > 30)
+    return MPSFT_ERR_INVALID_ARG;
+  if (bp.logM > mpsft::kMaxLogM) return MPSFT_ERR_UNSUPPORTED;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  mpsft_plan *p = new (std::nothrow) mpsft_plan;
+  if (!p) return MPSFT_ERR_NOMEM;
+  p->eng = new (std::nothrow) Engine(n, precision, 1);
+  if (!p->eng || p->eng->status() != MPSFT_OK) {
+    const int rc = p->eng ? p->eng->status() : MPSFT_ERR_NOMEM;
+    delete p->eng;
+    delete p;
+    return rc;
+  }
+  p->n = n;
+  p->bins = bins;
+  p->bits = bits;
+  p->delta = delta;
+  p->bp = bp;
+  p->bp.bits = bits;
+  *plan = p;
+  return MPSFT_OK;
+}
+
+int mpsft_plan_destroy(mpsft_plan *plan) {
+  if (!plan) return MPSFT_ERR_INVALID_ARG;
+  delete plan->eng;
+  delete plan;
+  return MPSFT_OK;
+}
+
+int mpsft_bin_in_time(mpsft_plan *plan, const void *x_dev, int64_t n, const mpsft_transform *tf,
+                      int64_t q, void *out_dev, void *stream) {
+  if (!plan || !x_dev || !out_dev) return MPSFT_ERR_INVALID_ARG;
+  if (n != plan->n) return MPSFT_ERR_SHAPE;
+  if (!TfOk(n, tf)) return MPSFT_ERR_INVALID_ARG;
+  return plan->eng->BinOnly(x_dev, plan->bins, plan->delta, plan->bits, Reduce(n, tf), q,
+                            static_cast<double2 *>(out_dev), S(stream));
+}
+
+int mpsft_bin_in_freq(mpsft_plan *plan, const int64_t *k0_dev, const void *coef_dev, int64_t m,
+                      const mpsft_transform *tf, int64_t q, void *out_dev, void *stream) {
+  if (!plan || !out_dev || m < 0 || (m > 0 && (!k0_dev || !coef_dev)))
+    return MPSFT_ERR_INVALID_ARG;
+  if (!TfOk(plan->n, tf)) return MPSFT_ERR_INVALID_ARG;
+  const int64_t n = plan->n;
+  const int64_t mask = (n & (n - 1)) == 0 ? n - 1 : -1;
+  int64_t qq = q % n;
+  if (qq < 0) qq += n;
+  return Hip(mpsft::LaunchBinInFreqDirect(n, mask, plan->bp, Reduce(n, tf), qq, k0_dev,
+                                          static_cast<const double2 *>(coef_dev), m,
+                                          static_cast<double2 *>(out_dev), S(stream)));
+}
+
+int mpsft_mat_pencil(const void *u_dev, int64_t m, double *sigma_dev, uint8_t *vote_dev,
+                     void *stream) {
+  if (m < 0 || (m > 0 && (!u_dev || !sigma_dev || !vote_dev))) return MPSFT_ERR_INVALID_ARG;
+  return Hip(mpsft::LaunchMatPencil(static_cast<const double2 *>(u_dev), m, sigma_dev, vote_dev,
+                                    S(stream)));
+}
+
+int mpsft_solver_create(int64_t n, const mpsft_solve_options *opt, int precision,
+                        int64_t capacity, mpsft_solver **s) {
+  if (!s || !opt) return MPSFT_ERR_INVALID_ARG;
+  *s = nullptr;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  if (opt->k < 0 || n < 2) return MPSFT_ERR_INVALID_ARG;
+  int64_t cap = capacity > 0 ? capacity : 8 * int64_t(opt->k) + 4096;
+  if (cap > n) cap = n;
+  mpsft_solver *p = new (std::nothrow) mpsft_solver;
+  if (!p) return MPSFT_ERR_NOMEM;
+  p->eng = new (std::nothrow) Engine(n, precision, cap);
+  if (!p->eng || p->eng->status() != MPSFT_OK) {
+    const int rc = p->eng ? p->eng->status() : MPSFT_ERR_NOMEM;
+    delete p->eng;
+    delete p;
+    return rc;
+  }
+  p->opt = *opt;
+  *s = p;
+  return MPSFT_OK;
+}
+
+int mpsft_solver_destroy(mpsft_solver *s) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  delete s->eng;
+  delete s;
+  return MPSFT_OK;
+}
+
+int mpsft_solve(mpsft_solver *s, const void *x_dev, uint32_t seed, int32_t *iters,
+                void *stream) {
+  if (!s || !x_dev || !iters) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->Solve(x_dev, s->opt, seed, iters, S(stream));
+}
+
+int mpsft_solve_profiled(mpsft_solver *s, const void *x_dev, uint32_t seed, int32_t *iters,
+                         double *kernel_ms, void *stream) {
+  if (!s || !x_dev || !iters || !kernel_ms) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->Solve(x_dev, s->opt, seed, iters, S(stream), kernel_ms);
+}
+
+int mpsft_iterate(mpsft_solver *s, const void *x_dev, const mpsft_iterate_options *opt,
+                  const mpsft_transform *tf, const int64_t *qs, int32_t *found, void *stream) {
+  if (!s || !x_dev || !opt || !qs || !found) return MPSFT_ERR_INVALID_ARG;
+  if (!TfOk(s->eng->n(), tf)) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->IterateOnce(x_dev, *opt, Reduce(s->eng->n(), tf), qs, found, S(stream));
+}
+
+int mpsft_solver_modes_count(mpsft_solver *s, int64_t *count) {
+  if (!s || !count) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->ModesCount(count);
+}
+
+int mpsft_solver_modes_get(mpsft_solver *s, int64_t *keys, double *coefs, int64_t cap,
+                           int64_t *count) {
+  if (!s || !count || (cap > 0 && (!keys || !coefs))) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->ModesGet(keys, coefs, cap, count);
+}
+
+int mpsft_solver_modes_set(mpsft_solver *s, const int64_t *keys, const double *coefs,
+                           int64_t m) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->ModesSet(keys, coefs, m);
+}
+
+int mpsft_solver_bins_get(mpsft_solver *s, double *out, int64_t cap_doubles, int32_t *trials,
+                          int32_t *rows, int32_t *bins) {
+  if (!s || !out || !trials || !rows || !bins) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->BinsGet(out, cap_doubles, trials, rows, bins);
+}
+
+int mpsft_solver_log_get(mpsft_solver *s, int32_t *bins, int32_t *found, int32_t cap,
+                         int32_t *n_iter) {
+  if (!s || !n_iter) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->LogGet(bins, found, cap, n_iter);
+}
+
+int mpsft_solver_transforms_get(mpsft_solver *s, mpsft_transform *tfs, int64_t *qs, int32_t cap,
+                                int32_t *count) {
+  if (!s || !count || (cap > 0 && (!tfs || !qs))) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->TransformsGet(reinterpret_cast<DevTf *>(tfs), qs, cap, s->opt.trials, count);
+}
+
+}  // extern "C"

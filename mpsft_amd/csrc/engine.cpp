@@ -1,0 +1,724 @@
+// engine.cpp -- host side of the MPSFT engine: tables, transform draws,
+// device buffers and the iteration graph.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace mpsft {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrt2 = 1.41421356237309504880;
+
+bool IsPow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int64_t Gcd(int64_t a, int64_t b) {
+  a = a < 0 ? -a : a;
+  b = b < 0 ? -b : b;
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int64_t PosMod(int64_t v, int64_t n) {
+  const int64_t r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename T>
+int Alloc(T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) return MPSFT_OK;
+  if (hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return MPSFT_ERR_NOMEM;
+  }
+  return MPSFT_OK;
+}
+
+#define HIP_OK(expr)                                 \
+  do {                                               \
+    if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
+  } while (0)
+
+// Segment count and k_bin_time tiles per trial for one (B, bits, f).
+void TileShape(int32_t B, int32_t bits, int32_t f, int32_t *tiles, int32_t *len0) {
+  int K = 0;
+  while (K < bits && (1 << K) < f) ++K;
+  const int32_t nseg = 1 + 2 * (bits - K);
+  const int32_t nblk = (B + kResBlock - 1) / kResBlock;
+  *tiles = nseg * nblk;
+  *len0 = (K ? (1 << K) : 0) + f;
+}
+
+int32_t ChunkIters() {
+  const char *e = std::getenv("MPSFT_GRAPH_ITERS");
+  if (e) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4096) return v;
+  }
+  return 8;
+}
+
+}  // namespace
+
+// NumBits (src/iterate.cc:30-42): 1 + floor(log2(n / bins)), at least 1.
+int32_t NumBits(int64_t n, int32_t bins) {
+  if (n < bins) return 1;
+  int32_t out = 0;
+  while (n >= bins) {
+    n >>= 1;
+    ++out;
+  }
+  return out;
+}
+
+// Window parameters with the reference's expressions (src/window.cc:28-50).
+bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp) {
+  if (n <= 0 || bins <= 0 || !(delta > 0) || bins % 2 != 1) return false;
+  const double width = 1.0 / (2.0 * bins);
+  const double sqrt_c_delta = std::sqrt(-std::log(delta));
+  const double sigma_f = 0.5 / (bins * 2.0 * kSqrt2 * sqrt_c_delta);
+  const double sigma_t = 1.0 / ((2.0 * kPi) * sigma_f);
+  const double tmp = 2.0 * kSqrt2 * sigma_t * sqrt_c_delta + 1;
+  int32_t factor = int32_t(std::ceil(tmp / double(bins)));
+  if (factor % 2 == 0) ++factor;
+  bp->B = bins;
+  bp->f = factor;
+  bp->P = factor * bins;
+  bp->bits = NumBits(n, bins);
+  int32_t logM = 1;
+  while ((int64_t(1) << logM) < 2 * int64_t(bins) - 1) ++logM;
+  bp->logM = logM;
+  bp->M = 1 << logM;
+  bp->off = 0;
+  bp->width = width;
+  bp->sigma_f = sigma_f;
+  bp->dfreq = kSqrt2 * kPi * sigma_t;
+  bp->bin_thr = 0;
+  bp->sv_thr = 0;
+  return true;
+}
+
+// Inverse modulo n by extended Euclid; 0 when a is not a unit.
+int64_t InvMod(int64_t a, int64_t n) {
+  int64_t r0 = n, r1 = PosMod(a, n);
+  int64_t s0 = 0, s1 = 1;
+  while (r1 != 0) {
+    const int64_t qq = r0 / r1;
+    int64_t t = r0 - qq * r1;
+    r0 = r1;
+    r1 = t;
+    t = s0 - qq * s1;
+    s0 = s1;
+    s1 = t;
+  }
+  if (r0 != 1) return 0;
+  return PosMod(s0, n);
+}
+
+DevTf ReduceTf(int64_t n, int64_t a, int64_t b, int64_t c, int64_t a_inv) {
+  DevTf t;
+  t.a = PosMod(a, n);
+  t.b = PosMod(b, n);
+  t.c = PosMod(c, n);
+  t.a_inv = PosMod(a_inv, n);
+  return t;
+}
+
+Engine::Engine(int64_t n, int precision, int64_t capacity)
+    : n_(n), mask_(IsPow2(n) ? n - 1 : -1), precision_(precision) {
+  if (n < 2 || n > 0x7fffffffLL || (precision != MPSFT_FP64 && precision != MPSFT_MIXED) ||
+      capacity <= 0 || capacity > n || capacity > 0x7fffffffLL) {
+    status_ = MPSFT_ERR_INVALID_ARG;
+    return;
+  }
+  cap_ = int32_t(capacity);
+  log_cap_ = 1 << 16;
+  int rc = MPSFT_OK;
+  rc |= Alloc(&d_state_, 1);
+  rc |= Alloc(&d_tw_, size_t(1) << (kMaxLogM - 1));
+  rc |= Alloc(&d_keys_, cap_);
+  rc |= Alloc(&d_coef_, size_t(n_));
+  rc |= Alloc(&d_present_, size_t((n_ + 31) / 32));
+  rc |= Alloc(&d_log_, size_t(2) * log_cap_);
+  rc |= Alloc(&d_tmp_, cap_);
+  rc |= Alloc(&d_tmpk_, cap_);
+  if (rc != MPSFT_OK) {
+    status_ = MPSFT_ERR_NOMEM;
+    return;
+  }
+  if (hipHostMalloc(reinterpret_cast<void **>(&h_state_), sizeof(DevState)) != hipSuccess) {
+    status_ = MPSFT_ERR_NOMEM;
+    return;
+  }
+  std::memset(h_state_, 0, sizeof(DevState));
+  if (hipMemset(d_state_, 0, sizeof(DevState)) != hipSuccess ||
+      hipMemset(d_coef_, 0, size_t(n_) * sizeof(double2)) != hipSuccess ||
+      hipMemset(d_present_, 0, size_t((n_ + 31) / 32) * sizeof(uint32_t)) != hipSuccess ||
+      LaunchBuildTwiddles(d_tw_, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    status_ = MPSFT_ERR_HIP;
+    return;
+  }
+}
+
+Engine::~Engine() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  (void)hipFree(d_state_);
+  (void)hipFree(d_btab_);
+  (void)hipFree(d_table_);
+  (void)hipFree(d_tw_);
+  (void)hipFree(d_tfs_);
+  (void)hipFree(d_qs_);
+  (void)hipFree(d_fold_);
+  (void)hipFree(d_A_);
+  (void)hipFree(d_terms_);
+  (void)hipFree(d_keys_);
+  (void)hipFree(d_coef_);
+  (void)hipFree(d_present_);
+  (void)hipFree(d_log_);
+  (void)hipFree(d_tmp_);
+  (void)hipFree(d_tmpk_);
+  if (h_state_) (void)hipHostFree(h_state_);
+}
+
+// Builds the window parameters, chirps and Bluestein filters of every B in
+// bs (for window delta) that is not built yet.
+int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t bits_override) {
+  if (delta != tab_delta_) {
+    hbtab_.clear();
+    built_.clear();
+    table_used_ = 0;
+    tab_delta_ = delta;
+    tab_sigma_ = -1;
+  }
+  int32_t Bmax = 0;
+  for (int32_t B : bs) {
+    if (B <= 0 || B % 2 != 1) return MPSFT_ERR_INVALID_ARG;
+    if (B > kMaxBins) return MPSFT_ERR_UNSUPPORTED;
+    Bmax = std::max(Bmax, B);
+  }
+  if (int32_t(hbtab_.size()) <= Bmax) {
+    hbtab_.resize(Bmax + 1);
+    built_.resize(Bmax + 1, 0);
+  }
+  std::vector<int32_t> todo;
+  int64_t need = 0;
+  int32_t Mmax = 0;
+  for (int32_t B : bs) {
+    if (built_[B]) continue;
+    BParams bp;
+    if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
+    if (bp.logM > kMaxLogM) return MPSFT_ERR_UNSUPPORTED;
+    bp.off = table_used_ + need;
+    need += B + bp.M;
+    hbtab_[B] = bp;
+    built_[B] = 1;
+    todo.push_back(B);
+    Mmax = std::max(Mmax, bp.M);
+  }
+  if (bits_override != tab_bits_override_) tab_bits_override_ = bits_override;
+  if (table_used_ + need > table_len_) {
+    const int64_t len = std::max<int64_t>(table_used_ + need, 2 * table_len_);
+    double2 *nt = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&nt), size_t(len) * sizeof(double2)) != hipSuccess)
+      return MPSFT_ERR_NOMEM;
+    if (d_table_) {
+      HIP_OK(hipMemcpy(nt, d_table_, size_t(table_used_) * sizeof(double2),
+                       hipMemcpyDeviceToDevice));
+      (void)hipFree(d_table_);
+    }
+    d_table_ = nt;
+    table_len_ = len;
+    ++shape_gen_;
+  }
+  table_used_ += need;
+  if (d_btab_len_ < int32_t(hbtab_.size())) {
+    if (Alloc(&d_btab_, hbtab_.size()) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+    d_btab_len_ = int32_t(hbtab_.size());
+    ++shape_gen_;
+    tab_sigma_ = -1;
+  }
+  HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
+                   hipMemcpyHostToDevice));
+  if (!todo.empty()) {
+    int32_t *d_list = nullptr;
+    if (Alloc(&d_list, todo.size()) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+    HIP_OK(hipMemcpy(d_list, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_OK(LaunchBuildTables(d_list, int32_t(todo.size()), d_btab_, d_table_, d_tw_, Mmax,
+                             nullptr));
+    HIP_OK(hipDeviceSynchronize());
+    (void)hipFree(d_list);
+  }
+  return MPSFT_OK;
+}
+
+// Demo1 thresholds per B: 3 sigma / sqrt(B) and sigma / sqrt(B)
+// (src/demo1.cc:50-51).
+int Engine::SetThresholds(double sigma) {
+  if (sigma == tab_sigma_) return MPSFT_OK;
+  for (size_t B = 0; B < hbtab_.size(); ++B) {
+    if (!built_[B]) continue;
+    hbtab_[B].bin_thr = 3.0 * sigma / std::sqrt(double(B));
+    hbtab_[B].sv_thr = 1.0 * sigma / std::sqrt(double(B));
+  }
+  HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
+                   hipMemcpyHostToDevice));
+  tab_sigma_ = sigma;
+  return MPSFT_OK;
+}
+
+// Sizes the per-iteration buffers for trials and every B in bs.
+int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
+  int32_t Bc = 0, rc = 0, lc = 0, fc = 0, tc = 0, Mc = 0;
+  for (int32_t B : bs) {
+    const BParams &bp = hbtab_[B];
+    const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
+    int32_t tiles, len0;
+    TileShape(B, bits, bp.f, &tiles, &len0);
+    Bc = std::max(Bc, B);
+    rc = std::max(rc, 1 + 2 * bits);
+    lc = std::max(lc, len0);
+    fc = std::max(fc, bp.f);
+    tc = std::max(tc, tiles);
+    Mc = std::max(Mc, bp.M);
+  }
+  const bool grow_rows = trials > trials_cap_ || rc > rows_cap_ || Bc > B_cap_;
+  if (grow_rows) {
+    trials_cap_ = std::max(trials_cap_, trials);
+    rows_cap_ = std::max(rows_cap_, rc);
+    B_cap_ = std::max(B_cap_, Bc);
+    const size_t sz = size_t(trials_cap_) * rows_cap_ * B_cap_;
+    if (Alloc(&d_fold_, sz) != MPSFT_OK || Alloc(&d_A_, sz) != MPSFT_OK ||
+        Alloc(&d_terms_, size_t(trials_cap_) * cap_) != MPSFT_OK)
+      return MPSFT_ERR_NOMEM;
+    HIP_OK(hipMemset(d_fold_, 0, sz * sizeof(double2)));
+    HIP_OK(hipMemset(d_A_, 0, sz * sizeof(double2)));
+    tf_valid_ = false;  // q stride follows trials_cap_
+    ++shape_gen_;
+  }
+  if (lc > len_cap_ || fc > f_cap_ || tc > tiles_cap_ || Mc > M_cap_) {
+    len_cap_ = std::max(len_cap_, lc);
+    f_cap_ = std::max(f_cap_, fc);
+    tiles_cap_ = std::max(tiles_cap_, tc);
+    M_cap_ = std::max(M_cap_, Mc);
+    ++shape_gen_;
+  }
+  return MPSFT_OK;
+}
+
+// Sizes the transform / q arrays for count iterations; *grown is set when
+// the arrays were reallocated (their content is then gone).
+int Engine::GrowTransforms(int32_t count, bool *grown) {
+  *grown = false;
+  if (count <= tf_cap_ && tf_qstride_ == trials_cap_) return MPSFT_OK;
+  const int32_t cap = std::max(count, std::max(256, 2 * tf_cap_));
+  if (Alloc(&d_tfs_, cap) != MPSFT_OK || Alloc(&d_qs_, size_t(cap) * trials_cap_) != MPSFT_OK)
+    return MPSFT_ERR_NOMEM;
+  tf_cap_ = cap;
+  tf_qstride_ = trials_cap_;
+  tf_valid_ = false;
+  *grown = true;
+  ++shape_gen_;
+  return MPSFT_OK;
+}
+
+// Draws the transforms and q's of iterations [0, count) for seed, in the
+// reference's order: Transform(n) draws a, b, c (src/integer.cc:44-49),
+// then Iterate draws RandomInt32() % n per trial (src/iterate.cc:103).
+// For n = 2^k the multiplier is drawn odd, a = 2 (r mod n/2) + 1; for other
+// composite n the draw of a repeats until gcd(a, n) = 1.
+int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s) {
+  bool grown = false;
+  const int rc = GrowTransforms(count, &grown);
+  if (rc != MPSFT_OK) return rc;
+  const bool fresh = !tf_valid_ || seed != tf_seed_ || trials != tf_trials_;
+  if (fresh) {
+    rng_.seed(seed);
+    h_tfs_.clear();
+    h_qs_.clear();
+    tf_count_ = 0;
+    tf_seed_ = seed;
+    tf_trials_ = trials;
+  }
+  if (!fresh && count <= tf_count_) return MPSFT_OK;
+  const int32_t first = fresh ? 0 : tf_count_;
+  const int32_t target = std::max(count, tf_count_);
+  const int64_t n = n_;
+  while (int32_t(h_tfs_.size()) < target) {
+    int64_t a;
+    if (IsPow2(n)) {
+      a = 2 * (RandomInt32() % (n / 2)) + 1;
+    } else {
+      a = (RandomInt32() % (n - 1)) + 1;
+      while (Gcd(a, n) != 1) a = (RandomInt32() % (n - 1)) + 1;
+    }
+    const int64_t b = RandomInt32() % n;
+    const int64_t c = RandomInt32() % n;
+    h_tfs_.push_back(ReduceTf(n, a, b, c, InvMod(a, n)));
+    for (int32_t t = 0; t < trials_cap_; ++t)
+      h_qs_.push_back(t < trials ? int64_t(RandomInt32() % n) : 0);
+  }
+  const int32_t from = first;
+  HIP_OK(hipMemcpyAsync(d_tfs_ + from, h_tfs_.data() + from,
+                        size_t(target - from) * sizeof(DevTf), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_qs_ + size_t(from) * trials_cap_, h_qs_.data() + size_t(from) * trials_cap_,
+                        size_t(target - from) * trials_cap_ * sizeof(int64_t),
+                        hipMemcpyHostToDevice, s));
+  HIP_OK(hipStreamSynchronize(s));
+  tf_count_ = target;
+  tf_valid_ = true;
+  return MPSFT_OK;
+}
+
+Ctx Engine::MakeCtx() const {
+  Ctx c;
+  c.st = d_state_;
+  c.btab = d_btab_;
+  c.table = d_table_;
+  c.tw = d_tw_;
+  c.tfs = d_tfs_;
+  c.qs = d_qs_;
+  c.n = n_;
+  c.mask = mask_;
+  c.fold = d_fold_;
+  c.A = A_override_ ? A_override_ : d_A_;
+  c.terms = d_terms_;
+  c.keys = d_keys_;
+  c.coef = d_coef_;
+  c.present = d_present_;
+  c.log = d_log_;
+  c.log_cap = log_cap_;
+  c.cap = cap_;
+  c.trials_cap = trials_cap_;
+  c.rows_cap = rows_cap_;
+  c.B_cap = B_cap_;
+  c.len_cap = len_cap_;
+  c.f_cap = f_cap_;
+  c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
+  return c;
+}
+
+size_t Engine::LdsBinTime() const {
+  return size_t(len_cap_ + f_cap_) * kResBlock * sizeof(double2) + kThreads * sizeof(double2);
+}
+size_t Engine::LdsFft() const { return size_t(M_cap_) * sizeof(double2); }
+int32_t Engine::TilesCap() const { return trials_cap_ * tiles_cap_; }
+int32_t Engine::IdentifyBlocks() const {
+  return std::max<int32_t>(1, (B_cap_ + kThreads - 1) / kThreads);
+}
+
+int Engine::LaunchIteration(const Ctx &c, hipStream_t s) {
+  HIP_OK(LaunchBinTime(c, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinFft(c, LdsFft(), s));
+  HIP_OK(LaunchIdentify(c, IdentifyBlocks(), s));
+  return MPSFT_OK;
+}
+
+int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
+  if (graph_exec_ && graph_key_ == shape_gen_ && graph_iters_ == iters) return MPSFT_OK;
+  if (graph_exec_) {
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+  hipStream_t cs;
+  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  const Ctx c = MakeCtx();
+  hipGraph_t g;
+  HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = MPSFT_OK;
+  for (int32_t i = 0; i < iters && rc == MPSFT_OK; ++i) rc = LaunchIteration(c, cs);
+  const hipError_t ec = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc != MPSFT_OK || ec != hipSuccess) return MPSFT_ERR_HIP;
+  const hipError_t ei = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    graph_exec_ = nullptr;
+    return MPSFT_ERR_HIP;
+  }
+  graph_iters_ = iters;
+  graph_key_ = shape_gen_;
+  (void)s;
+  return MPSFT_OK;
+}
+
+int Engine::ReadState(hipStream_t s) {
+  HIP_OK(hipMemcpyAsync(h_state_, d_state_, sizeof(DevState), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return MPSFT_OK;
+}
+
+int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
+                  int32_t *iters, hipStream_t s, double *kernel_ms) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!x || opt.trials <= 0 || opt.trials % 2 != 1 || !(opt.window_threshold > 0) ||
+      opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 || opt.max_stale_iter <= 0 ||
+      !(opt.window_delta > 0) || !(opt.window_delta < 1) || opt.sigma < 0)
+    return MPSFT_ERR_INVALID_ARG;
+  const int64_t want0 = 2 * int64_t(opt.k) + 1;
+  const int32_t B0 = int32_t(std::max<int64_t>(want0, opt.min_bins));
+  if (B0 > kMaxBins) return MPSFT_ERR_UNSUPPORTED;
+  std::vector<int32_t> bs;
+  for (int32_t B = opt.min_bins; B <= B0; B += 2) bs.push_back(B);
+  A_override_ = nullptr;
+  int rc = EnsureTables(bs, opt.window_delta, 0);
+  if (rc != MPSFT_OK) return rc;
+  rc = SetThresholds(opt.sigma);
+  if (rc != MPSFT_OK) return rc;
+  rc = Reserve(opt.trials, bs);
+  if (rc != MPSFT_OK) return rc;
+  const int32_t max_iter = opt.max_iter > 0 ? opt.max_iter : 100000;
+  const int32_t chunk = ChunkIters();
+  rc = EnsureTransforms(seed, std::min(max_iter, std::max(256, chunk)), opt.trials, s);
+  if (rc != MPSFT_OK) return rc;
+
+  Ctx c = MakeCtx();
+  HIP_OK(LaunchClearModes(c, 0, s));
+  InitArgs a{};
+  a.x = x;
+  a.mode = kModeSolve;
+  a.B = B0;
+  a.trials = opt.trials;
+  a.k = opt.k;
+  a.min_bins = opt.min_bins;
+  a.max_stale = opt.max_stale_iter;
+  a.max_iter = max_iter;
+  a.bits_override = 0;
+  a.use_table_thr = 1;
+  a.win_thr = opt.window_threshold;
+  HIP_OK(LaunchInit(c, a, s));
+
+  int32_t launched = 0;
+  if (kernel_ms) {
+    // Instrumented: one iteration at a time, each kernel bracketed by
+    // events on the stream it runs on.
+    hipEvent_t ev[4];
+    for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    for (int i = 0; i < 3; ++i) kernel_ms[i] = 0.0;
+    int out = MPSFT_OK;
+    for (; launched < max_iter; ++launched) {
+      if ((out = EnsureTransforms(seed, launched + 1, opt.trials, s)) != MPSFT_OK) break;
+      c = MakeCtx();
+      if (hipEventRecord(ev[0], s) != hipSuccess ||
+          LaunchBinTime(c, TilesCap(), LdsBinTime(), s) != hipSuccess ||
+          hipEventRecord(ev[1], s) != hipSuccess || LaunchBinFft(c, LdsFft(), s) != hipSuccess ||
+          hipEventRecord(ev[2], s) != hipSuccess ||
+          LaunchIdentify(c, IdentifyBlocks(), s) != hipSuccess ||
+          hipEventRecord(ev[3], s) != hipSuccess) {
+        out = MPSFT_ERR_HIP;
+        break;
+      }
+      if ((out = ReadState(s)) != MPSFT_OK) break;
+      for (int i = 0; i < 3; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) out = MPSFT_ERR_HIP;
+        kernel_ms[i] += ms;
+      }
+      if (out != MPSFT_OK || h_state_->done) break;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (out != MPSFT_OK) return out;
+  } else {
+    for (;;) {
+      const int32_t need = std::min(max_iter, launched + chunk);
+      rc = EnsureTransforms(seed, need, opt.trials, s);
+      if (rc != MPSFT_OK) return rc;
+      rc = EnsureGraph(chunk, s);
+      if (rc != MPSFT_OK) return rc;
+      HIP_OK(hipGraphLaunch(graph_exec_, s));
+      launched += chunk;
+      rc = ReadState(s);
+      if (rc != MPSFT_OK) return rc;
+      if (h_state_->done) break;
+      if (launched >= max_iter) break;
+    }
+  }
+  *iters = h_state_->it;
+  if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
+  if (!h_state_->done) return MPSFT_ERR_NOT_CONVERGED;
+  if (h_state_->stale < opt.max_stale_iter) return MPSFT_ERR_NOT_CONVERGED;
+  return MPSFT_OK;
+}
+
+int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const DevTf &tf,
+                        const int64_t *qs, int32_t *found, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!x || !qs || opt.trials <= 0 || opt.trials % 2 != 1 || !(opt.window_threshold > 0) ||
+      opt.bins <= 0 || opt.bins % 2 != 1 || !(opt.window_delta > 0))
+    return MPSFT_ERR_INVALID_ARG;
+  const int want_type = precision_ == MPSFT_FP64 ? 5 : 6;
+  if (opt.bin_in_time_type != want_type || opt.bin_in_freq_type != want_type)
+    return MPSFT_ERR_UNSUPPORTED;
+  if (opt.bins > kMaxBins) return MPSFT_ERR_UNSUPPORTED;
+  A_override_ = nullptr;
+  std::vector<int32_t> bs{opt.bins};
+  int rc = EnsureTables(bs, opt.window_delta, 0);
+  if (rc != MPSFT_OK) return rc;
+  rc = Reserve(opt.trials, bs);
+  if (rc != MPSFT_OK) return rc;
+  bool grown = false;
+  rc = GrowTransforms(1, &grown);
+  if (rc != MPSFT_OK) return rc;
+  tf_valid_ = false;
+  std::vector<int64_t> hq(trials_cap_, 0);
+  for (int32_t t = 0; t < opt.trials; ++t) hq[t] = PosMod(qs[t], n_);
+  HIP_OK(hipMemcpyAsync(d_tfs_, &tf, sizeof(DevTf), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_qs_, hq.data(), hq.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  Ctx c = MakeCtx();
+  InitArgs a{};
+  a.x = x;
+  a.mode = kModeIterate;
+  a.B = opt.bins;
+  a.trials = opt.trials;
+  a.max_stale = 1;
+  a.max_iter = 1;
+  a.keep_modes = 1;
+  a.bin_thr = opt.bin_threshold;
+  a.sv_thr = opt.sv_threshold;
+  a.win_thr = opt.window_threshold;
+  HIP_OK(LaunchInit(c, a, s));
+  rc = LaunchIteration(c, s);
+  if (rc != MPSFT_OK) return rc;
+  rc = ReadState(s);
+  if (rc != MPSFT_OK) return rc;
+  *found = h_state_->found_last;
+  if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
+  return MPSFT_OK;
+}
+
+int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const DevTf &tf,
+                    int64_t q, double2 *out, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!x || !out || bits < 1 || bits > 30) return MPSFT_ERR_INVALID_ARG;
+  std::vector<int32_t> bs{B};
+  int rc = EnsureTables(bs, delta, bits);
+  if (rc != MPSFT_OK) return rc;
+  rc = Reserve(1, bs);
+  if (rc != MPSFT_OK) return rc;
+  if (B_cap_ != B || rows_cap_ != 1 + 2 * bits || trials_cap_ != 1) return MPSFT_ERR_SHAPE;
+  bool grown = false;
+  rc = GrowTransforms(1, &grown);
+  if (rc != MPSFT_OK) return rc;
+  tf_valid_ = false;
+  const int64_t hq = PosMod(q, n_);
+  HIP_OK(hipMemcpyAsync(d_tfs_, &tf, sizeof(DevTf), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_qs_, &hq, sizeof(int64_t), hipMemcpyHostToDevice, s));
+  A_override_ = out;
+  Ctx c = MakeCtx();
+  InitArgs a{};
+  a.x = x;
+  a.mode = kModeBinOnly;
+  a.B = B;
+  a.trials = 1;
+  a.max_stale = 1;
+  a.max_iter = 1;
+  a.bits_override = bits;
+  a.win_thr = 1;
+  HIP_OK(LaunchInit(c, a, s));
+  HIP_OK(LaunchBinTime(c, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinFft(c, LdsFft(), s));
+  A_override_ = nullptr;
+  return MPSFT_OK;
+}
+
+int Engine::ModesCount(int64_t *count) {
+  const int rc = ReadState(nullptr);
+  if (rc != MPSFT_OK) return rc;
+  *count = std::min<int64_t>(h_state_->count, cap_);
+  return MPSFT_OK;
+}
+
+int Engine::ModesGet(int64_t *keys, double *coefs, int64_t cap, int64_t *count) {
+  int64_t m = 0;
+  int rc = ModesCount(&m);
+  if (rc != MPSFT_OK) return rc;
+  *count = m;
+  if (cap < m) return MPSFT_ERR_SHAPE;
+  if (m == 0) return MPSFT_OK;
+  Ctx c = MakeCtx();
+  HIP_OK(LaunchGetCoefs(c, d_tmp_, m, nullptr));
+  HIP_OK(hipMemcpy(keys, d_keys_, size_t(m) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(coefs, d_tmp_, size_t(m) * sizeof(double2), hipMemcpyDeviceToHost));
+  return MPSFT_OK;
+}
+
+int Engine::ModesSet(const int64_t *keys, const double *coefs, int64_t m) {
+  if (m < 0 || m > cap_ || (m > 0 && (!keys || !coefs))) return MPSFT_ERR_INVALID_ARG;
+  std::vector<int64_t> hk(keys, keys + m);
+  for (auto &k : hk) k = PosMod(k, n_);
+  Ctx c = MakeCtx();
+  HIP_OK(LaunchClearModes(c, 0, nullptr));
+  if (m > 0) {
+    HIP_OK(hipMemcpy(d_tmpk_, hk.data(), size_t(m) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_tmp_, coefs, size_t(m) * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  HIP_OK(LaunchSetModes(c, d_tmpk_, d_tmp_, m, nullptr));
+  HIP_OK(hipDeviceSynchronize());
+  return MPSFT_OK;
+}
+
+int Engine::BinsGet(double *out, int64_t cap, int32_t *trials, int32_t *rows, int32_t *bins) {
+  int rc = ReadState(nullptr);
+  if (rc != MPSFT_OK) return rc;
+  const int32_t T = h_state_->trials, R = h_state_->rows, B = h_state_->B;
+  *trials = T;
+  *rows = R;
+  *bins = B;
+  if (!d_A_ || T > trials_cap_ || R > rows_cap_ || B > B_cap_) return MPSFT_ERR_SHAPE;
+  if (cap < int64_t(2) * T * R * B) return MPSFT_ERR_SHAPE;
+  for (int32_t t = 0; t < T; ++t)
+    HIP_OK(hipMemcpy2D(out + size_t(2) * t * R * B, size_t(B) * sizeof(double2),
+                       d_A_ + size_t(t) * rows_cap_ * B_cap_, size_t(B_cap_) * sizeof(double2),
+                       size_t(B) * sizeof(double2), R, hipMemcpyDeviceToHost));
+  return MPSFT_OK;
+}
+
+int Engine::LogGet(int32_t *bins, int32_t *found, int32_t cap, int32_t *n_iter) {
+  int rc = ReadState(nullptr);
+  if (rc != MPSFT_OK) return rc;
+  const int32_t it = std::min(h_state_->it, log_cap_);
+  *n_iter = it;
+  const int32_t m = std::min(it, cap);
+  if (m > 0) {
+    if (bins) HIP_OK(hipMemcpy(bins, d_log_, size_t(m) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (found)
+      HIP_OK(hipMemcpy(found, d_log_ + log_cap_, size_t(m) * sizeof(int32_t),
+                       hipMemcpyDeviceToHost));
+  }
+  return MPSFT_OK;
+}
+
+}  // namespace mpsft
+
+namespace mpsft {
+
+int Engine::TransformsGet(DevTf *tfs, int64_t *qs, int32_t cap, int32_t trials, int32_t *count) {
+  int rc = ReadState(nullptr);
+  if (rc != MPSFT_OK) return rc;
+  if (!tf_valid_ || trials != tf_trials_) {
+    *count = 0;
+    return MPSFT_OK;
+  }
+  const int32_t m = std::min(std::min(h_state_->it, tf_count_), cap);
+  for (int32_t i = 0; i < m; ++i) {
+    tfs[i] = h_tfs_[i];
+    for (int32_t t = 0; t < trials; ++t) qs[int64_t(i) * trials + t] = h_qs_[size_t(i) * trials_cap_ + t];
+  }
+  *count = m;
+  return MPSFT_OK;
+}
+
+}  // namespace mpsft

@@ -1,0 +1,225 @@
+// engine.h -- device data layout and host engine of the MPSFT recovery loop.
+//
+// One Engine holds the device state of one signal length n.  An iteration
+// of Iterate (src/iterate.cc:80-177) is three kernels on one stream:
+//   k_bin_time  : permuted, windowed gather + fold (BinInTime without its
+//                 B-point DFT), plus the per-mode terms BinInFreq needs;
+//   k_bin_fft   : per (trial, row), the B-point forward DFT (Bluestein over
+//                 an LDS-resident power-of-two FFT), the row phase, and the
+//                 BinInFreq subtraction;
+//   k_identify  : per bin, the energy test, the matrix-pencil votes
+//                 (IdentifyFreq), the coefficient estimate and the insert
+//                 into the found-mode map; the last workgroup advances the
+//                 loop state (Demo1::Run, src/demo1.cc:36-59).
+// Every kernel reads the iteration's parameters (B, bits, transform, q)
+// from DevState in device memory, so one fixed launch sequence is captured
+// in a hipGraph and replayed for any iteration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "../../include/mpsft.h"
+
+namespace mpsft {
+
+constexpr int kThreads = 256;     // workgroup size of every kernel
+constexpr int kResBlock = 32;     // residues per k_bin_time tile
+constexpr int kMaxLogM = 13;      // Bluestein FFT length limit 8192 => B <= 4096
+constexpr int kPrepBlocks = 4;    // k_bin_time workgroups computing mode terms
+constexpr int kMaxBins = 4096;
+
+// Per-B window and bin-FFT parameters, computed on the host with the
+// reference's formulas (src/window.cc:28-50, src/iterate.cc:30-42).
+struct BParams {
+  int32_t B, P, f, bits, M, logM;
+  int64_t off;      // offset (in double2) of [chirp(B) | filter(M)] in table
+  double width;     // 1/(2B)
+  double sigma_f;   // window std-dev in frequency
+  double dfreq;     // sqrt(2)*pi*sigma_t  (SampleInFreq)
+  double bin_thr;   // 3*sigma/sqrt(B) in a solve
+  double sv_thr;    // sigma/sqrt(B) in a solve
+};
+
+// Transform reduced modulo n.
+struct DevTf {
+  int64_t a, b, c, a_inv;
+};
+
+enum : int32_t { kModeSolve = 0, kModeIterate = 1, kModeBinOnly = 2 };
+enum : int32_t { kFlagOverflow = 1 };
+
+// Loop state, read at the start of every kernel of an iteration.
+struct DevState {
+  const void *x;  // input signal (double2 or float2)
+  int32_t B, bits, rows, f, P, p2, M, logM;
+  int64_t tab_off;
+  double width, sigma_f, dfreq;
+  double bin_thr, sv_thr, win_thr;
+  int32_t trials;
+  int32_t it;          // iteration index into the transform table
+  int32_t stale;
+  int32_t done;
+  int32_t count;       // number of distinct found keys
+  int32_t found_iter;  // set when this iteration found anything
+  uint32_t blocks_done;
+  int32_t flags;
+  int32_t mode;
+  int32_t k, min_bins, max_stale, max_iter;
+  int32_t found_last;  // found_iter of the last completed iteration
+};
+
+// Per found mode and trial: what BinInFreq subtracts from bin `bin` of row
+// 0; rows 2b+1 / 2b+2 take base*g_b / base*conj(g_b).
+struct ModeTerm {
+  int64_t k1;
+  int32_t bin;
+  int32_t pad;
+  double2 base;
+};
+
+// Everything a kernel needs; passed by value.
+struct Ctx {
+  DevState *st;
+  const BParams *btab;   // indexed by B
+  const double2 *table;  // chirps and Bluestein filters
+  const double2 *tw;     // e^{-2 pi i k / 2^kMaxLogM}, k < 2^(kMaxLogM-1)
+  const DevTf *tfs;      // per iteration
+  const int64_t *qs;     // per iteration x trials_cap
+  int64_t n;
+  int64_t mask;          // n-1 if n is a power of two, else -1
+  double2 *fold;         // trials_cap x rows_cap x B_cap
+  double2 *A;            // trials_cap x rows_cap x B_cap (bin coefficients)
+  ModeTerm *terms;       // trials_cap x cap
+  int64_t *keys;         // cap
+  double2 *coef;         // n (dense found coefficients)
+  uint32_t *present;     // ceil(n/32) bits
+  int32_t *log;          // 2 x log_cap
+  int32_t log_cap;
+  int32_t cap;
+  int32_t trials_cap, rows_cap, B_cap;
+  int32_t len_cap, f_cap;  // k_bin_time LDS layout
+  int32_t mixed;
+};
+
+class Engine {
+ public:
+  Engine(int64_t n, int precision, int64_t capacity);
+  ~Engine();
+  int status() const { return status_; }
+
+  // kernel_ms (3 entries, or null): when given, the solve runs without the
+  // graph and accumulates the time of k_bin_time, k_bin_fft, k_identify.
+  int Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
+            int32_t *iters, hipStream_t s, double *kernel_ms = nullptr);
+  int IterateOnce(const void *x, const mpsft_iterate_options &opt,
+                  const DevTf &tf, const int64_t *qs, int32_t *found,
+                  hipStream_t s);
+  // BinInTime (bin_only) for a plan: out gets rows x B, row stride B.
+  int BinOnly(const void *x, int32_t B, double delta, int32_t bits,
+              const DevTf &tf, int64_t q, double2 *out, hipStream_t s);
+
+  int ModesCount(int64_t *count);
+  int ModesGet(int64_t *keys, double *coefs, int64_t cap, int64_t *count);
+  int ModesSet(const int64_t *keys, const double *coefs, int64_t m);
+  int BinsGet(double *out, int64_t cap, int32_t *trials, int32_t *rows,
+              int32_t *bins);
+  int LogGet(int32_t *bins, int32_t *found, int32_t cap, int32_t *n_iter);
+  int TransformsGet(DevTf *tfs, int64_t *qs, int32_t cap, int32_t trials, int32_t *count);
+
+  int64_t n() const { return n_; }
+  int precision() const { return precision_; }
+
+ private:
+  int EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t bits_override);
+  int SetThresholds(double sigma);
+  int Reserve(int32_t trials, const std::vector<int32_t> &bs);
+  int EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s);
+  int GrowTransforms(int32_t count, bool *grown);
+  Ctx MakeCtx() const;
+  int LaunchIteration(const Ctx &c, hipStream_t s);
+  int EnsureGraph(int32_t iters, hipStream_t s);
+  int ReadState(hipStream_t s);
+  size_t LdsBinTime() const;
+  size_t LdsFft() const;
+  int32_t TilesCap() const;
+  int32_t IdentifyBlocks() const;
+
+  int status_ = MPSFT_OK;
+  int64_t n_;
+  int64_t mask_;
+  int precision_;
+  int32_t cap_;
+
+  // host copy of the B table (index = B); built_[B] set when its chirp and
+  // filter exist on the device.  bits_[B] may be overridden for a plan.
+  std::vector<BParams> hbtab_;
+  std::vector<char> built_;
+  double tab_delta_ = -1;
+  double tab_sigma_ = -1;
+  int32_t tab_bits_override_ = 0;
+
+  int32_t trials_cap_ = 0, rows_cap_ = 0, B_cap_ = 0, len_cap_ = 0, f_cap_ = 0;
+  int32_t tiles_cap_ = 0;  // k_bin_time tiles per trial (max over B)
+  int32_t M_cap_ = 0;
+
+  // device memory
+  DevState *d_state_ = nullptr;
+  BParams *d_btab_ = nullptr;
+  int32_t d_btab_len_ = 0;
+  double2 *d_table_ = nullptr;
+  int64_t table_len_ = 0, table_used_ = 0;
+  double2 *d_tw_ = nullptr;
+  DevTf *d_tfs_ = nullptr;
+  int64_t *d_qs_ = nullptr;
+  int32_t tf_cap_ = 0;      // iterations the device arrays can hold
+  int32_t tf_count_ = 0;    // iterations currently drawn for tf_seed_
+  int32_t tf_trials_ = 0;
+  int32_t tf_qstride_ = 0;
+  bool tf_valid_ = false;
+  uint32_t tf_seed_ = 0;
+  double2 *d_fold_ = nullptr;
+  double2 *d_A_ = nullptr;
+  ModeTerm *d_terms_ = nullptr;
+  int64_t *d_keys_ = nullptr;
+  double2 *d_coef_ = nullptr;
+  uint32_t *d_present_ = nullptr;
+  int32_t *d_log_ = nullptr;
+  int32_t log_cap_ = 0;
+  DevState *h_state_ = nullptr;  // pinned mirror of d_state_
+  double2 *A_override_ = nullptr;
+  double2 *d_tmp_ = nullptr;     // cap double2 staging for mode I/O
+  int64_t *d_tmpk_ = nullptr;    // cap int64 staging for mode I/O
+
+  // host RNG of the transform draws (src/base.cc:32-36)
+  std::mt19937 rng_;
+  // RandomInt32 (src/base.cc:53): uniform_int_distribution<int32_t>() over
+  // mt19937 as libstdc++ computed it before GCC 11: 32-bit outputs below
+  // 2^31, the rest rejected.
+  int32_t RandomInt32() {
+    uint32_t r;
+    do {
+      r = uint32_t(rng_());
+    } while (r >= 0x80000000u);
+    return int32_t(r);
+  }
+  std::vector<DevTf> h_tfs_;
+  std::vector<int64_t> h_qs_;
+
+  // graph of graph_iters_ iterations for the current shapes
+  hipGraphExec_t graph_exec_ = nullptr;
+  int32_t graph_iters_ = 0;
+  int64_t graph_key_ = -1;
+  int64_t shape_gen_ = 0;
+};
+
+// Host-side helpers shared with the C-ABI.
+int32_t NumBits(int64_t n, int32_t bins);
+bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp);
+int64_t InvMod(int64_t a, int64_t n);
+DevTf ReduceTf(int64_t n, int64_t a, int64_t b, int64_t c, int64_t a_inv);
+
+}  // namespace mpsft

@@ -1,0 +1,40 @@
+// kernels.h -- launch wrappers of the HIP kernels (kernels.hip).
+#pragma once
+
+#include "engine.h"
+
+namespace mpsft {
+
+// Initial loop state for k_init.
+struct InitArgs {
+  const void *x;
+  int32_t mode;
+  int32_t B;          // first iteration's bin count
+  int32_t trials;
+  int32_t k, min_bins, max_stale, max_iter;
+  int32_t bits_override;  // > 0: use this bits instead of the table's
+  int32_t use_table_thr;  // 1: thresholds from the B table (solve)
+  int32_t keep_modes;     // 1: keep the found-mode map (Iterate)
+  double bin_thr, sv_thr, win_thr;
+};
+
+hipError_t LaunchBuildTwiddles(double2 *tw, hipStream_t s);
+hipError_t LaunchBuildTables(const int32_t *blist, int32_t nb, const BParams *btab,
+                             double2 *table, const double2 *tw, int32_t M_cap,
+                             hipStream_t s);
+hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s);
+hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s);
+hipError_t LaunchBinTime(const Ctx &c, int32_t tiles, size_t lds, hipStream_t s);
+hipError_t LaunchBinFft(const Ctx &c, size_t lds, hipStream_t s);
+hipError_t LaunchIdentify(const Ctx &c, int32_t blocks, hipStream_t s);
+hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
+                          int64_t m, hipStream_t s);
+hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);
+hipError_t LaunchBinInFreqDirect(int64_t n, int64_t mask, const BParams &bp,
+                                 const DevTf &tf, int64_t q, const int64_t *k0,
+                                 const double2 *coef, int64_t m, double2 *out,
+                                 hipStream_t s);
+hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma,
+                           uint8_t *vote, hipStream_t s);
+
+}  // namespace mpsft

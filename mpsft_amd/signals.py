@@ -1,0 +1,57 @@
+"""Synthetic test signals (host, numpy), after the reference's gen.cc.
+
+The reference builds x as x-hat (sparse modes plus complex Gaussian noise of
+per-component std sigma/sqrt(2n), GenerateXhatAlt, src/gen.cc:78-89) followed
+by an unnormalised backward DFT (src/demo1.cc:30-33), so each time sample
+carries noise of E|e|^2 = sigma^2.  These generators keep that construction
+and add the mode layouts of the benchmark workloads: log-uniform
+magnitudes, clustered locations.  numpy's own random generator is used; the
+reference's mt19937 stream is only needed for its RNG-driven transforms,
+which the solver draws itself.
+"""
+import numpy as np
+
+
+def make_signal(n, k, sigma, seed, magnitude="unit", clusters=None, dtype=np.complex128):
+    """Returns (x, xhat_modes) with x a length-n signal of k modes.
+
+    magnitude: "unit" (|c| = 1) or "loguniform" (|c| = 10^U[-1, 1]).
+    clusters: None, or (n_centres, fraction, halfwidth): that fraction of
+    the modes lies within +-halfwidth bins of n_centres random centres.
+    The phases are uniform.  xhat_modes maps location -> coefficient.
+    """
+    rng = np.random.default_rng(seed)
+    if clusters is None:
+        locs = rng.choice(n, size=k, replace=False)
+    else:
+        n_c, frac, hw = clusters
+        k_c = int(round(k * frac))
+        centres = rng.choice(n, size=n_c, replace=False)
+        chosen = set()
+        while len(chosen) < k_c:
+            c = centres[rng.integers(n_c)]
+            chosen.add(int((c + rng.integers(-hw, hw + 1)) % n))
+        while len(chosen) < k:
+            chosen.add(int(rng.integers(n)))
+        locs = np.array(sorted(chosen), dtype=np.int64)
+        rng.shuffle(locs)
+    if magnitude == "unit":
+        mag = np.ones(k)
+    elif magnitude == "loguniform":
+        mag = 10.0 ** rng.uniform(-1.0, 1.0, size=k)
+    else:
+        raise ValueError(magnitude)
+    coef = mag * np.exp(2j * np.pi * rng.uniform(0.0, 1.0, size=k))
+    xh = np.zeros(n, np.complex128)
+    xh[locs] = coef
+    scale = sigma / np.sqrt(2.0 * n)
+    xh += (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * scale
+    x = np.fft.ifft(xh) * n  # unnormalised backward DFT
+    return x.astype(dtype), dict(zip(locs.tolist(), coef.tolist()))
+
+
+def quantization_sigma(x128, x64):
+    """RMS per-sample error of storing x in complex64: the noise the mixed
+    form sees on top of sigma."""
+    d = x64.astype(np.complex128) - x128
+    return float(np.sqrt(np.mean(np.abs(d) ** 2)))

@@ -1,0 +1,97 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/mpsft.h declares, and its host-only helpers and argument checks
+behave like the reference (restated by the oracle)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import mpsft_amd
+from mpsft_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        with open(os.path.join(ROOT, "include", fn)) as f:
+            src = f.read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(mpsft_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_header_symbol():
+    names = header_functions()
+    assert len(names) >= 29
+    L = C.CDLL(_lib.LIB_PATH)
+    for name in sorted(names):
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) == names
+
+
+def test_version_and_status_strings():
+    L = mpsft_amd.lib()
+    assert L.mpsft_version() == 1
+    for st, text in _lib.STATUS.items():
+        assert L.mpsft_status_string(st).decode() == text
+
+
+@pytest.mark.parametrize("n,bins", [(2053, 11), (1 << 22, 2001), (1 << 22, 101), (1048573, 2001),
+                                    (5, 11), (1 << 26, 8001), (1109, 1)])
+def test_num_bits_matches_oracle(orc, n, bins):
+    assert mpsft_amd.num_bits(n, bins) == orc.num_bits(n, bins)
+
+
+@pytest.mark.parametrize("n,bins,delta", [(1109, 5, 1e-6), (1 << 22, 2001, 1e-6),
+                                          (1 << 22, 101, 1e-5), (1 << 20, 1, 1e-6),
+                                          (1 << 26, 8001, 1e-6)])
+def test_window_p_matches_oracle(orc, n, bins, delta):
+    assert mpsft_amd.lib().mpsft_window_p(n, bins, delta) == orc.window_params(n, bins, delta)["p"]
+
+
+def test_transform_make_matches_oracle(orc, kats):
+    for case in ("time_freq_match", "time_freq_match_bigger"):
+        k = kats[case]
+        tf = mpsft_amd.transform_make(k["n"], *k["tf"])
+        ref = orc.transform_make(k["n"], *k["tf"])
+        assert (tf.a, tf.b, tf.c, tf.a_inv) == tuple(int(v) for v in ref)
+    tf = mpsft_amd.transform_make(1 << 22, 12345, 7, 9)
+    assert (tf.a * tf.a_inv) % (1 << 22) == 1
+
+
+def test_argument_errors_need_no_gpu():
+    L = mpsft_amd.lib()
+    tf = mpsft_amd.Transform()
+    # even a has no inverse modulo 2^k
+    assert L.mpsft_transform_make(1 << 20, 4, 1, 1, C.byref(tf)) == 1
+    assert L.mpsft_window_p(1109, 4, 1e-6) == 0           # even bins (src/window.cc:33)
+    assert L.mpsft_window_p(1109, 5, 0.0) == 0            # delta must be > 0
+    h = C.c_void_p()
+    assert L.mpsft_plan_create(1109, 4, 1e-6, 2, 0, C.byref(h)) == 1   # even bins
+    assert L.mpsft_plan_create(1109, 5, 1e-6, 0, 0, C.byref(h)) == 1   # bits < 1
+    assert L.mpsft_plan_create(1109, 5, 1e-6, 2, 9, C.byref(h)) == 3   # unknown precision
+    assert L.mpsft_plan_create(1 << 20, 4097, 1e-6, 2, 0, C.byref(h)) == 3  # B too big
+    opt = mpsft_amd.solve_options(10, 1e-6)
+    assert L.mpsft_solver_create(1 << 16, C.byref(opt), 9, 0, C.byref(h)) == 3
+    assert L.mpsft_solve(None, None, 0, None, None) == 1
+    assert L.mpsft_iterate(None, None, None, None, None, None, None) == 1
+
+
+def test_binner_factory_rejects_reference_cpu_types():
+    win_ok = True
+    try:
+        w = mpsft_amd.Window(1109, 5, 1e-6)
+    except Exception:
+        win_ok = False
+    assert win_ok and w.p() == 185
+    for t in (0, 1, 2, 3, 4, 7):
+        with pytest.raises(mpsft_amd.MpsftError):
+            mpsft_amd.BinInTime.create(t, w, 2)
+    with pytest.raises(mpsft_amd.MpsftError):
+        mpsft_amd.Window(1109, 4, 1e-6)

@@ -1,0 +1,319 @@
+"""Parity of the HIP path with the CPU oracle, through the C-ABI.
+
+Bars: FP64 bin matrices within 1e-12 of the oracle (relative to the row's
+largest magnitude); the recovered support (integer locations) identical;
+coefficients within 1e-12 relative for FP64 and 1e-6 relative for MIXED
+(which the oracle runs on the same complex64-rounded input).
+"""
+import numpy as np
+import pytest
+
+from mpsft_amd.signals import make_signal, quantization_sigma
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def rel_err(got, ref):
+    scale = np.abs(ref).max()
+    return np.abs(got - ref).max() / (scale if scale > 0 else 1.0)
+
+
+def assert_coefs_close(got, ref, tol, where=""):
+    """|got - ref| <= tol * max(|ref|, 1e-3 * largest |ref|).  A coefficient
+    is estimated from bins whose magnitude is set by the largest modes, so a
+    tiny residual mode carries rounding at that scale; every coefficient
+    above 1e-3 of the largest is held to tol relative to itself."""
+    assert set(got) == set(ref), where
+    if not ref:
+        return
+    floor = 1e-3 * max(abs(v) for v in ref.values())
+    for key, v in ref.items():
+        assert abs(got[key] - v) <= tol * max(abs(v), floor), (where, key, got[key], v)
+
+
+def _tf(gpu, orc_tf, n):
+    return gpu.transform_make(n, int(orc_tf[0]), int(orc_tf[1]), int(orc_tf[2]))
+
+
+def _modes(orc, n, modes):
+    mm = orc.ModeMap({int(k): complex(re, im) for k, re, im in modes})
+    return mm, orc.evaluate_modes(n, mm)
+
+
+# ---------------------------------------------------------------- BinInTime
+@pytest.mark.parametrize("case", ["time_freq_match", "time_freq_match_bigger", "bin_in_time_match"])
+def test_bin_in_time_reference_cases(gpu, orc, kats, case):
+    k = kats[case]
+    n, bins, bits, q = k["n"], k["bins"], k["bits"], k["q"]
+    mm, x = _modes(orc, n, k["modes"])
+    tf_o = orc.transform_make(n, *k["tf"])
+    ref = orc.bin_in_time(x, bins, k["delta"], bits, tf_o, q, version=0)
+    plan = gpu.Plan(n, bins, k["delta"], bits)
+    got = plan.bin_in_time(x, _tf(gpu, tf_o, n), q)
+    assert rel_err(got, ref) < FP64_TOL
+    if "row0_bin1" in k:
+        assert abs(got[0, 1] - complex(*k["row0_bin1"])) < 1.19e-5 * 2.2
+    # BinInFreq on the same modes cancels BinInTime (src/binner_test.cc:98-102)
+    keys, coefs = mm.items()
+    fr = np.zeros_like(got)
+    plan.bin_in_freq(keys.astype(np.int64), coefs, _tf(gpu, tf_o, n), q, fr)
+    ref_fr = np.zeros_like(ref)
+    orc.bin_in_freq(keys, coefs, n, bins, k["delta"], bits, tf_o, q, ref_fr, version=1)
+    assert rel_err(fr, ref_fr) < FP64_TOL
+    assert np.all(np.abs(got[0] + fr[0]) < 1.19e-5)
+
+
+@pytest.mark.parametrize("n,bins,delta", [(1 << 16, 101, 1e-6), (1 << 18, 2001, 1e-6),
+                                          (1 << 14, 4095, 1e-6), (1048573, 2001, 1e-6),
+                                          (1 << 20, 101, 1e-5), (1 << 12, 1, 1e-6),
+                                          (4099, 3, 1e-6)])
+def test_bin_in_time_random(gpu, orc, n, bins, delta):
+    rng = np.random.default_rng(n ^ bins)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    bits = orc.num_bits(n, bins)
+    orc.seed(bins)
+    tf_o = orc.transform_draw(n)
+    q = int(rng.integers(0, n))
+    ref = orc.bin_in_time(x, bins, delta, bits, tf_o, q, version=0)
+    got = gpu.Plan(n, bins, delta, bits).bin_in_time(x, _tf(gpu, tf_o, n), q)
+    for r in range(ref.shape[0]):
+        assert rel_err(got[r], ref[r]) < FP64_TOL, r
+
+
+def test_bin_in_time_mixed_input(gpu, orc):
+    n, bins, delta = 1 << 18, 2001, 1e-6
+    rng = np.random.default_rng(9)
+    x64 = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    bits = orc.num_bits(n, bins)
+    orc.seed(3)
+    tf_o = orc.transform_draw(n)
+    ref = orc.bin_in_time(x64.astype(np.complex128), bins, delta, bits, tf_o, 12345, version=0)
+    got = gpu.Plan(n, bins, delta, bits, gpu.MPSFT_MIXED).bin_in_time(x64, _tf(gpu, tf_o, n), 12345)
+    assert rel_err(got, ref) < MIXED_TOL
+
+
+def test_bin_in_freq_many_modes(gpu, orc):
+    n, bins, delta = 1 << 16, 101, 1e-6
+    bits = orc.num_bits(n, bins)
+    rng = np.random.default_rng(2)
+    keys = rng.choice(n, 500, replace=False).astype(np.int64)
+    coefs = rng.standard_normal(500) + 1j * rng.standard_normal(500)
+    orc.seed(8)
+    tf_o = orc.transform_draw(n)
+    base = rng.standard_normal((1 + 2 * bits, bins)) + 1j * rng.standard_normal((1 + 2 * bits, bins))
+    ref = base.copy()
+    orc.bin_in_freq(keys.astype(np.int32), coefs, n, bins, delta, bits, tf_o, 77, ref, version=1)
+    got = base.copy()
+    gpu.Plan(n, bins, delta, bits).bin_in_freq(keys, coefs, _tf(gpu, tf_o, n), 77, got)
+    assert rel_err(got, ref) < FP64_TOL
+
+
+# ---------------------------------------------------------------- MatPencil
+def test_mat_pencil_kats_and_random(gpu, orc, kats):
+    cases = kats["pencil_kats"]["cases"]
+    u0 = np.array([complex(*c["u0"]) for c in cases])
+    u1 = np.array([complex(*c["u1"]) for c in cases])
+    u2 = np.array([complex(*c["u2"]) for c in cases])
+    votes, sig = gpu.mat_pencil(u0, u1, u2)
+    for i, c in enumerate(cases):
+        assert votes[i] == c["vote"]
+        assert np.allclose(sig[i], c["sigma"], rtol=1e-8)
+    rng = np.random.default_rng(4)
+    m = 4096
+    u = (rng.standard_normal((3, m)) + 1j * rng.standard_normal((3, m))) * 10.0 ** rng.uniform(-8, 2, m)
+    votes, sig = gpu.mat_pencil(u[0], u[1], u[2])
+    for i in range(m):
+        v, s = orc.mat_pencil(u[0, i], u[1, i], u[2, i])
+        assert v == votes[i]
+        assert np.allclose(s, sig[i], rtol=1e-13, atol=0)
+
+
+# ---------------------------------------------------------------- Iterate
+def _iterate_both(gpu, orc, x, n, iopts, orc_opt, steps, seed, precision=0, x_orc=None):
+    """Runs `steps` Iterate calls on the GPU and on the oracle with the same
+    transforms (drawn from the oracle's RNG) and compares them."""
+    orc.seed(seed)
+    solver = gpu.Solver(n, gpu.solve_options(1, 0.0), precision)
+    xd = solver.upload(x)
+    mm = orc.ModeMap()
+    for step in range(steps):
+        tf_o = orc.transform_draw(n)
+        qs = [orc.rand_int32() % n for _ in range(orc_opt.trials)]
+        f_ref, tr = orc.iterate(x if x_orc is None else x_orc, orc_opt, mm, tf=tf_o,
+                                qs=np.array(qs, np.int32), want_a=True)
+        f_got = solver.iterate(xd, iopts, _tf(gpu, tf_o, n), qs)
+        a_got = solver.bins_matrix(orc_opt.trials, tr["a"].shape[1], orc_opt.bins)
+        tol = FP64_TOL if precision == 0 else MIXED_TOL
+        for t in range(orc_opt.trials):
+            for r in range(a_got.shape[1]):
+                assert rel_err(a_got[t, r], tr["a"][t, r]) < tol, (step, t, r)
+        assert f_got == f_ref
+        assert_coefs_close(solver.modes_dict(), mm.to_dict(), tol, f"step {step}")
+    return mm
+
+
+@pytest.mark.parametrize("trials", [1, 3])
+def test_iterate_parity(gpu, orc, trials):
+    n, k, sigma = 1 << 16, 300, 1e-6
+    x, truth = make_signal(n, k, sigma, seed=21, magnitude="loguniform")
+    bins = 2 * k + 1
+    kw = dict(bins=bins, window_delta=1e-6, trials=trials, bin_threshold=3 * sigma / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
+    _iterate_both(gpu, orc, x, n, gpu.iterate_options(**kw), orc.IterateOptions(**kw), 3,
+                  seed=5 + trials)
+
+
+def test_reference_iterate_once_on_gpu(gpu, orc, kats):
+    """IterateOnce (src/iterate_test.cc:47-81) through the HIP path: prime n,
+    B=11, trials=5; identical to the oracle and within the test's bounds."""
+    k = kats["iterate_once"]
+    orc.seed(k["seed"])
+    modes = {int(m[0]): complex(m[1], m[2]) for m in k["modes"]}
+    xh = orc.generate_xhat(k["n"], orc.ModeMap(modes), k["sigma"])
+    x = np.fft.ifft(xh) * k["n"]
+    kw = dict(bins=k["bins"], window_delta=k["delta"], trials=k["trials"],
+              bin_threshold=k["bin_threshold"], window_threshold=k["window_threshold"])
+    mm = _iterate_both(gpu, orc, x, k["n"], gpu.iterate_options(**kw), orc.IterateOptions(**kw),
+                       1, seed=k["seed"] + 1)
+    d = mm.to_dict()
+    assert len(d) == 2
+    for key, want in modes.items():
+        assert abs(d[key] - want) < k["tolerance"]
+
+
+def test_reference_iterate_full_on_gpu(gpu, orc, kats):
+    """IterateFull (src/iterate_test.cc:126-175): n=1048573, 1000 modes,
+    B=2001, trials=3, four iterations, identical to the oracle."""
+    k = kats["iterate_full"]
+    orc.seed(k["seed"])
+    truth = orc.generate_mode_map(k["n"], k["num_modes"])
+    xh = orc.generate_xhat(k["n"], truth, k["sigma"])
+    x = np.fft.ifft(xh) * k["n"]
+    s = k["sigma"]
+    kw = dict(bins=k["bins"], window_delta=k["delta"], trials=k["trials"],
+              bin_threshold=3.0 * s / np.sqrt(k["bins"]), sv_threshold=1.0 * s / np.sqrt(k["bins"]),
+              window_threshold=k["window_threshold"])
+    mm = _iterate_both(gpu, orc, x, k["n"], gpu.iterate_options(**kw), orc.IterateOptions(**kw),
+                       4, seed=k["seed"] + 2)
+    td = truth.to_dict()
+    good = sum(1 for key in mm.to_dict() if key in td)
+    assert good >= k["min_good"][3]
+
+
+def test_iterate_mixed_parity(gpu, orc):
+    n, k, sigma = 1 << 16, 300, 1e-6
+    x, _ = make_signal(n, k, sigma, seed=22, magnitude="loguniform")
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    bins = 2 * k + 1
+    kw = dict(bins=bins, window_delta=1e-6, trials=1, bin_threshold=3 * sig_eff / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sig_eff / np.sqrt(bins))
+    _iterate_both(gpu, orc, x64, n, gpu.iterate_options(precision=1, **kw),
+                  orc.IterateOptions(**kw), 2, seed=31, precision=1,
+                  x_orc=x64.astype(np.complex128))
+
+
+# ---------------------------------------------------------------- Demo1::Run
+def _solve_both(gpu, orc, x_gpu, x_orc, n, k, sigma, seed, precision=0, **kw):
+    solver = gpu.Solver(n, gpu.solve_options(k, sigma, **kw), precision)
+    xd = solver.upload(x_gpu)
+    iters = solver.solve(xd, seed)
+    orc.seed(seed)
+    found, bins_log, found_log = orc.demo1_run(x_orc, k, sigma, orc.Demo1Options(**kw))
+    g_bins, g_found = solver.log()
+    assert iters == len(bins_log)
+    assert np.array_equal(g_bins, bins_log)
+    assert np.array_equal(g_found, found_log)
+    ref = found.to_dict()
+    got = solver.modes_dict()
+    assert set(got) == set(ref)
+    return got, ref, solver, xd
+
+
+def test_solve_parity_config1(gpu, orc):
+    """N=2^20, S=50 unit modes, sigma=1e-8, Demo1Options defaults."""
+    n, k, sigma = 1 << 20, 50, 1e-8
+    x, truth = make_signal(n, k, sigma, seed=1)
+    got, ref, solver, xd = _solve_both(gpu, orc, x, x, n, k, sigma, seed=7)
+    assert_coefs_close(got, ref, FP64_TOL)
+    xh = np.fft.fft(x) / n  # global oracle: the dense FFT
+    for key in truth:
+        assert key in got and abs(got[key] - xh[key]) <= 1e-6 * abs(xh[key])
+    # a second solve on the same solver reproduces the first
+    assert solver.solve(xd, 7) == len(solver.log()[0])
+    assert solver.modes_dict().keys() == got.keys()
+
+
+def test_solve_parity_config2_fp64(gpu, orc):
+    """N=2^22, S=1000, |c| log-uniform in [0.1, 10], sigma=1e-7, complex128."""
+    n, k, sigma = 1 << 22, 1000, 1e-7
+    x, truth = make_signal(n, k, sigma, seed=2, magnitude="loguniform")
+    got, ref, _, _ = _solve_both(gpu, orc, x, x, n, k, sigma, seed=11)
+    assert_coefs_close(got, ref, FP64_TOL)
+
+
+def test_solve_parity_config2_mixed(gpu, orc):
+    """The same signal stored as complex64; thresholds calibrated to
+    sigma_eff = sqrt(sigma^2 + sigma_q^2); compared with the oracle on the
+    same complex64-rounded input."""
+    n, k, sigma = 1 << 22, 1000, 1e-7
+    x, truth = make_signal(n, k, sigma, seed=2, magnitude="loguniform")
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    got, ref, _, _ = _solve_both(gpu, orc, x64, x64.astype(np.complex128), n, k, sig_eff,
+                                 seed=11, precision=1)
+    assert_coefs_close(got, ref, MIXED_TOL)
+    for key in truth:  # every recovered true mode within 1e-6 of itself
+        if key in got:
+            assert abs(got[key] - ref[key]) <= MIXED_TOL * abs(ref[key])
+    hits = [key for key in truth if key in got]
+    assert len(hits) >= 0.95 * k
+
+
+def test_solve_trials3_prime_n(gpu, orc):
+    n, k, sigma = 1048573, 100, 1e-6
+    x, _ = make_signal(n, k, sigma, seed=3)
+    got, ref, _, _ = _solve_both(gpu, orc, x, x, n, k, sigma, seed=13, trials=3)
+    assert_coefs_close(got, ref, FP64_TOL)
+
+
+# ---------------------------------------------------------------- edge cases
+def test_solve_zero_modes_and_errors(gpu):
+    n = 1 << 12
+    x = np.zeros(n, np.complex128)
+    s = gpu.Solver(n, gpu.solve_options(0, 1e-6))
+    xd = s.upload(x)
+    it = s.solve(xd, 1)
+    assert it == 3 and s.modes()[0].size == 0  # three stale iterations at B=101
+    with pytest.raises(gpu.MpsftError):
+        gpu.Solver(n, gpu.solve_options(5, 1e-6, trials=2)).solve(xd, 1)  # even trials
+    with pytest.raises(gpu.MpsftError):
+        gpu.Solver(n, gpu.solve_options(5, 1e-6, window_threshold=0.0)).solve(xd, 1)
+    with pytest.raises(gpu.MpsftError):
+        gpu.Solver(n, gpu.solve_options(5, 1e-6, min_bins=100)).solve(xd, 1)
+    bad = gpu.iterate_options(bins=11)
+    bad.bin_in_time_type = 4  # a reference CPU binner type
+    with pytest.raises(gpu.MpsftError):
+        s.iterate(xd, bad, gpu.transform_make(n, 3, 0, 0), [0])
+
+
+def test_capacity_overflow_is_reported(gpu):
+    n, k = 1 << 14, 50
+    x, _ = make_signal(n, k, 1e-6, seed=5)
+    s = gpu.Solver(n, gpu.solve_options(k, 1e-6), capacity=8)
+    with pytest.raises(gpu.MpsftError) as e:
+        s.solve(s.upload(x), 3)
+    assert e.value.status == 6
+
+
+def test_modes_set_get_roundtrip(gpu):
+    n = 1 << 16
+    s = gpu.Solver(n, gpu.solve_options(10, 1e-6))
+    d = {5: 1 + 2j, 70000 % n: -0.5j, n - 1: 3.0 + 0j}
+    s.set_modes(d)
+    assert s.modes_dict() == d
+    s.set_modes({})
+    assert s.modes_dict() == {}

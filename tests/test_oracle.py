@@ -1,0 +1,398 @@
+"""Pins the CPU oracle (oracle/mpsft_oracle.cpp) before it is trusted.
+
+Each check is independent of the oracle's own formulas: the reference's
+golden vectors (tests/golden/reference_kats.json, literals of src/*_test.cc),
+numpy's FFT and SVD, a long-double brute-force evaluation of the binning sum
+straight from its definition, and numpy's classic MT19937 stream.
+"""
+import numpy as np
+import pytest
+
+from conftest import approx
+from mpsft_amd.signals import make_signal
+
+
+def c(v):
+    return complex(v[0], v[1])
+
+
+# ---------------------------------------------------------------- RNG
+def test_rng_int32_is_mt19937_stream(orc):
+    """RandomInt32 (src/base.cc:53) = uniform_int_distribution<int32_t>() over
+    mt19937 as libstdc++ computed it before GCC 11: raw 32-bit outputs below
+    2^31, the rest rejected.  numpy's RandomState gives the raw MT19937
+    stream for a 32-bit seed (init_genrand), independently of libstdc++."""
+    for seed in (123534, 123537, 1, 42):
+        orc.seed(seed)
+        got = [orc.rand_int32() for _ in range(200)]
+        rs = np.random.RandomState(seed)
+        raw = rs.randint(0, 2**32, size=1000, dtype=np.uint64)
+        want = [int(v) for v in raw if v < 2**31][:200]
+        assert got == want
+
+
+# ---------------------------------------------------------------- integer
+def test_transform_kat(orc, kats):
+    k = kats["transform_kat"]
+    tf = orc.transform_make(k["n"], k["a"], k["b"], k["c"])
+    assert list(tf[:3]) == [k["a"], k["b"], k["c"]]
+    assert (int(tf[3]) * k["a"]) % k["n"] == 1
+
+
+@pytest.mark.parametrize("n", [536870909, 1048573, 2053, 1 << 22, 1 << 16, 2])
+def test_transform_draws_are_units(orc, n):
+    orc.seed(99)
+    for _ in range(2000):
+        a, b, cc, ainv = (int(v) for v in orc.transform_draw(n))
+        assert 1 <= a < n and 0 <= b < n and 0 <= cc < n
+        assert (a * ainv) % n == 1 % n
+        assert ainv == pow(a, -1, n) if n > 1 else True
+
+
+def test_transform_unreduced_inverse(orc, kats):
+    k = kats["time_freq_match_bigger"]
+    n = k["n"]
+    a, b, cc = k["tf"]
+    tf = orc.transform_make(n, a, b, cc)
+    assert (int(tf[3]) * a) % n == 1
+
+
+def test_mulmod_truncates_like_cpp(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        a, b = (int(v) for v in rng.integers(-2**31, 2**31, size=2))
+        d = int(rng.integers(2, 2**31))
+        p = a * b
+        want = abs(p) % d * (1 if p >= 0 else -1)  # C++ % truncates
+        assert orc.mul_mod(a, b, d) == want
+        assert orc.mul_pos_mod(a, b, d) == want % d
+
+
+@pytest.mark.parametrize("n,bins,want", [(2053, 11, 8), (1 << 22, 2001, 12), (1 << 22, 101, 16),
+                                         (1048573, 2001, 10), (5, 11, 1), (22, 11, 2),
+                                         (23, 11, 2)])
+def test_num_bits(orc, n, bins, want):
+    # NumBits = 1 + floor(log2(n / bins)) for n >= bins (src/iterate.cc:30-42)
+    assert orc.num_bits(n, bins) == want
+
+
+# ---------------------------------------------------------------- FFT
+def test_fft_kat(orc, kats):
+    k = kats["fft_kat"]
+    a = np.full(k["n"], c(k["fill"]))
+    b = orc.fft(a, -1)
+    assert approx(b[0].real, k["forward_x0"][0]) and approx(b[0].imag, k["forward_x0"][1])
+    back = orc.fft(b, +1)
+    for v in back:
+        assert approx(v.real, k["backward_all"][0]) and approx(v.imag, k["backward_all"][1])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 101, 127, 1999, 2001, 4096, 8001, 1109, 65537])
+def test_fft_matches_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    fwd = orc.fft(x, -1)
+    bwd = orc.fft(x, +1)
+    ref_f = np.fft.fft(x)
+    ref_b = np.fft.ifft(x) * n
+    scale = np.abs(ref_f).max()
+    assert np.abs(fwd - ref_f).max() <= 1e-12 * scale
+    assert np.abs(bwd - ref_b).max() <= 1e-12 * scale
+
+
+def test_dft_naive_matches_numpy(orc):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(37) + 1j * rng.standard_normal(37)
+    assert np.abs(orc.dft_naive(x, -1) - np.fft.fft(x)).max() < 1e-12
+
+
+# ---------------------------------------------------------------- window
+def test_window_kat(orc, kats):
+    """src/window_test.cc:30-79 with numpy's FFT as the transform."""
+    k = kats["window_kat"]
+    n, bins, delta = k["n"], k["bins"], k["delta"]
+    wt, t = orc.window_taps(n, bins, delta)
+    p = wt.size
+    p2 = (p - 1) // 2
+    assert p % 2 == 1 and p % bins == 0
+    a = np.zeros(n, np.complex128)
+    a[0] = wt[0]
+    for i in range(1, p2 + 1):
+        assert approx(wt[i], wt[p - i])
+        assert approx(orc.sample_in_time(n, bins, delta, i), orc.sample_in_time(n, bins, delta, -i))
+        a[i] = a[n - i] = wt[i]
+    ah = np.fft.fft(a)
+    assert np.all(np.abs(ah.imag) < 1.2e-5)
+    for i in (0, 1, n - 1, 2, n - 2):
+        assert approx(ah[i].real, 1.0)
+    for i in range(1, n // (4 * bins)):
+        assert ah[i].real > 0.5 and ah[n - i].real > 0.5
+    w = n // (2 * bins)
+    assert np.all(np.abs(ah[w:n - w].real) < delta)
+    assert approx(orc.sample_in_freq(n, bins, delta, 0.0), ah[0].real)
+    for i in range(1, (n - 1) // 2 + 1):
+        assert approx(orc.sample_in_freq(n, bins, delta, i / n), ah[i].real)
+        assert approx(orc.sample_in_freq(n, bins, delta, -i / n), ah[n - i].real)
+
+
+@pytest.mark.parametrize("bins,delta", [(2001, 1e-6), (101, 1e-6), (101, 1e-5), (5, 1e-6)])
+def test_window_stopband_large_n(orc, bins, delta):
+    """The taps' DTFT is below delta outside +-1/(2B) and matches
+    SampleInFreq on a grid (the same KAT at the benchmark sizes)."""
+    n = 1 << 20
+    wt, t = orc.window_taps(n, bins, delta)
+    a = np.zeros(n, np.complex128)
+    a[t % n] = wt
+    ah = np.fft.fft(a).real
+    xi = np.fft.fftfreq(n)
+    stop = np.abs(xi) >= 1.0 / (2 * bins) + 1.0 / n
+    assert np.abs(ah[stop]).max() < delta
+    # SampleInFreq is the untruncated window's response; truncating the taps
+    # at P changes it by less than delta.
+    idx = np.linspace(0, n - 1, 97).astype(int)
+    for i in idx:
+        assert abs(orc.sample_in_freq(n, bins, delta, xi[i]) - ah[i]) < delta
+
+
+# ---------------------------------------------------------------- pencil
+def test_pencil_kats(orc, kats):
+    for case in kats["pencil_kats"]["cases"]:
+        vote, s = orc.mat_pencil(c(case["u0"]), c(case["u1"]), c(case["u2"]))
+        assert vote == case["vote"]
+        assert approx(s[0], case["sigma"][0]) and approx(s[1], case["sigma"][1])
+
+
+def test_pencil_matches_numpy_svd(orc):
+    rng = np.random.default_rng(11)
+    for _ in range(3000):
+        u = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+        u *= 10.0 ** rng.uniform(-6, 3)
+        M = np.array([[u[0], u[2]], [u[1], u[0]]])
+        _, S, Vh = np.linalg.svd(M)
+        v = np.conj(Vh[0])  # dominant right singular vector
+        w = np.conj(v[0]) * v[1]
+        vote, s = orc.mat_pencil(u[0], u[1], u[2])
+        assert np.allclose(s, S, rtol=1e-10, atol=1e-14 * S[0])
+        if abs(w.imag) > 1e-9 * abs(w):
+            assert vote == (w.imag < 0)
+
+
+# ---------------------------------------------------------------- binners
+def _mode_signal(orc, n, modes):
+    mm = orc.ModeMap({int(k): complex(re, im) for k, re, im in modes})
+    return mm, orc.evaluate_modes(n, mm)
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_time_freq_match_golden(orc, kats, version):
+    k = kats["time_freq_match"]
+    n, bins, bits, q = k["n"], k["bins"], k["bits"], k["q"]
+    mm, x = _mode_signal(orc, n, k["modes"])
+    tf = orc.transform_make(n, *k["tf"])
+    out = orc.bin_in_time(x, bins, k["delta"], bits, tf, q, version=version)
+    assert approx(out[0, 1].real, k["row0_bin1"][0])
+    assert approx(out[0, 1].imag, k["row0_bin1"][1])
+    for i in range(bins):
+        if i != 1:
+            assert approx(abs(out[0, i]), 0.0)
+    for fv in (0, 1):
+        fr = np.zeros_like(out)
+        keys, coefs = mm.items()
+        orc.bin_in_freq(keys, coefs, n, bins, k["delta"], bits, tf, q, fr, version=fv)
+        assert approx(fr[0, 1].real, -k["row0_bin1"][0])
+        assert np.all(np.abs(out + fr) < 1.19e-5)
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_time_freq_match_bigger(orc, kats, version):
+    k = kats["time_freq_match_bigger"]
+    n, bins, bits, q = k["n"], k["bins"], k["bits"], k["q"]
+    mm, x = _mode_signal(orc, n, k["modes"])
+    tf = orc.transform_make(n, *k["tf"])
+    out = orc.bin_in_time(x, bins, k["delta"], bits, tf, q, version=version)
+    for fv in (0, 1):
+        fr = np.zeros_like(out)
+        keys, coefs = mm.items()
+        orc.bin_in_freq(keys, coefs, n, bins, k["delta"], bits, tf, q, fr, version=fv)
+        assert np.all(np.abs(out[0] + fr[0]) < 1.19e-5)
+
+
+def test_bin_in_time_match_v0_v1(orc, kats):
+    k = kats["bin_in_time_match"]
+    mm, x = _mode_signal(orc, k["n"], k["modes"])
+    tf = orc.transform_make(k["n"], *k["tf"])
+    a0 = orc.bin_in_time(x, k["bins"], k["delta"], k["bits"], tf, k["q"], version=0)
+    a1 = orc.bin_in_time(x, k["bins"], k["delta"], k["bits"], tf, k["q"], version=1)
+    assert np.all(np.abs(a0 - a1) < 1.19e-5)
+
+
+def _brute_bin_in_time(x, n, bins, bits, tf, q, wt):
+    """U_row[j] = sum_t y_{t+tau} e^{-2 pi i t (j + 1/2)/B} wt(t), with
+    y_s = x[(a s + c) mod n] e^{2 pi i b s/n} (report3 eq. for the binning,
+    src/binner.cc:93-110), summed in long double without folding."""
+    a, b, cc, _ = (int(v) for v in tf)
+    p = wt.size
+    p2 = (p - 1) // 2
+    ts = np.arange(-p2, p2 + 1)
+    w = np.concatenate([wt[p2 + 1:], wt[:p2 + 1]]).astype(np.longdouble)  # t order
+    rows = 1 + 2 * bits
+    out = np.zeros((rows, bins), np.clongdouble)
+    two_pi = 2 * np.pi
+    for u in range(rows):
+        if u == 0:
+            tau = q
+        elif u % 2:
+            tau = q + bins * (1 << ((u - 1) // 2))
+        else:
+            tau = q - bins * (1 << (u // 2 - 1))
+        s = ts + tau
+        idx = (a * s + cc) % n
+        ph = ((b * s) % n).astype(np.longdouble) / n
+        y = x[idx].astype(np.clongdouble) * np.exp(1j * two_pi * ph.astype(np.float64)).astype(np.clongdouble)
+        for j in range(bins):
+            e = np.exp(-1j * two_pi * (ts * (j + 0.5) / bins % 1.0))
+            out[u, j] = np.sum(y * w * e)
+    return out.astype(np.complex128)
+
+
+@pytest.mark.parametrize("n,bins,bits,q", [(67, 3, 2, 5), (128, 5, 3, 100), (211, 1, 2, 0),
+                                           (1 << 12, 9, 4, 777)])
+def test_bin_in_time_brute_force(orc, n, bins, bits, q):
+    rng = np.random.default_rng(n + bins)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    tf = orc.transform_draw(n) if n != 128 else orc.transform_make(n, 77, 13, 5)
+    wt, _ = orc.window_taps(n, bins, 1e-6)
+    ref = _brute_bin_in_time(x, n, bins, bits, tf, q, wt)
+    for version in (0, 1):
+        got = orc.bin_in_time(x, bins, 1e-6, bits, tf, q, version=version)
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- iterate
+def _xhat_to_x(xh):
+    return np.fft.ifft(xh) * xh.size
+
+
+def test_iterate_once(orc, kats):
+    """IterateOnce (src/iterate_test.cc:47-81)."""
+    k = kats["iterate_once"]
+    orc.seed(k["seed"])
+    modes = {int(m[0]): complex(m[1], m[2]) for m in k["modes"]}
+    mm = orc.ModeMap(modes)
+    x = _xhat_to_x(orc.generate_xhat(k["n"], mm, k["sigma"]))
+    opt = orc.IterateOptions(bins=k["bins"], window_delta=k["delta"], trials=k["trials"],
+                             bin_threshold=k["bin_threshold"],
+                             window_threshold=k["window_threshold"])
+    found = orc.ModeMap()
+    orc.iterate(x, opt, found)
+    d = found.to_dict()
+    assert len(d) == 2
+    for key, want in modes.items():
+        assert key in d and abs(d[key] - want) < k["tolerance"]
+
+
+def _count_good_bad(found, truth):
+    good = sum(1 for key in found if key in truth)
+    return good, len(found) - good
+
+
+def test_iterate_more(orc, kats):
+    """IterateMore (src/iterate_test.cc:84-121)."""
+    k = kats["iterate_more"]
+    orc.seed(k["seed"])
+    truth = orc.generate_mode_map(k["n"], k["num_modes"])
+    x = _xhat_to_x(orc.generate_xhat(k["n"], truth, k["sigma"]))
+    opt = orc.IterateOptions(bins=k["bins"], window_delta=k["delta"], trials=k["trials"],
+                             bin_threshold=k["bin_threshold"],
+                             window_threshold=k["window_threshold"])
+    found = orc.ModeMap()
+    td = truth.to_dict()
+    orc.iterate(x, opt, found)
+    g, b = _count_good_bad(found.to_dict(), td)
+    assert g >= k["min_good"][0] and b <= k["max_bad"][0]
+    opt.bins = k["bins2"]
+    orc.iterate(x, opt, found)
+    g, b = _count_good_bad(found.to_dict(), td)
+    assert g >= k["min_good"][1] and b <= k["max_bad"][1]
+
+
+def test_iterate_full(orc, kats):
+    """IterateFull (src/iterate_test.cc:126-175)."""
+    k = kats["iterate_full"]
+    orc.seed(k["seed"])
+    truth = orc.generate_mode_map(k["n"], k["num_modes"])
+    x = _xhat_to_x(orc.generate_xhat(k["n"], truth, k["sigma"]))
+    s = k["sigma"]
+    opt = orc.IterateOptions(bins=k["bins"], window_delta=k["delta"], trials=k["trials"],
+                             bin_threshold=3.0 * s / np.sqrt(k["bins"]),
+                             sv_threshold=1.0 * s / np.sqrt(k["bins"]),
+                             window_threshold=k["window_threshold"])
+    found = orc.ModeMap()
+    td = truth.to_dict()
+    for i in range(4):
+        orc.iterate(x, opt, found)
+        g, b = _count_good_bad(found.to_dict(), td)
+        assert g >= k["min_good"][i] and b <= k["max_bad"][i]
+
+
+def test_demo1_recovers_against_dense_fft(orc):
+    """Global check: every true mode is recovered and its coefficient
+    matches x-hat = FFT(x)/n from numpy's dense FFT."""
+    # sigma near the window's delta: below it the window's own leakage
+    # (<= delta per mode) fails the sigma-calibrated pencil test.
+    n, k, sigma = 1 << 14, 20, 1e-6
+    x, truth = make_signal(n, k, sigma, seed=4)
+    xh = np.fft.fft(x) / n
+    orc.seed(17)
+    found, bins_log, found_log = orc.demo1_run(x, k, sigma)
+    d = found.to_dict()
+    for key in truth:
+        assert key in d
+        assert abs(d[key] - xh[key]) <= 1e-5 * abs(xh[key])
+    assert bins_log[0] == max(2 * k + 1, 101)
+    assert np.all(np.diff(found_log) >= 0)
+
+
+# ---------------------------------------------------------------- gen
+def test_generate_xhat_energy(orc):
+    """GenerateXhatBasic / GenerateXhatFFT (src/gen_test.cc:26-52)."""
+    orc.seed(1)
+    mm = orc.ModeMap({2: 0.5 + 0.6j})
+    xh = orc.generate_xhat(5, mm, 8.0)
+    xh[2] = 0
+    assert approx(np.sqrt(np.sum(np.abs(xh) ** 2)), 8.0)
+    xh = orc.generate_xhat(10000, orc.ModeMap(), 5.5)
+    x = _xhat_to_x(xh)
+    assert approx(np.sqrt(np.sum(np.abs(x) ** 2) / x.size), 5.5)
+
+
+def test_generate_xhat_alt_energy(orc):
+    """GenerateXhatAltBasic (src/gen_test.cc:54-70)."""
+    orc.seed(2)
+    mm = orc.ModeMap({2: 0.5 + 0.6j})
+    tot = 0.0
+    trials = 100000
+    for _ in range(trials):
+        xh = orc.generate_xhat_alt(5, mm, 8.0)
+        xh[2] -= 0.5 + 0.6j
+        tot += np.sum(np.abs(xh) ** 2)
+    assert abs(tot / trials - 64.0) < 0.1 * 64.0 / 8.0 * 8.0 / 8.0 + 0.5
+
+
+def test_generate_mode_map_unit(orc):
+    orc.seed(3)
+    mm = orc.generate_mode_map(1000, 50)
+    keys, coefs = mm.items()
+    assert len(set(keys.tolist())) == 50 and keys.min() >= 0 and keys.max() < 1000
+    assert np.allclose(np.abs(coefs), 1.0)
+
+
+def test_post_analyze(orc):
+    truth = orc.ModeMap({1: 1 + 0j, 5: 2j})
+    xh = np.zeros(8, np.complex128)
+    xh[1], xh[5] = 1.0, 2j
+    found = orc.ModeMap({1: 1.1 + 0j, 3: 0.5 + 0j})
+    st = orc.post_analyze(8, truth, xh, found)
+    assert st["hit"] == 1 and st["miss"] == 1
+    assert abs(st["hit_l1"] - 0.1) < 1e-12 and abs(st["miss_l1"] - 0.5) < 1e-12
+    assert abs(st["max"] - 2.0) < 1e-12
