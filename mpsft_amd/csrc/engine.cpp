@@ -3,6 +3,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -53,23 +54,30 @@ int Alloc(T **p, size_t count) {
     if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
   } while (0)
 
-// Segment count and k_bin_time tiles per trial for one (B, bits, f).
-void TileShape(int32_t B, int32_t bits, int32_t f, int32_t *tiles, int32_t *len0) {
+// Distinct lags of one iteration (the union of every row's f lags).
+int32_t NumLags(int32_t bits, int32_t f) {
   int K = 0;
   while (K < bits && (1 << K) < f) ++K;
-  const int32_t nseg = 1 + 2 * (bits - K);
-  const int32_t nblk = (B + kResBlock - 1) / kResBlock;
-  *tiles = nseg * nblk;
-  *len0 = (K ? (1 << K) : 0) + f;
+  return (K ? (1 << K) : 0) + f + 2 * (bits - K) * f;
+}
+
+// Residues per k_bin_time tile: the largest power of two <= 8 that still
+// gives kTargetTiles tiles, and keeps the tile's LDS under 64 KiB.
+int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f) {
+  int32_t R = 8;
+  while (R > 1 && ((B + R - 1) / R < kTargetTiles ||
+                   size_t(nlam + f) * R * sizeof(double2) > 65536))
+    R /= 2;
+  return R;
 }
 
 int32_t ChunkIters() {
-  const char *e = std::getenv("MPSFT_GRAPH_ITERS");
+  const char *e = std::getenv("MPSFT_LOOKAHEAD");
   if (e) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4096) return v;
   }
-  return 8;
+  return 2;
 }
 
 }  // namespace
@@ -104,6 +112,7 @@ bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp) {
   bp->logM = logM;
   bp->M = 1 << logM;
   bp->off = 0;
+  bp->R = 1;
   bp->width = width;
   bp->sigma_f = sigma_f;
   bp->dfreq = kSqrt2 * kPi * sigma_t;
@@ -164,6 +173,15 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
     status_ = MPSFT_ERR_NOMEM;
     return;
   }
+  int32_t *hp = nullptr;
+  if (hipHostMalloc(reinterpret_cast<void **>(&hp), 2 * sizeof(int32_t),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void **>(&d_progress_), hp, 0) != hipSuccess) {
+    status_ = MPSFT_ERR_NOMEM;
+    return;
+  }
+  h_progress_ = hp;
+  hp[0] = hp[1] = 0;
   std::memset(h_state_, 0, sizeof(DevState));
   if (hipMemset(d_state_, 0, sizeof(DevState)) != hipSuccess ||
       hipMemset(d_coef_, 0, size_t(n_) * sizeof(double2)) != hipSuccess ||
@@ -184,7 +202,8 @@ Engine::~Engine() {
   (void)hipFree(d_qs_);
   (void)hipFree(d_fold_);
   (void)hipFree(d_A_);
-  (void)hipFree(d_terms_);
+  (void)hipFree(d_term_bin_);
+  (void)hipFree(d_term_val_);
   (void)hipFree(d_keys_);
   (void)hipFree(d_coef_);
   (void)hipFree(d_present_);
@@ -192,6 +211,7 @@ Engine::~Engine() {
   (void)hipFree(d_tmp_);
   (void)hipFree(d_tmpk_);
   if (h_state_) (void)hipHostFree(h_state_);
+  if (h_progress_) (void)hipHostFree(const_cast<int32_t *>(h_progress_));
 }
 
 // Builds the window parameters, chirps and Bluestein filters of every B in
@@ -222,6 +242,7 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     BParams bp;
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogM) return MPSFT_ERR_UNSUPPORTED;
+    bp.R = ResiduesPerTile(B, NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f), bp.f);
     bp.off = table_used_ + need;
     need += B + bp.M;
     hbtab_[B] = bp;
@@ -282,19 +303,20 @@ int Engine::SetThresholds(double sigma) {
 
 // Sizes the per-iteration buffers for trials and every B in bs.
 int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
-  int32_t Bc = 0, rc = 0, lc = 0, fc = 0, tc = 0, Mc = 0;
+  int32_t Bc = 0, rc = 0, tc = 0, Mc = 0;
+  size_t lbt = 0, lfft = 0;
   for (int32_t B : bs) {
     const BParams &bp = hbtab_[B];
     const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
-    int32_t tiles, len0;
-    TileShape(B, bits, bp.f, &tiles, &len0);
     Bc = std::max(Bc, B);
     rc = std::max(rc, 1 + 2 * bits);
-    lc = std::max(lc, len0);
-    fc = std::max(fc, bp.f);
-    tc = std::max(tc, tiles);
+    tc = std::max(tc, (B + bp.R - 1) / bp.R);
     Mc = std::max(Mc, bp.M);
+    lbt = std::max(lbt, (size_t(NumLags(bits, bp.f) + bp.f) * bp.R + kThreads) * sizeof(double2));
+    lfft = std::max(lfft, (size_t(bp.M) + (bp.M <= kLdsTwMaxM ? bp.M / 2 : 0)) * sizeof(double2));
   }
+  // k_identify keeps a bin's trials x rows column in 64 x kColSlices lanes.
+  if (int64_t(trials) * rc > 64 * kColSlices) return MPSFT_ERR_UNSUPPORTED;
   const bool grow_rows = trials > trials_cap_ || rc > rows_cap_ || Bc > B_cap_;
   if (grow_rows) {
     trials_cap_ = std::max(trials_cap_, trials);
@@ -302,18 +324,19 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     B_cap_ = std::max(B_cap_, Bc);
     const size_t sz = size_t(trials_cap_) * rows_cap_ * B_cap_;
     if (Alloc(&d_fold_, sz) != MPSFT_OK || Alloc(&d_A_, sz) != MPSFT_OK ||
-        Alloc(&d_terms_, size_t(trials_cap_) * cap_) != MPSFT_OK)
+        Alloc(&d_term_bin_, size_t(trials_cap_) * cap_) != MPSFT_OK ||
+        Alloc(&d_term_val_, size_t(trials_cap_) * rows_cap_ * cap_) != MPSFT_OK)
       return MPSFT_ERR_NOMEM;
     HIP_OK(hipMemset(d_fold_, 0, sz * sizeof(double2)));
     HIP_OK(hipMemset(d_A_, 0, sz * sizeof(double2)));
     tf_valid_ = false;  // q stride follows trials_cap_
     ++shape_gen_;
   }
-  if (lc > len_cap_ || fc > f_cap_ || tc > tiles_cap_ || Mc > M_cap_) {
-    len_cap_ = std::max(len_cap_, lc);
-    f_cap_ = std::max(f_cap_, fc);
+  if (tc > tiles_cap_ || Mc > M_cap_ || lbt > lds_bin_time_ || lfft > lds_fft_) {
     tiles_cap_ = std::max(tiles_cap_, tc);
     M_cap_ = std::max(M_cap_, Mc);
+    lds_bin_time_ = std::max(lds_bin_time_, lbt);
+    lds_fft_ = std::max(lds_fft_, lfft);
     ++shape_gen_;
   }
   return MPSFT_OK;
@@ -395,35 +418,34 @@ Ctx Engine::MakeCtx() const {
   c.mask = mask_;
   c.fold = d_fold_;
   c.A = A_override_ ? A_override_ : d_A_;
-  c.terms = d_terms_;
+  c.term_bin = d_term_bin_;
+  c.term_val = d_term_val_;
   c.keys = d_keys_;
   c.coef = d_coef_;
   c.present = d_present_;
   c.log = d_log_;
   c.log_cap = log_cap_;
+  c.progress = d_progress_;
   c.cap = cap_;
   c.trials_cap = trials_cap_;
   c.rows_cap = rows_cap_;
   c.B_cap = B_cap_;
-  c.len_cap = len_cap_;
-  c.f_cap = f_cap_;
   c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
   return c;
 }
 
-size_t Engine::LdsBinTime() const {
-  return size_t(len_cap_ + f_cap_) * kResBlock * sizeof(double2) + kThreads * sizeof(double2);
-}
-size_t Engine::LdsFft() const { return size_t(M_cap_) * sizeof(double2); }
+size_t Engine::LdsBinTime() const { return lds_bin_time_; }
+size_t Engine::LdsFft() const { return lds_fft_; }
 int32_t Engine::TilesCap() const { return trials_cap_ * tiles_cap_; }
 int32_t Engine::IdentifyBlocks() const {
-  return std::max<int32_t>(1, (B_cap_ + kThreads - 1) / kThreads);
+  const int32_t waves = kThreads / 64;  // one wavefront per bin
+  return std::max<int32_t>(1, (B_cap_ + waves - 1) / waves);
 }
 
-int Engine::LaunchIteration(const Ctx &c, hipStream_t s) {
-  HIP_OK(LaunchBinTime(c, TilesCap(), LdsBinTime(), s));
-  HIP_OK(LaunchBinFft(c, LdsFft(), s));
-  HIP_OK(LaunchIdentify(c, IdentifyBlocks(), s));
+int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s) {
+  HIP_OK(LaunchBinTime(c, parity, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinFft(c, parity, LdsFft(), s));
+  HIP_OK(LaunchIdentify(c, parity, IdentifyBlocks(), s));
   return MPSFT_OK;
 }
 
@@ -439,7 +461,9 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
   hipGraph_t g;
   HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   int rc = MPSFT_OK;
-  for (int32_t i = 0; i < iters && rc == MPSFT_OK; ++i) rc = LaunchIteration(c, cs);
+  // iters is even: the graph starts at an even iteration, so node i runs
+  // with parity i & 1.
+  for (int32_t i = 0; i < iters && rc == MPSFT_OK; ++i) rc = LaunchIteration(c, i & 1, cs);
   const hipError_t ec = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);
   if (rc != MPSFT_OK || ec != hipSuccess) return MPSFT_ERR_HIP;
@@ -464,8 +488,8 @@ int Engine::ReadState(hipStream_t s) {
 int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
                   int32_t *iters, hipStream_t s, double *kernel_ms) {
   if (status_ != MPSFT_OK) return status_;
-  if (!x || opt.trials <= 0 || opt.trials % 2 != 1 || !(opt.window_threshold > 0) ||
-      opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 || opt.max_stale_iter <= 0 ||
+  if (!x || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
+      !(opt.window_threshold > 0) || opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 || opt.max_stale_iter <= 0 ||
       !(opt.window_delta > 0) || !(opt.window_delta < 1) || opt.sigma < 0)
     return MPSFT_ERR_INVALID_ARG;
   const int64_t want0 = 2 * int64_t(opt.k) + 1;
@@ -486,6 +510,8 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   if (rc != MPSFT_OK) return rc;
 
   Ctx c = MakeCtx();
+  h_progress_[0] = 0;  // the previous solve has finished; the device writes
+  h_progress_[1] = 0;  // these again only after k_init below
   HIP_OK(LaunchClearModes(c, 0, s));
   InitArgs a{};
   a.x = x;
@@ -497,7 +523,6 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   a.max_stale = opt.max_stale_iter;
   a.max_iter = max_iter;
   a.bits_override = 0;
-  a.use_table_thr = 1;
   a.win_thr = opt.window_threshold;
   HIP_OK(LaunchInit(c, a, s));
 
@@ -510,53 +535,90 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
     for (int i = 0; i < 3; ++i) kernel_ms[i] = 0.0;
     int out = MPSFT_OK;
     for (; launched < max_iter; ++launched) {
-      if ((out = EnsureTransforms(seed, launched + 1, opt.trials, s)) != MPSFT_OK) break;
+      if ((out = EnsureTransforms(seed, std::min(max_iter, launched + 2), opt.trials, s)) !=
+          MPSFT_OK)
+        break;
       c = MakeCtx();
+      const int par = launched & 1;
       if (hipEventRecord(ev[0], s) != hipSuccess ||
-          LaunchBinTime(c, TilesCap(), LdsBinTime(), s) != hipSuccess ||
-          hipEventRecord(ev[1], s) != hipSuccess || LaunchBinFft(c, LdsFft(), s) != hipSuccess ||
+          LaunchBinTime(c, par, TilesCap(), LdsBinTime(), s) != hipSuccess ||
+          hipEventRecord(ev[1], s) != hipSuccess ||
+          LaunchBinFft(c, par, LdsFft(), s) != hipSuccess ||
           hipEventRecord(ev[2], s) != hipSuccess ||
-          LaunchIdentify(c, IdentifyBlocks(), s) != hipSuccess ||
+          LaunchIdentify(c, par, IdentifyBlocks(), s) != hipSuccess ||
           hipEventRecord(ev[3], s) != hipSuccess) {
         out = MPSFT_ERR_HIP;
         break;
       }
       if ((out = ReadState(s)) != MPSFT_OK) break;
+      if (h_state_->L[par].done) break;  // the loop had stopped before this iteration
       for (int i = 0; i < 3; ++i) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) out = MPSFT_ERR_HIP;
         kernel_ms[i] += ms;
       }
-      if (out != MPSFT_OK || h_state_->done) break;
+      if (out != MPSFT_OK) break;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (out != MPSFT_OK) return out;
   } else {
-    for (;;) {
-      const int32_t need = std::min(max_iter, launched + chunk);
-      rc = EnsureTransforms(seed, need, opt.trials, s);
-      if (rc != MPSFT_OK) return rc;
-      rc = EnsureGraph(chunk, s);
-      if (rc != MPSFT_OK) return rc;
-      HIP_OK(hipGraphLaunch(graph_exec_, s));
-      launched += chunk;
-      rc = ReadState(s);
-      if (rc != MPSFT_OK) return rc;
-      if (h_state_->done) break;
-      if (launched >= max_iter) break;
+    // One graph per two iterations, launched ahead of the device by at most
+    // `lookahead` iterations.  The first kernel of every iteration reports
+    // {iterations done, stopped} in pinned host memory, so the host paces
+    // launches without waiting on the stream and stops once the loop stops.
+    rc = EnsureGraph(2, s);
+    if (rc != MPSFT_OK) return rc;
+    const int32_t lookahead = chunk < 2 ? 2 : chunk;
+    auto last_change = std::chrono::steady_clock::now();
+    int32_t seen = 0, synced = 0;
+    bool done = false;
+    for (uint64_t spin = 0; !done; ++spin) {
+      const int32_t hp0 = h_progress_[0];
+      const int32_t progressed = hp0 > synced ? hp0 : synced;
+      // Launch up to iteration max_iter: its first kernel sees the cap and
+      // stops the loop (logging the last iteration).
+      if (h_progress_[1] || launched > max_iter) break;
+      if (launched - progressed < lookahead) {
+        rc = EnsureTransforms(seed, std::min(max_iter, launched + 3), opt.trials, s);
+        if (rc != MPSFT_OK) return rc;
+        HIP_OK(hipGraphLaunch(graph_exec_, s));
+        launched += 2;
+        continue;
+      }
+      if (progressed != seen) {
+        seen = progressed;
+        last_change = std::chrono::steady_clock::now();
+      } else if ((spin & 4095) == 0 &&
+                 std::chrono::steady_clock::now() - last_change > std::chrono::milliseconds(200)) {
+        // No progress report for a while (a long iteration): wait on the
+        // stream instead of spinning, then continue from the device state.
+        rc = ReadState(s);
+        if (rc != MPSFT_OK) return rc;
+        synced = std::max(h_state_->L[0].it, h_state_->L[1].it);
+        done = (h_state_->L[0].done | h_state_->L[1].done) != 0;
+        last_change = std::chrono::steady_clock::now();
+      }
     }
+    rc = ReadState(s);
+    if (rc != MPSFT_OK) return rc;
   }
-  *iters = h_state_->it;
+  const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
+  if (!stopped) {
+    // Only when max_iter stops the launches before the device saw it.
+    *iters = std::max(h_state_->L[0].it, h_state_->L[1].it) + 1;
+    return MPSFT_ERR_NOT_CONVERGED;
+  }
+  *iters = h_state_->iters;
   if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
-  if (!h_state_->done) return MPSFT_ERR_NOT_CONVERGED;
-  if (h_state_->stale < opt.max_stale_iter) return MPSFT_ERR_NOT_CONVERGED;
+  if (h_state_->stop_stale < opt.max_stale_iter) return MPSFT_ERR_NOT_CONVERGED;
   return MPSFT_OK;
 }
 
 int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const DevTf &tf,
                         const int64_t *qs, int32_t *found, hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
-  if (!x || !qs || opt.trials <= 0 || opt.trials % 2 != 1 || !(opt.window_threshold > 0) ||
+  if (!x || !qs || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
+      !(opt.window_threshold > 0) ||
       opt.bins <= 0 || opt.bins % 2 != 1 || !(opt.window_delta > 0))
     return MPSFT_ERR_INVALID_ARG;
   const int want_type = precision_ == MPSFT_FP64 ? 5 : 6;
@@ -590,11 +652,11 @@ int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const D
   a.sv_thr = opt.sv_threshold;
   a.win_thr = opt.window_threshold;
   HIP_OK(LaunchInit(c, a, s));
-  rc = LaunchIteration(c, s);
+  rc = LaunchIteration(c, 0, s);
   if (rc != MPSFT_OK) return rc;
   rc = ReadState(s);
   if (rc != MPSFT_OK) return rc;
-  *found = h_state_->found_last;
+  *found = h_state_->found_iter[0];
   if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
   return MPSFT_OK;
 }
@@ -628,8 +690,8 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   a.bits_override = bits;
   a.win_thr = 1;
   HIP_OK(LaunchInit(c, a, s));
-  HIP_OK(LaunchBinTime(c, TilesCap(), LdsBinTime(), s));
-  HIP_OK(LaunchBinFft(c, LdsFft(), s));
+  HIP_OK(LaunchBinTime(c, 0, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinFft(c, 0, LdsFft(), s));
   A_override_ = nullptr;
   return MPSFT_OK;
 }
@@ -673,7 +735,7 @@ int Engine::ModesSet(const int64_t *keys, const double *coefs, int64_t m) {
 int Engine::BinsGet(double *out, int64_t cap, int32_t *trials, int32_t *rows, int32_t *bins) {
   int rc = ReadState(nullptr);
   if (rc != MPSFT_OK) return rc;
-  const int32_t T = h_state_->trials, R = h_state_->rows, B = h_state_->B;
+  const int32_t T = h_state_->trials, R = h_state_->P[0].rows, B = h_state_->P[0].B;
   *trials = T;
   *rows = R;
   *bins = B;
@@ -689,7 +751,7 @@ int Engine::BinsGet(double *out, int64_t cap, int32_t *trials, int32_t *rows, in
 int Engine::LogGet(int32_t *bins, int32_t *found, int32_t cap, int32_t *n_iter) {
   int rc = ReadState(nullptr);
   if (rc != MPSFT_OK) return rc;
-  const int32_t it = std::min(h_state_->it, log_cap_);
+  const int32_t it = std::min(h_state_->iters, log_cap_);
   *n_iter = it;
   const int32_t m = std::min(it, cap);
   if (m > 0) {
@@ -712,7 +774,7 @@ int Engine::TransformsGet(DevTf *tfs, int64_t *qs, int32_t cap, int32_t trials, 
     *count = 0;
     return MPSFT_OK;
   }
-  const int32_t m = std::min(std::min(h_state_->it, tf_count_), cap);
+  const int32_t m = std::min(std::min(h_state_->iters, tf_count_), cap);
   for (int32_t i = 0; i < m; ++i) {
     tfs[i] = h_tfs_[i];
     for (int32_t t = 0; t < trials; ++t) qs[int64_t(i) * trials + t] = h_qs_[size_t(i) * trials_cap_ + t];

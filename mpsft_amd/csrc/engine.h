@@ -2,15 +2,17 @@
 //
 // One Engine holds the device state of one signal length n.  An iteration
 // of Iterate (src/iterate.cc:80-177) is three kernels on one stream:
-//   k_bin_time  : permuted, windowed gather + fold (BinInTime without its
-//                 B-point DFT), plus the per-mode terms BinInFreq needs;
+//   k_bin_time  : derives the iteration's parameters from the loop state
+//                 (Demo1::Run, src/demo1.cc:36-59; workgroup 0 publishes
+//                 them), then the permuted, windowed gather + fold
+//                 (BinInTime without its B-point DFT), plus the per-mode
+//                 terms BinInFreq needs;
 //   k_bin_fft   : per (trial, row), the B-point forward DFT (Bluestein over
 //                 an LDS-resident power-of-two FFT), the row phase, and the
 //                 BinInFreq subtraction;
 //   k_identify  : per bin, the energy test, the matrix-pencil votes
 //                 (IdentifyFreq), the coefficient estimate and the insert
-//                 into the found-mode map; the last workgroup advances the
-//                 loop state (Demo1::Run, src/demo1.cc:36-59).
+//                 into the found-mode map.
 // Every kernel reads the iteration's parameters (B, bits, transform, q)
 // from DevState in device memory, so one fixed launch sequence is captured
 // in a hipGraph and replayed for any iteration.
@@ -26,16 +28,21 @@
 
 namespace mpsft {
 
-constexpr int kThreads = 256;     // workgroup size of every kernel
-constexpr int kResBlock = 32;     // residues per k_bin_time tile
-constexpr int kMaxLogM = 13;      // Bluestein FFT length limit 8192 => B <= 4096
-constexpr int kPrepBlocks = 4;    // k_bin_time workgroups computing mode terms
+constexpr int kThreads = 256;      // workgroup size of most kernels
+constexpr int kFftThreads = 512;   // workgroup size of k_bin_fft
+constexpr int kMaxLogM = 13;       // Bluestein FFT length limit 8192 => B <= 4096
+constexpr int kLdsTwMaxM = 4096;   // k_bin_fft stages twiddles in LDS up to this M
+constexpr int kPrepBlocks = 16;    // k_bin_time workgroups computing mode terms
 constexpr int kMaxBins = 4096;
+constexpr int kTargetTiles = 512;  // k_bin_time tiles aimed for per trial
+constexpr int kColSlices = 4;      // k_identify holds trials*rows <= 256 entries per bin
+constexpr int kMaxTrials = 15;     // trials per iteration (odd)
 
 // Per-B window and bin-FFT parameters, computed on the host with the
 // reference's formulas (src/window.cc:28-50, src/iterate.cc:30-42).
 struct BParams {
   int32_t B, P, f, bits, M, logM;
+  int32_t R;        // residues per k_bin_time tile
   int64_t off;      // offset (in double2) of [chirp(B) | filter(M)] in table
   double width;     // 1/(2B)
   double sigma_f;   // window std-dev in frequency
@@ -52,33 +59,42 @@ struct DevTf {
 enum : int32_t { kModeSolve = 0, kModeIterate = 1, kModeBinOnly = 2 };
 enum : int32_t { kFlagOverflow = 1 };
 
-// Loop state, read at the start of every kernel of an iteration.
-struct DevState {
-  const void *x;  // input signal (double2 or float2)
-  int32_t B, bits, rows, f, P, p2, M, logM;
+// The parameters of one iteration.  Every k_bin_time workgroup derives them
+// from the previous iteration's slot and the counters k_identify left;
+// workgroup 0 publishes them in slot (iteration & 1) for the iteration's
+// k_bin_fft and k_identify.
+struct IterParams {
+  DevTf tf;
+  int64_t q[kMaxTrials];
   int64_t tab_off;
-  double width, sigma_f, dfreq;
-  double bin_thr, sv_thr, win_thr;
-  int32_t trials;
-  int32_t it;          // iteration index into the transform table
-  int32_t stale;
-  int32_t done;
-  int32_t count;       // number of distinct found keys
-  int32_t found_iter;  // set when this iteration found anything
-  uint32_t blocks_done;
-  int32_t flags;
-  int32_t mode;
-  int32_t k, min_bins, max_stale, max_iter;
-  int32_t found_last;  // found_iter of the last completed iteration
+  double width, sigma_f, dfreq;  // window (src/window.cc:28-50)
+  double bin_thr, sv_thr;
+  int32_t B, bits, rows, f, P, p2, M, logM, R;
+  int32_t it;    // iteration index
+  int32_t done;  // the loop stopped before this iteration
 };
 
-// Per found mode and trial: what BinInFreq subtracts from bin `bin` of row
-// 0; rows 2b+1 / 2b+2 take base*g_b / base*conj(g_b).
-struct ModeTerm {
-  int64_t k1;
-  int32_t bin;
+struct LoopCounters {
+  int32_t it;     // iteration index (-1 before the first)
+  int32_t stale;  // iterations that found nothing (src/demo1.cc:55-57)
+  int32_t done;
   int32_t pad;
-  double2 base;
+};
+
+// Device-resident loop state.
+struct DevState {
+  const void *x;  // input signal (double2 or float2)
+  // fixed for a call
+  int32_t mode, trials, k, min_bins, max_stale, max_iter, bits_override, B_fixed;
+  double win_thr, bin_thr_fixed, sv_thr_fixed;
+  // updated during a call
+  int32_t count;          // number of distinct found keys
+  int32_t flags;
+  int32_t found_iter[2];  // per parity: this iteration inserted something
+  int32_t iters;          // iterations run when the loop stopped
+  int32_t stop_stale;     // stale count when the loop stopped
+  LoopCounters L[2];
+  IterParams P[2];
 };
 
 // Everything a kernel needs; passed by value.
@@ -93,15 +109,16 @@ struct Ctx {
   int64_t mask;          // n-1 if n is a power of two, else -1
   double2 *fold;         // trials_cap x rows_cap x B_cap
   double2 *A;            // trials_cap x rows_cap x B_cap (bin coefficients)
-  ModeTerm *terms;       // trials_cap x cap
+  int32_t *term_bin;     // trials_cap x cap: bin of each found mode
+  double2 *term_val;     // trials_cap x rows_cap x cap: BinInFreq terms
   int64_t *keys;         // cap
   double2 *coef;         // n (dense found coefficients)
   uint32_t *present;     // ceil(n/32) bits
   int32_t *log;          // 2 x log_cap
+  int32_t *progress;     // host-mapped {iterations done, done flag}, or null
   int32_t log_cap;
   int32_t cap;
   int32_t trials_cap, rows_cap, B_cap;
-  int32_t len_cap, f_cap;  // k_bin_time LDS layout
   int32_t mixed;
 };
 
@@ -140,7 +157,7 @@ class Engine {
   int EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s);
   int GrowTransforms(int32_t count, bool *grown);
   Ctx MakeCtx() const;
-  int LaunchIteration(const Ctx &c, hipStream_t s);
+  int LaunchIteration(const Ctx &c, int parity, hipStream_t s);
   int EnsureGraph(int32_t iters, hipStream_t s);
   int ReadState(hipStream_t s);
   size_t LdsBinTime() const;
@@ -162,9 +179,10 @@ class Engine {
   double tab_sigma_ = -1;
   int32_t tab_bits_override_ = 0;
 
-  int32_t trials_cap_ = 0, rows_cap_ = 0, B_cap_ = 0, len_cap_ = 0, f_cap_ = 0;
+  int32_t trials_cap_ = 0, rows_cap_ = 0, B_cap_ = 0;
   int32_t tiles_cap_ = 0;  // k_bin_time tiles per trial (max over B)
   int32_t M_cap_ = 0;
+  size_t lds_bin_time_ = 0, lds_fft_ = 0;
 
   // device memory
   DevState *d_state_ = nullptr;
@@ -183,13 +201,16 @@ class Engine {
   uint32_t tf_seed_ = 0;
   double2 *d_fold_ = nullptr;
   double2 *d_A_ = nullptr;
-  ModeTerm *d_terms_ = nullptr;
+  int32_t *d_term_bin_ = nullptr;
+  double2 *d_term_val_ = nullptr;
   int64_t *d_keys_ = nullptr;
   double2 *d_coef_ = nullptr;
   uint32_t *d_present_ = nullptr;
   int32_t *d_log_ = nullptr;
   int32_t log_cap_ = 0;
   DevState *h_state_ = nullptr;  // pinned mirror of d_state_
+  volatile int32_t *h_progress_ = nullptr;  // pinned, written by k_bin_time
+  int32_t *d_progress_ = nullptr;           // device address of h_progress_
   double2 *A_override_ = nullptr;
   double2 *d_tmp_ = nullptr;     // cap double2 staging for mode I/O
   int64_t *d_tmpk_ = nullptr;    // cap int64 staging for mode I/O

@@ -9,11 +9,10 @@ namespace mpsft {
 struct InitArgs {
   const void *x;
   int32_t mode;
-  int32_t B;          // first iteration's bin count
+  int32_t B;          // bins for Iterate / BinInTime (the solve derives its own)
   int32_t trials;
   int32_t k, min_bins, max_stale, max_iter;
   int32_t bits_override;  // > 0: use this bits instead of the table's
-  int32_t use_table_thr;  // 1: thresholds from the B table (solve)
   int32_t keep_modes;     // 1: keep the found-mode map (Iterate)
   double bin_thr, sv_thr, win_thr;
 };
@@ -24,9 +23,10 @@ hipError_t LaunchBuildTables(const int32_t *blist, int32_t nb, const BParams *bt
                              hipStream_t s);
 hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s);
 hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s);
-hipError_t LaunchBinTime(const Ctx &c, int32_t tiles, size_t lds, hipStream_t s);
-hipError_t LaunchBinFft(const Ctx &c, size_t lds, hipStream_t s);
-hipError_t LaunchIdentify(const Ctx &c, int32_t blocks, hipStream_t s);
+// parity = iteration & 1: the IterParams slot the kernels use.
+hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s);
+hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);
+hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

@@ -1,12 +1,14 @@
 // kernels.hip -- HIP kernels of the MPSFT recovery loop for gfx950 (CDNA4).
 //
 // Layout in HBM (all row-major, interleaved complex):
-//   x      : n samples, double2 (FP64) or float2 (MIXED), read by gathers
-//   fold   : [trial][row][m]  folded, windowed samples, m < B (double2)
-//   A      : [trial][row][b]  bin coefficients after BinInFreq (double2)
-//   coef   : n double2, dense coefficients of the found modes
-//   present: n bits, membership of the found-mode map
-//   keys   : found-mode locations in insertion order
+//   x        : n samples, double2 (FP64) or float2 (MIXED), read by gathers
+//   fold     : [trial][row][m]  folded, windowed samples, m < B (double2)
+//   A        : [trial][row][b]  bin coefficients after BinInFreq (double2)
+//   term_bin : [trial][mode]    bin a found mode falls in this iteration
+//   term_val : [trial][row][mode] what BinInFreq subtracts there
+//   coef     : n double2, dense coefficients of the found modes
+//   present  : n bits, membership of the found-mode map
+//   keys     : found-mode locations in insertion order
 // Nothing here is a dense contraction, so nothing uses MFMA: the gather is
 // bound by random-access memory traffic, the bin FFT by LDS latency and the
 // identify/estimate by transcendental latency.
@@ -18,6 +20,7 @@ namespace mpsft {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxM = 1 << kMaxLogM;
 
 __device__ __forceinline__ int64_t ModN(int64_t v, int64_t n, int64_t mask) {
   if (mask >= 0) return v & mask;
@@ -38,6 +41,9 @@ __device__ __forceinline__ double2 Sub(double2 a, double2 b) {
 __device__ __forceinline__ double2 Scale(double2 a, double s) {
   return make_double2(a.x * s, a.y * s);
 }
+// a * (-i) and a * (+i)
+__device__ __forceinline__ double2 MulNegI(double2 a) { return make_double2(a.y, -a.x); }
+__device__ __forceinline__ double2 MulPosI(double2 a) { return make_double2(-a.y, a.x); }
 
 // e^{2 pi i turns}; sincospi reduces its argument exactly.
 __device__ __forceinline__ double2 CisTurns(double turns) {
@@ -49,12 +55,6 @@ __device__ __forceinline__ double2 CisTurns(double turns) {
 // (src/sincos.h:42-45).
 __device__ __forceinline__ double2 CisFrac(int64_t r, int64_t n) {
   return CisTurns(double(r) / double(n));
-}
-
-__device__ __forceinline__ double LoadRe(const double2 *x, int64_t j, double *im) {
-  const double2 v = x[j];
-  *im = v.y;
-  return v.x;
 }
 
 template <typename Tx>
@@ -110,44 +110,94 @@ __device__ __forceinline__ int64_t RowOffset(int row) {
   return -(int64_t(1) << (row / 2 - 1));
 }
 
-// In-place radix-2 FFTs over M = 2^logM points in LDS.  tw[k] =
-// e^{-2 pi i k / 2^kMaxLogM}.  Dit: bit-reversed in, natural out.
-// Dif: natural in, bit-reversed out.  inverse flips the sign.
-__device__ void FftDit(double2 *a, int logM, bool inverse, const double2 *tw) {
+// ---------------------------------------------------------------------------
+// Power-of-two FFTs in LDS, radix 4 (two radix-2 stages per pass, held in
+// registers) with a final radix-2 stage when logM is odd.  Twiddles come
+// from a table w[j] = e^{-2 pi i j / M}, j < M/2, read as tw[j * tstride].
+struct Twiddles {
+  const double2 *tw;
+  int stride;
+  bool inverse;
+  __device__ __forceinline__ double2 operator()(int j) const {
+    double2 w = tw[j * stride];
+    if (inverse) w.y = -w.y;
+    return w;
+  }
+};
+
+// Decimation in time: bit-reversed input, natural-order output.
+__device__ void FftDit(double2 *a, int logM, const Twiddles &W) {
   const int M = 1 << logM;
-  for (int s = 1; s <= logM; ++s) {
+  int s = 1;
+  for (; s + 1 <= logM; s += 2) {
     const int half = 1 << (s - 1);
-    const int tstride = 1 << (kMaxLogM - s);
+    const int sh1 = logM - s, sh2 = logM - s - 1;  // W_len^k = W_M^(k << sh)
+    for (int idx = threadIdx.x; idx < M / 4; idx += blockDim.x) {
+      const int k = idx & (half - 1);
+      const int base = ((idx >> (s - 1)) << (s + 1)) + k;
+      double2 e0 = a[base], e1 = a[base + half], e2 = a[base + 2 * half], e3 = a[base + 3 * half];
+      const double2 w1 = W(k << sh1);
+      const double2 w2 = W(k << sh2);
+      const double2 w3 = W.inverse ? MulPosI(w2) : MulNegI(w2);  // W_len2^(k + half)
+      double2 t = Cmul(e1, w1);
+      e1 = Sub(e0, t);
+      e0 = Add(e0, t);
+      t = Cmul(e3, w1);
+      e3 = Sub(e2, t);
+      e2 = Add(e2, t);
+      t = Cmul(e2, w2);
+      a[base + 2 * half] = Sub(e0, t);
+      a[base] = Add(e0, t);
+      t = Cmul(e3, w3);
+      a[base + 3 * half] = Sub(e1, t);
+      a[base + half] = Add(e1, t);
+    }
+    __syncthreads();
+  }
+  if (s == logM) {  // last radix-2 stage
+    const int half = 1 << (s - 1);
     for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
       const int k = idx & (half - 1);
       const int i = ((idx >> (s - 1)) << s) + k;
-      const int j = i + half;
-      double2 w = tw[k * tstride];
-      if (inverse) w.y = -w.y;
-      const double2 u = a[i];
-      const double2 v = Cmul(a[j], w);
+      const double2 u = a[i], v = Cmul(a[i + half], W(k));
       a[i] = Add(u, v);
-      a[j] = Sub(u, v);
+      a[i + half] = Sub(u, v);
     }
     __syncthreads();
   }
 }
 
-__device__ void FftDif(double2 *a, int logM, bool inverse, const double2 *tw) {
+// Decimation in frequency: natural input, bit-reversed output.
+__device__ void FftDif(double2 *a, int logM, const Twiddles &W) {
   const int M = 1 << logM;
-  for (int s = logM; s >= 1; --s) {
-    const int half = 1 << (s - 1);
-    const int tstride = 1 << (kMaxLogM - s);
-    for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
+  int s = logM;
+  for (; s - 1 >= 1; s -= 2) {
+    // stages s (len2 = 2^s) then s-1 (len = 2^(s-1)), half = 2^(s-2)
+    const int half = 1 << (s - 2);
+    const int sh2 = logM - s, sh1 = logM - s + 1;
+    for (int idx = threadIdx.x; idx < M / 4; idx += blockDim.x) {
       const int k = idx & (half - 1);
-      const int i = ((idx >> (s - 1)) << s) + k;
-      const int j = i + half;
-      double2 w = tw[k * tstride];
-      if (inverse) w.y = -w.y;
-      const double2 u = a[i];
-      const double2 v = a[j];
+      const int base = ((idx >> (s - 2)) << s) + k;
+      const double2 e0 = a[base], e1 = a[base + half], e2 = a[base + 2 * half],
+                    e3 = a[base + 3 * half];
+      const double2 w2 = W(k << sh2);
+      const double2 w3 = W.inverse ? MulPosI(w2) : MulNegI(w2);
+      const double2 w1 = W(k << sh1);
+      const double2 f0 = Add(e0, e2), f2 = Cmul(Sub(e0, e2), w2);
+      const double2 f1 = Add(e1, e3), f3 = Cmul(Sub(e1, e3), w3);
+      a[base] = Add(f0, f1);
+      a[base + half] = Cmul(Sub(f0, f1), w1);
+      a[base + 2 * half] = Add(f2, f3);
+      a[base + 3 * half] = Cmul(Sub(f2, f3), w1);
+    }
+    __syncthreads();
+  }
+  if (s == 1) {
+    for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
+      const int i = 2 * idx;
+      const double2 u = a[i], v = a[i + 1];
       a[i] = Add(u, v);
-      a[j] = Cmul(Sub(u, v), w);
+      a[i + 1] = Sub(u, v);
     }
     __syncthreads();
   }
@@ -164,10 +214,10 @@ __device__ __forceinline__ double2 Chirp(int64_t m, int32_t B) {
 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) k_build_twiddles(double2 *tw) {
-  const int half = 1 << (kMaxLogM - 1);
+  const int half = kMaxM / 2;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < half; k += gridDim.x * blockDim.x) {
     double s, c;
-    sincospi(-2.0 * double(k) / double(1 << kMaxLogM), &s, &c);
+    sincospi(-2.0 * double(k) / double(kMaxM), &s, &c);
     tw[k] = make_double2(c, s);
   }
 }
@@ -193,7 +243,7 @@ __global__ void __launch_bounds__(kThreads)
     lds[BitRev(j, bp.logM)] = v;
   }
   __syncthreads();
-  FftDit(lds, bp.logM, false, tw);
+  FftDit(lds, bp.logM, Twiddles{tw, kMaxM / M, false});
   const double inv = 1.0 / double(M);
   for (int k = threadIdx.x; k < M; k += blockDim.x) filt[k] = Scale(lds[k], inv);
 }
@@ -217,21 +267,24 @@ __global__ void __launch_bounds__(kThreads) k_clear_modes(Ctx c, int32_t full) {
   }
 }
 
-__device__ void LoadBParams(DevState *st, const BParams &bp, int32_t bits_override) {
-  st->B = bp.B;
-  st->bits = bits_override > 0 ? bits_override : bp.bits;
-  st->rows = 1 + 2 * st->bits;
-  st->f = bp.f;
-  st->P = bp.P;
-  st->p2 = (bp.P - 1) / 2;
-  st->M = bp.M;
-  st->logM = bp.logM;
-  st->tab_off = bp.off;
-  st->width = bp.width;
-  st->sigma_f = bp.sigma_f;
-  st->dfreq = bp.dfreq;
+__device__ __forceinline__ void FillParams(IterParams *ip, const BParams &bp, int32_t bits) {
+  ip->B = bp.B;
+  ip->bits = bits;
+  ip->rows = 1 + 2 * bits;
+  ip->f = bp.f;
+  ip->P = bp.P;
+  ip->p2 = (bp.P - 1) / 2;
+  ip->M = bp.M;
+  ip->logM = bp.logM;
+  ip->R = bp.R;
+  ip->tab_off = bp.off;
+  ip->width = bp.width;
+  ip->sigma_f = bp.sigma_f;
+  ip->dfreq = bp.dfreq;
 }
 
+// Fixed per-call settings; iteration -1 sits in slot 1 as "found", so the
+// first iteration (slot 0) starts with stale = 0.
 __global__ void k_init(Ctx c, InitArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevState *st = c.st;
@@ -242,26 +295,99 @@ __global__ void k_init(Ctx c, InitArgs a) {
   st->min_bins = a.min_bins;
   st->max_stale = a.max_stale;
   st->max_iter = a.max_iter;
-  st->it = 0;
-  st->stale = 0;
-  st->done = 0;
+  st->bits_override = a.bits_override;
+  st->B_fixed = a.B;
+  st->win_thr = a.win_thr;
+  st->bin_thr_fixed = a.bin_thr;
+  st->sv_thr_fixed = a.sv_thr;
   if (!a.keep_modes) {
     st->count = 0;
     st->flags = 0;
   }
-  st->found_iter = 0;
-  st->found_last = 0;
-  st->blocks_done = 0u;
-  const BParams bp = c.btab[a.B];
-  LoadBParams(st, bp, a.bits_override);
-  if (a.use_table_thr) {
-    st->bin_thr = bp.bin_thr;
-    st->sv_thr = bp.sv_thr;
-  } else {
-    st->bin_thr = a.bin_thr;
-    st->sv_thr = a.sv_thr;
+  st->found_iter[0] = 0;
+  st->found_iter[1] = 1;
+  st->iters = 0;
+  st->stop_stale = 0;
+  st->L[1].it = -1;
+  st->L[1].stale = 0;
+  st->L[1].done = 0;
+  st->L[0] = st->L[1];  // no stale "stopped" from an earlier call
+  st->P[1].B = a.B;
+  st->P[1].done = 0;
+  if (c.progress) {
+    __hip_atomic_store(&c.progress[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c.progress[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  st->win_thr = a.win_thr;
+}
+
+// The next iteration's parameters, as Demo1::Run derives them
+// (src/demo1.cc:45-58): stale += (previous iteration found nothing); stop
+// at max_stale or max_iter; otherwise B = max((k - |found|)*2 + 1, min_bins)
+// with thresholds 3 sigma/sqrt(B), sigma/sqrt(B), and the iteration's
+// transform and q's.  Reads only state that no kernel of this iteration
+// writes before k_bin_time ends, so every workgroup computes the same.
+__device__ void NextIteration(const Ctx &c, int p, IterParams *ip, LoopCounters *lc) {
+  const DevState *st = c.st;
+  const LoopCounters prev = st->L[1 - p];
+  if (prev.done) {
+    *lc = prev;
+    ip->done = 1;
+    ip->it = prev.it;
+    return;
+  }
+  lc->it = prev.it + 1;
+  lc->stale = prev.stale + (st->found_iter[1 - p] ? 0 : 1);
+  lc->pad = 0;
+  const bool done = (st->flags & kFlagOverflow) || lc->stale >= st->max_stale ||
+                    lc->it >= st->max_iter;
+  lc->done = done ? 1 : 0;
+  ip->done = lc->done;
+  ip->it = lc->it;
+  if (done) return;
+  int32_t B = st->B_fixed;
+  if (st->mode == kModeSolve) {
+    const int64_t want = (int64_t(st->k) - int64_t(st->count)) * 2 + 1;
+    B = int32_t(want > st->min_bins ? want : st->min_bins);
+  }
+  const BParams bp = c.btab[B];
+  FillParams(ip, bp, st->bits_override > 0 ? st->bits_override : bp.bits);
+  if (st->mode == kModeSolve) {
+    ip->bin_thr = bp.bin_thr;
+    ip->sv_thr = bp.sv_thr;
+  } else {
+    ip->bin_thr = st->bin_thr_fixed;
+    ip->sv_thr = st->sv_thr_fixed;
+  }
+  ip->tf = c.tfs[lc->it];
+  // ip->q is filled by PublishIteration only: workgroups load the q of
+  // their own trial, keeping the struct out of private memory.
+}
+
+// Workgroup 0 of k_bin_time: publish the iteration, log the previous one,
+// report progress to the host.
+__device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
+                                 const LoopCounters &lc) {
+  DevState *st = c.st;
+  const LoopCounters prev = st->L[1 - p];
+  st->P[p] = ip;
+  if (!ip.done)
+    for (int t = 0; t < st->trials; ++t)
+      st->P[p].q[t] = c.qs[int64_t(lc.it) * c.trials_cap + t];
+  st->L[p] = lc;
+  st->found_iter[p] = 0;
+  if (prev.done) return;
+  if (st->mode == kModeSolve && prev.it >= 0 && prev.it < c.log_cap) {
+    c.log[prev.it] = st->P[1 - p].B;
+    c.log[c.log_cap + prev.it] = st->count;
+  }
+  if (lc.done) {
+    st->iters = lc.it;
+    st->stop_stale = lc.stale;
+  }
+  if (c.progress) {
+    __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c.progress[1], lc.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // Inserts keys/coefs into an empty map (ModeMap assignment).
@@ -284,241 +410,315 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---------------------------------------------------------------------------
+// Lag layout shared by every row of an iteration.  Row `row` reads samples
+// u = q - p2 + lambda*B + rho for lambda = off(row) + k, k < f.  Rows with
+// 2^k < f overlap row 0: their lambdas form one run [-2^(K-1), 2^(K-1) + f)
+// of len0 lags ("segment 0"); every other row owns a run of f lags.  The
+// union, nlam lags, is staged once per residue.
+struct Lags {
+  int K, len0, nlam, f;
+  __device__ Lags(int bits, int f_) : f(f_) {
+    K = 0;
+    while (K < bits && (1 << K) < f) ++K;
+    len0 = (K ? (1 << K) : 0) + f;
+    nlam = len0 + 2 * (bits - K) * f;
+  }
+  // lambda of staged index li
+  __device__ __forceinline__ int64_t Lambda(int li) const {
+    if (li < len0) return K ? li - (1 << (K - 1)) : li;
+    const int h = li - len0;
+    const int seg = h / f;
+    const int kk = K + seg / 2;
+    const int64_t lo = (seg & 1) ? -(int64_t(1) << kk) : (int64_t(1) << kk);
+    return lo + (h - seg * f);
+  }
+  // staged index of the first lag of row `row`
+  __device__ __forceinline__ int RowStart(int row) const {
+    const int kk = row == 0 ? -1 : (row - 1) / 2;
+    if (kk < K) return (K ? (1 << (K - 1)) : 0) + int(RowOffset(row));
+    const int seg = 2 * (kk - K) + ((row & 1) ? 0 : 1);
+    return len0 + seg * f;
+  }
+};
+
 // k_bin_time: the permuted, windowed gather-fold of BinInTime
-// (src/binner.cc:81-111 semantics).  With s = t + tau - (q - p2) the row
-// offset, every row's samples sit at u = q - p2 + lambda*B + rho, and the
-// row with offset off (in units of B) folds
+// (src/binner.cc:81-111 semantics), without its DFT.  Factoring the row's
+// own phase e^{2 pi i b tau/n} out (applied after the DFT), row `row` folds
 //   fold[row][m(rho)] = sum_{k<f} x[(a u + c) mod n] * tap(k*B + rho - p2),
-//   u = q - p2 + (off + k)*B + rho,  m(rho) = (rho - (B-1)/2) mod B,
-// where tap(t) = wt(t) e^{2 pi i [(b t mod n)/n + frac(-t/(2B))]}; the row's
-// own phase e^{2 pi i b tau/n} is applied after the DFT.  Rows with
-// 2^k < f overlap row 0; they form segment 0 and share one staged copy of
-// each sample (lambda in [-2^(K-1), 2^(K-1)+f)).  Every other row is its own
-// segment.  A tile = (trial, segment, block of kResBlock residues rho):
-//   phase A gathers the segment's samples for the block into LDS,
-//   phase T computes the block's f taps into LDS,
-//   phase B sums per (row, rho) and writes the fold.
-// Workgroups below kPrepBlocks instead compute the BinInFreq mode terms.
+//   u = q - p2 + (off(row) + k)*B + rho,   m(rho) = (rho - (B-1)/2) mod B,
+// tap(t) = wt(t) e^{2 pi i [(b t mod n)/n + frac(-t/(2B))]}.
+// A tile = (trial, block of R residues rho) covers every row:
+//   phase A gathers the nlam x R distinct samples into LDS (each once),
+//   phase T computes the f x R taps into LDS (each once per iteration),
+//   phase B forms every (row, rho) sum, split over `parts` threads.
+// Workgroups below kPrepBlocks instead compute the BinInFreq terms of every
+// found mode for this iteration.
 template <typename Tx>
-__global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c) {
+__global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
   extern __shared__ double2 lds[];
   const DevState *st = c.st;
-  if (st->done) return;
-  const int32_t it = st->it;
+  IterParams ip;
+  LoopCounters lc;
+  NextIteration(c, p, &ip, &lc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) PublishIteration(c, p, ip, lc);
+  if (ip.done) return;
   const int32_t trials = st->trials;
-  const int32_t B = st->B, f = st->f, bits = st->bits, p2 = st->p2;
-  const DevTf tf = c.tfs[it];
+  const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
+  const DevTf tf = ip.tf;
   const int64_t n = c.n, mask = c.mask;
 
   if (blockIdx.x < kPrepBlocks) {
-    // BinInFreqV1 terms (src/binner.cc:565-591) for every found mode.
+    // BinInFreqV1 (src/binner.cc:565-591) per found mode and trial: bin,
+    // base = wf * coef * e^{2 pi i (q k1 + c k0)/n}, and per bit
+    // g = e^{2 pi i (B 2^bit k1)/n} for rows 2bit+1 (base g) and 2bit+2
+    // (base conj(g)).
     const int32_t cnt = min(st->count, c.cap);
-    const double width = st->width, dfreq = st->dfreq;
+    const double width = ip.width, dfreq = ip.dfreq;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
          i += int64_t(kPrepBlocks) * blockDim.x) {
       const int32_t trial = int32_t(i / cnt);
       const int32_t mi = int32_t(i - int64_t(trial) * cnt);
       const int64_t k0 = c.keys[mi];
       const double2 cf = c.coef[k0];
-      const int64_t q = c.qs[int64_t(it) * c.trials_cap + trial];
+      const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
       const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
       const int32_t bin = int32_t(k1 * B / n);
       const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
       const double wf = SampleInFreq(xi, width, dfreq);
-      const double2 ph = CisFrac(ModN(q * k1 + tf.c * k0, n, mask), n);
-      ModeTerm t;
-      t.k1 = k1;
-      t.bin = bin;
-      t.pad = 0;
-      t.base = Cmul(Scale(cf, wf), ph);
-      c.terms[int64_t(trial) * c.cap + mi] = t;
+      const double2 base = Cmul(Scale(cf, wf), CisFrac(ModN(q * k1 + tf.c * k0, n, mask), n));
+      c.term_bin[int64_t(trial) * c.cap + mi] = bin;
+      double2 *val = c.term_val + int64_t(trial) * c.rows_cap * c.cap + mi;
+      val[0] = base;
+      for (int32_t bit = 0; bit < bits; ++bit) {
+        const double2 g = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
+        val[int64_t(2 * bit + 1) * c.cap] = Cmul(base, g);
+        val[int64_t(2 * bit + 2) * c.cap] = Cmul(base, Conj(g));
+      }
     }
     return;
   }
 
+  const int32_t R = ip.R;
+  const int32_t nblk = (B + R - 1) / R;
   int32_t tile = blockIdx.x - kPrepBlocks;
-  int K = 0;
-  while (K < bits && (1 << K) < f) ++K;
-  const int32_t nseg = 1 + 2 * (bits - K);
-  const int32_t nblk = (B + kResBlock - 1) / kResBlock;
-  const int32_t per_trial = nseg * nblk;
-  if (tile >= trials * per_trial) return;
-  const int32_t trial = tile / per_trial;
-  tile -= trial * per_trial;
-  const int32_t seg = tile / nblk;
-  const int32_t blk = tile - seg * nblk;
-
-  int32_t lam_lo, len, nrows, row1;
-  if (seg == 0) {
-    lam_lo = K ? -(1 << (K - 1)) : 0;
-    len = (K ? (1 << K) : 0) + f;
-    nrows = 1 + 2 * K;
-    row1 = 0;
-  } else {
-    const int32_t kk = K + (seg - 1) / 2;
-    const bool neg = ((seg - 1) & 1) != 0;
-    lam_lo = neg ? -(1 << kk) : (1 << kk);
-    len = f;
-    nrows = 1;
-    row1 = neg ? 2 * kk + 2 : 2 * kk + 1;
-  }
-  const int32_t rho0 = blk * kResBlock;
-  const int32_t nrho = min(kResBlock, B - rho0);
-  const int64_t q = c.qs[int64_t(it) * c.trials_cap + trial];
+  if (tile >= trials * nblk) return;
+  const int32_t trial = tile / nblk;
+  const int32_t rho0 = (tile - trial * nblk) * R;
+  const int32_t nrho = min(R, B - rho0);
+  const Lags L(bits, f);
+  const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
   const Tx *x = static_cast<const Tx *>(st->x);
 
-  double2 *X = lds;                                   // [len_cap][kResBlock]
-  double2 *T = lds + c.len_cap * kResBlock;           // [f_cap][kResBlock]
-  double2 *red = T + c.f_cap * kResBlock;             // [kThreads]
+  double2 *X = lds;               // [nlam][R]
+  double2 *T = lds + L.nlam * R;  // [f][R]
+  double2 *red = T + f * R;       // [kThreads]
 
-  // Phase A: gather.
-  const int64_t ubase = q - p2 + int64_t(lam_lo) * B + rho0;
-  for (int idx = threadIdx.x; idx < len * nrho; idx += blockDim.x) {
-    const int li = idx / nrho;
-    const int ri = idx - li * nrho;
-    const int64_t u = ubase + int64_t(li) * B + ri;
-    const int64_t j = ModN(tf.a * u + tf.c, n, mask);
-    X[li * kResBlock + ri] = LoadX<Tx>(x, j);
+  // Phase A: gather every distinct sample of the tile once.
+  const int64_t ubase = q - p2 + rho0;
+  for (int idx = threadIdx.x; idx < L.nlam * R; idx += blockDim.x) {
+    const int li = idx / R;
+    const int ri = idx - li * R;
+    if (ri < nrho) {
+      const int64_t u = ubase + L.Lambda(li) * B + ri;
+      X[idx] = LoadX<Tx>(x, ModN(tf.a * u + tf.c, n, mask));
+    }
   }
-  // Phase T: taps tap(t), t = k*B + rho - p2.
+  // Phase T: taps tap(t), t = k*B + rho - p2 (Window::SampleInTime and the
+  // phase of src/binner.cc:100-104 with tau factored out).
   {
-    const double width = st->width, sigma_f = st->sigma_f;
+    const double width = ip.width, sigma_f = ip.sigma_f;
     const double delta = -0.5 / double(B);
-    for (int idx = threadIdx.x; idx < f * nrho; idx += blockDim.x) {
-      const int k = idx / nrho;
-      const int ri = idx - k * nrho;
-      const int64_t t = int64_t(k) * B + rho0 + ri - p2;
-      const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
-      const int64_t r = ModN(tf.b * t, n, mask);
-      const double d = delta * double(t);
-      const double turns = double(r) / double(n) + (d - floor(d));
-      T[k * kResBlock + ri] = Scale(CisTurns(turns), w);
+    for (int idx = threadIdx.x; idx < f * R; idx += blockDim.x) {
+      const int k = idx / R;
+      const int ri = idx - k * R;
+      if (ri < nrho) {
+        const int64_t t = int64_t(k) * B + rho0 + ri - p2;
+        const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
+        const int64_t r = ModN(tf.b * t, n, mask);
+        const double d = delta * double(t);
+        const double turns = double(r) / double(n) + (d - floor(d));
+        T[idx] = Scale(CisTurns(turns), w);
+      }
     }
   }
   __syncthreads();
 
-  // Phase B: per (row, rho) sums over k, split in `parts` when few pairs.
-  const int32_t pairs = nrows * nrho;
+  // Phase B: per (row, rho) sums over k, split in `parts` when pairs are few.
+  const int32_t pairs = rows * nrho;
   int32_t parts = 1;
   while (parts * 2 * pairs <= kThreads && parts * 2 <= f) parts *= 2;
   const int32_t h = (B - 1) / 2;
   double2 *fold_t = c.fold + int64_t(trial) * c.rows_cap * c.B_cap;
-  if (parts == 1) {
-    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
-      const int rr = p / nrho;
-      const int ri = p - rr * nrho;
-      const int row = row1 + rr;
-      const int off = int(RowOffset(row)) - lam_lo;
-      double2 acc = make_double2(0.0, 0.0);
-      for (int k = 0; k < f; ++k) acc = Add(acc, Cmul(X[(off + k) * kResBlock + ri], T[k * kResBlock + ri]));
-      const int rho = rho0 + ri;
-      const int m = rho >= h ? rho - h : rho - h + B;
-      fold_t[int64_t(row) * c.B_cap + m] = acc;
-    }
-  } else {
-    const int p = threadIdx.x;
+  for (int base = 0; base < pairs * parts; base += blockDim.x) {
+    const int p = base + threadIdx.x;
     const int pair = p / parts;
     const int part = p - pair * parts;
     double2 acc = make_double2(0.0, 0.0);
     int row = 0, ri = 0;
     if (pair < pairs) {
-      const int rr = pair / nrho;
-      ri = pair - rr * nrho;
-      row = row1 + rr;
-      const int off = int(RowOffset(row)) - lam_lo;
+      row = pair / nrho;
+      ri = pair - row * nrho;
+      const int s0 = L.RowStart(row);
       for (int k = part; k < f; k += parts)
-        acc = Add(acc, Cmul(X[(off + k) * kResBlock + ri], T[k * kResBlock + ri]));
+        acc = Add(acc, Cmul(X[(s0 + k) * R + ri], T[k * R + ri]));
     }
-    red[p] = acc;
-    __syncthreads();
+    if (parts > 1) {
+      red[threadIdx.x] = acc;
+      __syncthreads();
+      if (pair < pairs && part == 0)
+        for (int j = 1; j < parts; ++j) acc = Add(acc, red[threadIdx.x + j]);
+      __syncthreads();
+    }
     if (pair < pairs && part == 0) {
-      double2 s = red[p];
-      for (int j = 1; j < parts; ++j) s = Add(s, red[p + j]);
       const int rho = rho0 + ri;
       const int m = rho >= h ? rho - h : rho - h + B;
-      fold_t[int64_t(row) * c.B_cap + m] = s;
+      fold_t[int64_t(row) * c.B_cap + m] = acc;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 // k_bin_fft: per (trial, row) the forward B-point DFT of the fold by
-// Bluestein (X_k = c_k sum_j (v_j c_j) conj(c_{k-j})), an LDS FFT of
-// M = 2^logM >= 2B-1 points each way; then the row phase e^{2 pi i b tau/n}
-// and the BinInFreqV1 subtraction (src/binner.cc:565-591) of every found
-// mode that falls in this row.
-__global__ void __launch_bounds__(kThreads) k_bin_fft(Ctx c) {
+// Bluestein (X_k = c_k sum_j (v_j c_j) conj(c_{k-j})) over two LDS FFTs of
+// M = 2^logM >= 2B-1 points, then the row phase e^{2 pi i b tau/n} and the
+// BinInFreqV1 subtraction (src/binner.cc:565-591) of every found mode
+// (terms from k_bin_time).  The twiddle table is staged in LDS when
+// M <= kLdsTwMaxM.
+__global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   extern __shared__ double2 lds[];
+  constexpr int T = kFftThreads;
+  constexpr int kPerB = kMaxBins / T;            // fold / chirp entries per thread
+  constexpr int kPerM = kMaxM / T;               // filter entries per thread
+  constexpr int kPerTw = (kLdsTwMaxM / 2) / T;   // twiddles staged per thread
+  constexpr int kPerTerm = 2;                    // BinInFreq terms prefetched per thread
   const DevState *st = c.st;
-  if (st->done) return;
-  const int32_t trials = st->trials, rows = st->rows;
+  const IterParams &ip = st->P[p];
+  if (ip.done) return;
+  const int32_t trials = st->trials, rows = ip.rows;
   const int32_t trial = blockIdx.x / c.rows_cap;
   const int32_t row = blockIdx.x - trial * c.rows_cap;
   if (trial >= trials || row >= rows) return;
-  const int32_t B = st->B, logM = st->logM;
+  const int32_t B = ip.B, logM = ip.logM;
   const int M = 1 << logM;
-  const int32_t it = st->it;
-  const DevTf tf = c.tfs[it];
+  const int tid = threadIdx.x;
   const int64_t n = c.n, mask = c.mask;
-  const double2 *chirp = c.table + st->tab_off;
+  const double2 *chirp = c.table + ip.tab_off;
   const double2 *filt = chirp + B;
   const double2 *fold = c.fold + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-
-  for (int m = threadIdx.x; m < M; m += blockDim.x) {
-    double2 v = make_double2(0.0, 0.0);
-    if (m < B) v = Cmul(fold[m], chirp[m]);
-    lds[BitRev(m, logM)] = v;
-  }
-  __syncthreads();
-  FftDit(lds, logM, false, c.tw);
-  for (int k = threadIdx.x; k < M; k += blockDim.x) lds[k] = Cmul(lds[k], filt[k]);
-  __syncthreads();
-  FftDif(lds, logM, true, c.tw);
-
-  const int64_t q = c.qs[int64_t(it) * c.trials_cap + trial];
-  const int64_t tau = ModN(q + RowOffset(row) * B, n, mask);
-  const double2 rp = CisFrac(ModN(tf.b * tau, n, mask), n);
-  constexpr int kPer = (1 << kMaxLogM) / 2 / kThreads + 1;  // >= ceil(B/256)
-  double2 vals[kPer];
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int k = threadIdx.x + r * kThreads;
-    if (k < B) vals[r] = Cmul(Cmul(lds[BitRev(k, logM)], chirp[k]), rp);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int k = threadIdx.x + r * kThreads;
-    if (k < B) lds[k] = vals[r];
-  }
-  __syncthreads();
-
   const int32_t cnt = min(st->count, c.cap);
-  const ModeTerm *terms = c.terms + int64_t(trial) * c.cap;
-  const int64_t off = RowOffset(row) < 0 ? -RowOffset(row) : RowOffset(row);
-  for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const ModeTerm t = terms[i];
-    double2 contrib = t.base;
-    if (row != 0) {
-      double2 g = CisFrac(ModN(off * B * t.k1, n, mask), n);
-      if ((row & 1) == 0) g = Conj(g);
-      contrib = Cmul(t.base, g);
+  const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
+  const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
+  const bool tw_lds = M <= kLdsTwMaxM;
+
+  // Every global load the workgroup needs is issued here, before the first
+  // barrier, so the kernel waits on memory once instead of once per stage.
+  double2 fv[kPerB], cv[kPerB], hv[kPerM], twv[kPerTw], tval[kPerTerm];
+  int tbin[kPerTerm];
+#pragma unroll
+  for (int r = 0; r < kPerB; ++r) {
+    const int k = tid + r * T;
+    fv[r] = cv[r] = make_double2(0.0, 0.0);
+    if (k < B) {
+      fv[r] = fold[k];
+      cv[r] = chirp[k];
     }
-    atomicAdd(&lds[t.bin].x, -contrib.x);
-    atomicAdd(&lds[t.bin].y, -contrib.y);
+  }
+#pragma unroll
+  for (int r = 0; r < kPerM; ++r) {
+    const int k = tid + r * T;
+    hv[r] = k < M ? filt[k] : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < kPerTw; ++r) {
+    const int j = tid + r * T;
+    twv[r] = (tw_lds && j < M / 2) ? c.tw[j * (kMaxM / M)] : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) {
+    const int i = tid + r * T;
+    tbin[r] = 0;
+    tval[r] = make_double2(0.0, 0.0);
+    if (i < cnt) {
+      tbin[r] = tb[i];
+      tval[r] = tv[i];
+    }
+  }
+  const int64_t tau = ModN(ip.q[trial] + RowOffset(row) * B, n, mask);
+  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+
+  double2 *a = lds;
+  Twiddles Wf{c.tw, kMaxM / M, false};
+  if (tw_lds) {
+    double2 *twl = lds + M;
+#pragma unroll
+    for (int r = 0; r < kPerTw; ++r) {
+      const int j = tid + r * T;
+      if (j < M / 2) twl[j] = twv[r];
+    }
+    Wf = Twiddles{twl, 1, false};
+  }
+#pragma unroll
+  for (int r = 0; r < kPerM; ++r) {
+    const int m = tid + r * T;
+    if (m < M) {
+      double2 v = make_double2(0.0, 0.0);
+      if (r < kPerB && m < B) v = Cmul(fv[r], cv[r]);
+      a[BitRev(m, logM)] = v;
+    }
+  }
+  __syncthreads();
+  FftDit(a, logM, Wf);
+#pragma unroll
+  for (int r = 0; r < kPerM; ++r) {
+    const int k = tid + r * T;
+    if (k < M) a[k] = Cmul(a[k], hv[r]);
+  }
+  __syncthreads();
+  Twiddles Wi = Wf;
+  Wi.inverse = true;
+  FftDif(a, logM, Wi);
+
+  double2 vals[kPerB];
+#pragma unroll
+  for (int r = 0; r < kPerB; ++r) {
+    const int k = tid + r * T;
+    if (k < B) vals[r] = Cmul(Cmul(a[BitRev(k, logM)], cv[r]), rp);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPerB; ++r) {
+    const int k = tid + r * T;
+    if (k < B) a[k] = vals[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) {
+    if (tid + r * T < cnt) {
+      atomicAdd(&a[tbin[r]].x, -tval[r].x);
+      atomicAdd(&a[tbin[r]].y, -tval[r].y);
+    }
+  }
+  for (int i = tid + kPerTerm * T; i < cnt; i += T) {
+    const double2 v = tv[i];
+    const int bin = tb[i];
+    atomicAdd(&a[bin].x, -v.x);
+    atomicAdd(&a[bin].y, -v.y);
   }
   __syncthreads();
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  for (int k = threadIdx.x; k < B; k += blockDim.x) A[k] = lds[k];
+  for (int k = tid; k < B; k += T) A[k] = a[k];
 }
 
 // ---------------------------------------------------------------------------
-// k_identify: per bin, Iterate's loop body (src/iterate.cc:113-175):
-// energy test, IdentifyFreq (src/iterate.cc:44-76) with the closed-form 2x2
-// pencil, k1 rounding, window test, coefficient estimate, un-permutation,
-// and mm[k0] += coef.  FP contraction is off so the decision arithmetic
-// rounds as the reference's does.  The last workgroup to finish advances
-// the loop state as Demo1::Run does (src/demo1.cc:45-58).
+// k_identify: Iterate's per-bin body (src/iterate.cc:113-175), one
+// wavefront per bin.  Each lane loads one or more (trial, row) entries of
+// the bin's column of A (one memory round trip); lanes read each other's
+// entries through cross-lane shuffles.  Then: the energy test; IdentifyFreq
+// (src/iterate.cc:44-76) with lane j voting for bit j through the
+// closed-form 2x2 pencil; k1 rounding and the window test; the coefficient
+// estimate reduced across the lanes; un-permutation and mm[k0] += coef.  FP
+// contraction is off so the decision arithmetic rounds as the reference's
+// does.
 __device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v) {
   const uint32_t bit = 1u << (k0 & 31);
   const uint32_t old = atomicOr(&c.present[k0 >> 5], bit);
@@ -533,126 +733,131 @@ __device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v) 
   atomicAdd(&c.coef[k0].y, v.y);
 }
 
-__global__ void __launch_bounds__(kThreads) k_identify(Ctx c) {
+__device__ __forceinline__ double WaveSum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Entry `src` of a column held as col[s] = entry (lane + 64 s).  Every lane
+// of the wave must call it (the shuffles read other lanes' registers).
+__device__ __forceinline__ double2 ColFetch(const double2 (&col)[kColSlices], int src) {
+  double2 r = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int s = 0; s < kColSlices; ++s) {
+    const double x = __shfl(col[s].x, src & 63);
+    const double y = __shfl(col[s].y, src & 63);
+    if ((src >> 6) == s) r = make_double2(x, y);
+  }
+  return r;
+}
+
+__global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
 #pragma clang fp contract(off)
   DevState *st = c.st;
-  if (st->done) return;
-  const int32_t B = st->B, bits = st->bits, trials = st->trials, it = st->it;
+  const IterParams &ip = st->P[p];
+  if (ip.done) return;
+  const int32_t B = ip.B, bits = ip.bits, trials = st->trials;
+  const int32_t rows = 1 + 2 * bits;
+  const int32_t npairs = trials * rows;  // <= 64 * kColSlices (host-checked)
   const int64_t n = c.n, mask = c.mask;
-  const DevTf tf = c.tfs[it];
-  const double bin_thr = st->bin_thr, sv_thr = st->sv_thr, win_thr = st->win_thr;
-  const double width = st->width, dfreq = st->dfreq;
+  const DevTf tf = ip.tf;
+  const double bin_thr = ip.bin_thr, sv_thr = ip.sv_thr, win_thr = st->win_thr;
+  const double width = ip.width, dfreq = ip.dfreq;
   const int64_t rstride = c.B_cap, tstride = int64_t(c.rows_cap) * c.B_cap;
+  const int lane = threadIdx.x & 63;
+  const int waves = blockDim.x >> 6;
   bool any = false;
 
-  for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+  for (int32_t b = blockIdx.x * waves + (threadIdx.x >> 6); b < B; b += gridDim.x * waves) {
     const double2 *Ab = c.A + b;
+    double2 col[kColSlices];
+#pragma unroll
+    for (int s = 0; s < kColSlices; ++s) {
+      const int p = lane + 64 * s;
+      col[s] = make_double2(0.0, 0.0);
+      if (p < npairs) {
+        const int t = p / rows;
+        col[s] = Ab[t * tstride + (p - t * rows) * rstride];
+      }
+    }
     if (bin_thr > 0) {
       double e = 0;
       for (int t = 0; t < trials; ++t) {
-        const double2 v = Ab[t * tstride];
+        const double2 v = ColFetch(col, t * rows);
         e += v.x * v.x + v.y * v.y;
       }
       e /= trials;
       if (e < bin_thr * bin_thr) continue;
     }
-    // IdentifyFreq: majority vote per bit, most significant bit first.
-    int32_t xi_sum = 0;
-    bool reject = false;
-    for (int32_t bit = 0; bit < bits && !reject; ++bit) {
-      int32_t count = 0;
-      for (int t = 0; t < trials; ++t) {
-        const double2 u0 = Ab[t * tstride];
-        const double2 u1 = Ab[t * tstride + (2 * bit + 1) * rstride];
-        const double2 u2 = Ab[t * tstride + (2 * bit + 2) * rstride];
-        // MatPencil (src/freqid.cc:27-48) in closed form: vote is
-        // Im(conj(u0) u2 + conj(u1) u0) > 0, sigma2 = |u0^2 - u1 u2| / sigma1.
-        const double alpha = (u0.x * u0.x + u0.y * u0.y) + (u1.x * u1.x + u1.y * u1.y);
-        const double gamma = (u2.x * u2.x + u2.y * u2.y) + (u0.x * u0.x + u0.y * u0.y);
-        const double2 beta = Add(Cmul(Conj(u0), u2), Cmul(Conj(u1), u0));
-        if (beta.y > 0) ++count;
-        if (sv_thr > 0) {
-          const double hh = 0.5 * (alpha - gamma);
-          const double lam1 =
-              0.5 * (alpha + gamma) + sqrt(hh * hh + (beta.x * beta.x + beta.y * beta.y));
-          const double s1 = sqrt(lam1);
-          const double2 det = Sub(Cmul(u0, u0), Cmul(u1, u2));
-          const double s2 = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
-          if (s2 * 0.5 > sv_thr) {
-            reject = true;
-            break;
-          }
-        }
+    // IdentifyFreq: lane j < bits takes bit j (offset B 2^j).  MatPencil
+    // (src/freqid.cc:27-48) in closed form: the vote is
+    // Im(conj(u0) u2 + conj(u1) u0) > 0 and sigma2 = |u0^2 - u1 u2|/sigma1.
+    const int jb = lane < bits ? lane : 0;
+    int32_t count = 0;
+    bool rej = false;
+    for (int t = 0; t < trials; ++t) {
+      const double2 u0 = ColFetch(col, t * rows);
+      const double2 u1 = ColFetch(col, t * rows + 2 * jb + 1);
+      const double2 u2 = ColFetch(col, t * rows + 2 * jb + 2);
+      const double alpha = (u0.x * u0.x + u0.y * u0.y) + (u1.x * u1.x + u1.y * u1.y);
+      const double gamma = (u2.x * u2.x + u2.y * u2.y) + (u0.x * u0.x + u0.y * u0.y);
+      const double2 beta = Add(Cmul(Conj(u0), u2), Cmul(Conj(u1), u0));
+      if (beta.y > 0) ++count;
+      if (sv_thr > 0) {
+        const double hh = 0.5 * (alpha - gamma);
+        const double lam1 =
+            0.5 * (alpha + gamma) + sqrt(hh * hh + (beta.x * beta.x + beta.y * beta.y));
+        const double s1 = sqrt(lam1);
+        const double2 det = Sub(Cmul(u0, u0), Cmul(u1, u2));
+        const double s2 = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
+        if (s2 * 0.5 > sv_thr) rej = true;
       }
-      xi_sum <<= 1;
-      if (count > trials / 2) ++xi_sum;
     }
-    if (reject) continue;
+    const bool vote = lane < bits && count > trials / 2;
+    if (__any(lane < bits && rej)) continue;
+    const uint64_t vm = __ballot(vote);
+    int32_t xi_sum = 0;
+    for (int j = 0; j < bits; ++j) xi_sum = (xi_sum << 1) | int32_t((vm >> j) & 1);
     const double xi1 = double(2 * xi_sum + 1) / double(1 << (bits + 1));
     // k1 = std::round(n (b + xi1) / B)  (round half away from zero).
     const int64_t k1 = int64_t(round(double(n) * (double(b) + xi1) / double(B)));
     const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
     const double wf = SampleInFreq(xi0, width, dfreq);
     if (fabs(wf) < win_thr) continue;
-    // Coefficient estimate (src/iterate.cc:147-166).
-    double2 sum = make_double2(0.0, 0.0);
-    for (int t = 0; t < trials; ++t) {
-      const int64_t q = c.qs[int64_t(it) * c.trials_cap + t];
-      const double2 fac = Conj(CisFrac(ModN(q * k1, n, mask), n));
-      sum = Add(sum, Cmul(Ab[t * tstride], fac));
-      for (int32_t bit = 0; bit < bits; ++bit) {
-        const int64_t off = int64_t(B) << bit;
-        const double2 fac2 = Conj(CisFrac(ModN(off * k1, n, mask), n));
-        const double2 f1 = Cmul(fac, fac2);
-        const double2 f2 = Cmul(fac, Conj(fac2));
-        sum = Add(sum, Cmul(Ab[t * tstride + (2 * bit + 1) * rstride], f1));
-        sum = Add(sum, Cmul(Ab[t * tstride + (2 * bit + 2) * rstride], f2));
+    // Coefficient estimate (src/iterate.cc:147-166): sum over trials and rows
+    // of A * e^{-2 pi i (q + off*B) k1 / n}.
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int s = 0; s < kColSlices; ++s) {
+      const int p = lane + 64 * s;
+      if (p < npairs) {
+        const int t = p / rows;
+        const int row = p - t * rows;
+        const int64_t q = ip.q[t];
+        const double2 fac = Conj(CisFrac(ModN(q * k1, n, mask), n));
+        double2 f = fac;
+        if (row > 0) {
+          const int bit = (row - 1) / 2;
+          const double2 fac2 = Conj(CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n));
+          f = Cmul(fac, (row & 1) ? fac2 : Conj(fac2));
+        }
+        acc = Add(acc, Cmul(col[s], f));
       }
     }
-    const double denom = double(trials * (1 + 2 * bits));
-    double2 coef = make_double2(sum.x / denom, sum.y / denom);
-    // Undo the permutation: k0 = a_inv (k1 - b) mod n; coef *= e^{-2 pi i c k0/n}.
-    const int64_t k0 = ModN(tf.a_inv * ModN(k1 - tf.b, n, mask), n, mask);
-    coef = Cmul(coef, Conj(CisFrac(ModN(tf.c * k0, n, mask), n)));
-    coef = make_double2(coef.x / wf, coef.y / wf);
-    InsertMode(c, k0, coef);
-    any = true;
-  }
-
-  if (any) atomicOr(&st->found_iter, 1);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t ticket = atomicAdd(&st->blocks_done, 1u);
-    if (ticket == gridDim.x - 1) {
-      __threadfence();
-      // Last workgroup: every insert of this iteration is visible.
-      const int32_t cnt = atomicAdd(&st->count, 0);
-      const int32_t fi = atomicOr(&st->found_iter, 0);
-      if (it < c.log_cap) {
-        c.log[it] = B;
-        c.log[c.log_cap + it] = cnt;
-      }
-      st->found_last = fi;
-      if (!fi) st->stale += 1;
-      st->it = it + 1;
-      st->found_iter = 0;
-      st->blocks_done = 0u;
-      const bool overflow = (atomicOr(&st->flags, 0) & kFlagOverflow) != 0;
-      if (st->mode != kModeSolve || overflow || st->stale >= st->max_stale ||
-          st->it >= st->max_iter) {
-        st->done = 1;
-      } else {
-        // B = max((k - |found|)*2 + 1, min_bins) (src/demo1.cc:48-51).
-        const int64_t want = (int64_t(st->k) - int64_t(cnt)) * 2 + 1;
-        const int32_t Bn = int32_t(want > st->min_bins ? want : st->min_bins);
-        const BParams bp = c.btab[Bn];
-        LoadBParams(st, bp, 0);
-        st->bin_thr = bp.bin_thr;
-        st->sv_thr = bp.sv_thr;
-      }
+    acc.x = WaveSum(acc.x);
+    acc.y = WaveSum(acc.y);
+    if (lane == 0) {
+      const double denom = double(trials * (1 + 2 * bits));
+      double2 coef = make_double2(acc.x / denom, acc.y / denom);
+      // Undo the permutation: k0 = a_inv (k1 - b) mod n; coef *= e^{-2 pi i c k0/n}.
+      const int64_t k0 = ModN(tf.a_inv * ModN(k1 - tf.b, n, mask), n, mask);
+      coef = Cmul(coef, Conj(CisFrac(ModN(tf.c * k0, n, mask), n)));
+      coef = make_double2(coef.x / wf, coef.y / wf);
+      InsertMode(c, k0, coef);
+      any = true;
     }
   }
+  if (any) atomicOr(&st->found_iter[p], 1);
 }
 
 // BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix: one
@@ -711,7 +916,7 @@ int32_t Blocks(int64_t work) {
 }  // namespace
 
 hipError_t LaunchBuildTwiddles(double2 *tw, hipStream_t s) {
-  k_build_twiddles<<<Blocks(1 << (kMaxLogM - 1)), kThreads, 0, s>>>(tw);
+  k_build_twiddles<<<Blocks(kMaxM / 2), kThreads, 0, s>>>(tw);
   return hipGetLastError();
 }
 
@@ -733,21 +938,21 @@ hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t LaunchBinTime(const Ctx &c, int32_t tiles, size_t lds, hipStream_t s) {
+hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s) {
   if (c.mixed)
-    k_bin_time<float2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c);
+    k_bin_time<float2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c, parity);
   else
-    k_bin_time<double2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c);
+    k_bin_time<double2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c, parity);
   return hipGetLastError();
 }
 
-hipError_t LaunchBinFft(const Ctx &c, size_t lds, hipStream_t s) {
-  k_bin_fft<<<c.trials_cap * c.rows_cap, kThreads, lds, s>>>(c);
+hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s) {
+  k_bin_fft<<<c.trials_cap * c.rows_cap, kFftThreads, lds, s>>>(c, parity);
   return hipGetLastError();
 }
 
-hipError_t LaunchIdentify(const Ctx &c, int32_t blocks, hipStream_t s) {
-  k_identify<<<blocks, kThreads, 0, s>>>(c);
+hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s) {
+  k_identify<<<blocks, kThreads, 0, s>>>(c, parity);
   return hipGetLastError();
 }
 
