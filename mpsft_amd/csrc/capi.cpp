@@ -113,7 +113,7 @@ int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits, int p
   mpsft::BParams bp;
   if (!mpsft::WindowParams(n, bins, delta, &bp) || bits < 1 || bits > 30)
     return MPSFT_ERR_INVALID_ARG;
-  if (bp.logM > mpsft::kMaxLogM) return MPSFT_ERR_UNSUPPORTED;
+  if (bp.logM > mpsft::kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
   if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
   mpsft_plan *p = new (std::nothrow) mpsft_plan;
   if (!p) return MPSFT_ERR_NOMEM;

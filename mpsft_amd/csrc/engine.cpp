@@ -158,7 +158,7 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
   log_cap_ = 1 << 16;
   int rc = MPSFT_OK;
   rc |= Alloc(&d_state_, 1);
-  rc |= Alloc(&d_tw_, size_t(1) << (kMaxLogM - 1));
+  rc |= Alloc(&d_tw_, size_t(1) << (kTwLogM - 1));
   rc |= Alloc(&d_keys_, cap_);
   rc |= Alloc(&d_coef_, size_t(n_));
   rc |= Alloc(&d_present_, size_t((n_ + 31) / 32));
@@ -202,6 +202,7 @@ Engine::~Engine() {
   (void)hipFree(d_qs_);
   (void)hipFree(d_fold_);
   (void)hipFree(d_A_);
+  (void)hipFree(d_big_);
   (void)hipFree(d_term_bin_);
   (void)hipFree(d_term_val_);
   (void)hipFree(d_keys_);
@@ -214,8 +215,8 @@ Engine::~Engine() {
   if (h_progress_) (void)hipHostFree(const_cast<int32_t *>(h_progress_));
 }
 
-// Builds the window parameters, chirps and Bluestein filters of every B in
-// bs (for window delta) that is not built yet.
+// Builds the window parameters of every B in bs (for window delta) that is
+// not built yet, and for B <= kMaxLdsBins its chirp and Bluestein filter.
 int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t bits_override) {
   if (delta != tab_delta_) {
     hbtab_.clear();
@@ -241,14 +242,17 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     if (built_[B]) continue;
     BParams bp;
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
-    if (bp.logM > kMaxLogM) return MPSFT_ERR_UNSUPPORTED;
+    if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
     bp.R = ResiduesPerTile(B, NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f), bp.f);
-    bp.off = table_used_ + need;
-    need += B + bp.M;
+    bp.off = 0;
+    if (bp.logM <= kMaxLogM) {
+      bp.off = table_used_ + need;
+      need += B + bp.M;
+      todo.push_back(B);
+      Mmax = std::max(Mmax, bp.M);
+    }
     hbtab_[B] = bp;
     built_[B] = 1;
-    todo.push_back(B);
-    Mmax = std::max(Mmax, bp.M);
   }
   if (bits_override != tab_bits_override_) tab_bits_override_ = bits_override;
   if (table_used_ + need > table_len_) {
@@ -303,8 +307,9 @@ int Engine::SetThresholds(double sigma) {
 
 // Sizes the per-iteration buffers for trials and every B in bs.
 int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
-  int32_t Bc = 0, rc = 0, tc = 0, Mc = 0;
+  int32_t Bc = 0, rc = 0, tc = 0, Mc = 0, Mbig = 0;
   size_t lbt = 0, lfft = 0;
+  bool small = false, big = false;
   for (int32_t B : bs) {
     const BParams &bp = hbtab_[B];
     const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
@@ -313,7 +318,13 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     tc = std::max(tc, (B + bp.R - 1) / bp.R);
     Mc = std::max(Mc, bp.M);
     lbt = std::max(lbt, (size_t(NumLags(bits, bp.f) + bp.f) * bp.R + kThreads) * sizeof(double2));
-    lfft = std::max(lfft, (size_t(bp.M) + (bp.M <= kLdsTwMaxM ? bp.M / 2 : 0)) * sizeof(double2));
+    if (bp.logM <= kMaxLogM) {
+      small = true;
+      lfft = std::max(lfft, (size_t(bp.M) + (bp.M <= kLdsTwMaxM ? bp.M / 2 : 0)) * sizeof(double2));
+    } else {
+      big = true;
+      Mbig = std::max(Mbig, bp.M);
+    }
   }
   // k_identify keeps a bin's trials x rows column in 64 x kColSlices lanes.
   if (int64_t(trials) * rc > 64 * kColSlices) return MPSFT_ERR_UNSUPPORTED;
@@ -330,6 +341,22 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     HIP_OK(hipMemset(d_fold_, 0, sz * sizeof(double2)));
     HIP_OK(hipMemset(d_A_, 0, sz * sizeof(double2)));
     tf_valid_ = false;  // q stride follows trials_cap_
+    ++shape_gen_;
+  }
+  if (big) {
+    // scratch slots: every (trial, row) plus the filter row, M_cap each
+    const int32_t mcap = std::max(big_M_cap_, Mbig);
+    const size_t want = (size_t(trials_cap_) * rows_cap_ + 1) * size_t(mcap);
+    if (mcap != big_M_cap_ || want != big_len_) {
+      if (Alloc(&d_big_, want) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+      big_M_cap_ = mcap;
+      big_len_ = want;
+      ++shape_gen_;
+    }
+  }
+  if (small != fft_small_ || big != fft_big_) {
+    fft_small_ = small;
+    fft_big_ = big;
     ++shape_gen_;
   }
   if (tc > tiles_cap_ || Mc > M_cap_ || lbt > lds_bin_time_ || lfft > lds_fft_) {
@@ -430,6 +457,8 @@ Ctx Engine::MakeCtx() const {
   c.trials_cap = trials_cap_;
   c.rows_cap = rows_cap_;
   c.B_cap = B_cap_;
+  c.big = d_big_;
+  c.big_gpr = fft_big_ ? big_M_cap_ / kBigTile : 0;
   c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
   return c;
 }
@@ -442,9 +471,19 @@ int32_t Engine::IdentifyBlocks() const {
   return std::max<int32_t>(1, (B_cap_ + waves - 1) / waves);
 }
 
+// The bin FFT of every B the current call may use: k_bin_fft for
+// B <= kMaxLdsBins, the four-step kernels above; each exits at once for an
+// iteration whose B is the other kind.
+int Engine::LaunchFft(const Ctx &c, int parity, hipStream_t s) {
+  if (fft_small_) HIP_OK(LaunchBinFft(c, parity, LdsFft(), s));
+  if (fft_big_) HIP_OK(LaunchBigFft(c, parity, s));
+  return MPSFT_OK;
+}
+
 int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s) {
   HIP_OK(LaunchBinTime(c, parity, TilesCap(), LdsBinTime(), s));
-  HIP_OK(LaunchBinFft(c, parity, LdsFft(), s));
+  const int rc = LaunchFft(c, parity, s);
+  if (rc != MPSFT_OK) return rc;
   HIP_OK(LaunchIdentify(c, parity, IdentifyBlocks(), s));
   return MPSFT_OK;
 }
@@ -542,8 +581,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       const int par = launched & 1;
       if (hipEventRecord(ev[0], s) != hipSuccess ||
           LaunchBinTime(c, par, TilesCap(), LdsBinTime(), s) != hipSuccess ||
-          hipEventRecord(ev[1], s) != hipSuccess ||
-          LaunchBinFft(c, par, LdsFft(), s) != hipSuccess ||
+          hipEventRecord(ev[1], s) != hipSuccess || LaunchFft(c, par, s) != MPSFT_OK ||
           hipEventRecord(ev[2], s) != hipSuccess ||
           LaunchIdentify(c, par, IdentifyBlocks(), s) != hipSuccess ||
           hipEventRecord(ev[3], s) != hipSuccess) {
@@ -691,8 +729,9 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   a.win_thr = 1;
   HIP_OK(LaunchInit(c, a, s));
   HIP_OK(LaunchBinTime(c, 0, TilesCap(), LdsBinTime(), s));
-  HIP_OK(LaunchBinFft(c, 0, LdsFft(), s));
+  rc = LaunchFft(c, 0, s);
   A_override_ = nullptr;
+  if (rc != MPSFT_OK) return rc;
   return MPSFT_OK;
 }
 

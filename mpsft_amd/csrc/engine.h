@@ -9,7 +9,10 @@
 //                 terms BinInFreq needs;
 //   k_bin_fft   : per (trial, row), the B-point forward DFT (Bluestein over
 //                 an LDS-resident power-of-two FFT), the row phase, and the
-//                 BinInFreq subtraction;
+//                 BinInFreq subtraction; for B > kMaxLdsBins the same
+//                 Bluestein DFT runs as a four-step FFT in three kernels
+//                 (k_big_cols, k_big_rows, k_big_out) through a scratch
+//                 buffer that stays in cache;
 //   k_identify  : per bin, the energy test, the matrix-pencil votes
 //                 (IdentifyFreq), the coefficient estimate and the insert
 //                 into the found-mode map.
@@ -30,10 +33,14 @@ namespace mpsft {
 
 constexpr int kThreads = 256;      // workgroup size of most kernels
 constexpr int kFftThreads = 512;   // workgroup size of k_bin_fft
-constexpr int kMaxLogM = 13;       // Bluestein FFT length limit 8192 => B <= 4096
+constexpr int kMaxLogM = 13;       // k_bin_fft (one LDS FFT) up to M = 8192, B <= 4096
+constexpr int kMaxLdsBins = 4096;  // largest B of k_bin_fft
 constexpr int kLdsTwMaxM = 4096;   // k_bin_fft stages twiddles in LDS up to this M
+constexpr int kMaxLogMBig = 16;    // four-step Bluestein FFT up to M = 65536
+constexpr int kTwLogM = kMaxLogMBig;  // twiddle table e^{-2 pi i j / 2^kTwLogM}
+constexpr int kBigTile = 1024;     // elements of one four-step workgroup
 constexpr int kPrepBlocks = 16;    // k_bin_time workgroups computing mode terms
-constexpr int kMaxBins = 4096;
+constexpr int kMaxBins = 32767;    // largest B (2B-1 <= 2^kMaxLogMBig)
 constexpr int kTargetTiles = 512;  // k_bin_time tiles aimed for per trial
 constexpr int kColSlices = 4;      // k_identify holds trials*rows <= 256 entries per bin
 constexpr int kMaxTrials = 15;     // trials per iteration (odd)
@@ -44,6 +51,7 @@ struct BParams {
   int32_t B, P, f, bits, M, logM;
   int32_t R;        // residues per k_bin_time tile
   int64_t off;      // offset (in double2) of [chirp(B) | filter(M)] in table
+                    // (B <= kMaxLdsBins only; larger B compute both on the fly)
   double width;     // 1/(2B)
   double sigma_f;   // window std-dev in frequency
   double dfreq;     // sqrt(2)*pi*sigma_t  (SampleInFreq)
@@ -102,13 +110,14 @@ struct Ctx {
   DevState *st;
   const BParams *btab;   // indexed by B
   const double2 *table;  // chirps and Bluestein filters
-  const double2 *tw;     // e^{-2 pi i k / 2^kMaxLogM}, k < 2^(kMaxLogM-1)
+  const double2 *tw;     // e^{-2 pi i k / 2^kTwLogM}, k < 2^(kTwLogM-1)
   const DevTf *tfs;      // per iteration
   const int64_t *qs;     // per iteration x trials_cap
   int64_t n;
   int64_t mask;          // n-1 if n is a power of two, else -1
   double2 *fold;         // trials_cap x rows_cap x B_cap
   double2 *A;            // trials_cap x rows_cap x B_cap (bin coefficients)
+  double2 *big;          // (trials_cap x rows_cap + 1) x M_cap: four-step FFT scratch
   int32_t *term_bin;     // trials_cap x cap: bin of each found mode
   double2 *term_val;     // trials_cap x rows_cap x cap: BinInFreq terms
   int64_t *keys;         // cap
@@ -119,6 +128,7 @@ struct Ctx {
   int32_t log_cap;
   int32_t cap;
   int32_t trials_cap, rows_cap, B_cap;
+  int32_t big_gpr;       // four-step workgroups per row (M_cap / kBigTile), 0 if unused
   int32_t mixed;
 };
 
@@ -158,6 +168,7 @@ class Engine {
   int GrowTransforms(int32_t count, bool *grown);
   Ctx MakeCtx() const;
   int LaunchIteration(const Ctx &c, int parity, hipStream_t s);
+  int LaunchFft(const Ctx &c, int parity, hipStream_t s);
   int EnsureGraph(int32_t iters, hipStream_t s);
   int ReadState(hipStream_t s);
   size_t LdsBinTime() const;
@@ -182,6 +193,10 @@ class Engine {
   int32_t trials_cap_ = 0, rows_cap_ = 0, B_cap_ = 0;
   int32_t tiles_cap_ = 0;  // k_bin_time tiles per trial (max over B)
   int32_t M_cap_ = 0;
+  bool fft_small_ = false;  // some reserved B runs k_bin_fft
+  bool fft_big_ = false;    // some reserved B runs the four-step kernels
+  int32_t big_M_cap_ = 0;   // largest four-step M reserved
+  size_t big_len_ = 0;      // elements of d_big_
   size_t lds_bin_time_ = 0, lds_fft_ = 0;
 
   // device memory
@@ -201,6 +216,7 @@ class Engine {
   uint32_t tf_seed_ = 0;
   double2 *d_fold_ = nullptr;
   double2 *d_A_ = nullptr;
+  double2 *d_big_ = nullptr;
   int32_t *d_term_bin_ = nullptr;
   double2 *d_term_val_ = nullptr;
   int64_t *d_keys_ = nullptr;

@@ -26,6 +26,8 @@ hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s);
 // parity = iteration & 1: the IterParams slot the kernels use.
 hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s);
 hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);
+// The four-step bin FFT (B > kMaxLdsBins): three kernels.
+hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s);
 hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);

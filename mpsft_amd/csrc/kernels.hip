@@ -20,7 +20,8 @@ namespace mpsft {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kMaxM = 1 << kMaxLogM;
+constexpr int kMaxM = 1 << kMaxLogM;   // largest k_bin_fft FFT
+constexpr int kTwM = 1 << kTwLogM;     // twiddle table period
 
 __device__ __forceinline__ int64_t ModN(int64_t v, int64_t n, int64_t mask) {
   if (mask >= 0) return v & mask;
@@ -112,7 +113,8 @@ __device__ __forceinline__ int64_t RowOffset(int row) {
 
 // ---------------------------------------------------------------------------
 // Power-of-two FFTs in LDS, radix 4 (two radix-2 stages per pass, held in
-// registers) with a final radix-2 stage when logM is odd.  Twiddles come
+// registers) with a final radix-2 stage when logM is odd.  `count`
+// independent transforms of M points sit back to back in a.  Twiddles come
 // from a table w[j] = e^{-2 pi i j / M}, j < M/2, read as tw[j * tstride].
 struct Twiddles {
   const double2 *tw;
@@ -126,39 +128,41 @@ struct Twiddles {
 };
 
 // Decimation in time: bit-reversed input, natural-order output.
-__device__ void FftDit(double2 *a, int logM, const Twiddles &W) {
+__device__ void FftDit(double2 *a, int logM, const Twiddles &W, int count = 1) {
   const int M = 1 << logM;
   int s = 1;
   for (; s + 1 <= logM; s += 2) {
     const int half = 1 << (s - 1);
     const int sh1 = logM - s, sh2 = logM - s - 1;  // W_len^k = W_M^(k << sh)
-    for (int idx = threadIdx.x; idx < M / 4; idx += blockDim.x) {
+    for (int t = threadIdx.x; t < count * (M / 4); t += blockDim.x) {
+      const int idx = t & (M / 4 - 1);
       const int k = idx & (half - 1);
-      const int base = ((idx >> (s - 1)) << (s + 1)) + k;
+      const int base = ((t >> (logM - 2)) << logM) + ((idx >> (s - 1)) << (s + 1)) + k;
       double2 e0 = a[base], e1 = a[base + half], e2 = a[base + 2 * half], e3 = a[base + 3 * half];
       const double2 w1 = W(k << sh1);
       const double2 w2 = W(k << sh2);
       const double2 w3 = W.inverse ? MulPosI(w2) : MulNegI(w2);  // W_len2^(k + half)
-      double2 t = Cmul(e1, w1);
-      e1 = Sub(e0, t);
-      e0 = Add(e0, t);
-      t = Cmul(e3, w1);
-      e3 = Sub(e2, t);
-      e2 = Add(e2, t);
-      t = Cmul(e2, w2);
-      a[base + 2 * half] = Sub(e0, t);
-      a[base] = Add(e0, t);
-      t = Cmul(e3, w3);
-      a[base + 3 * half] = Sub(e1, t);
-      a[base + half] = Add(e1, t);
+      double2 v = Cmul(e1, w1);
+      e1 = Sub(e0, v);
+      e0 = Add(e0, v);
+      v = Cmul(e3, w1);
+      e3 = Sub(e2, v);
+      e2 = Add(e2, v);
+      v = Cmul(e2, w2);
+      a[base + 2 * half] = Sub(e0, v);
+      a[base] = Add(e0, v);
+      v = Cmul(e3, w3);
+      a[base + 3 * half] = Sub(e1, v);
+      a[base + half] = Add(e1, v);
     }
     __syncthreads();
   }
   if (s == logM) {  // last radix-2 stage
     const int half = 1 << (s - 1);
-    for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
+    for (int t = threadIdx.x; t < count * (M / 2); t += blockDim.x) {
+      const int idx = t & (M / 2 - 1);
       const int k = idx & (half - 1);
-      const int i = ((idx >> (s - 1)) << s) + k;
+      const int i = ((t >> (logM - 1)) << logM) + ((idx >> (s - 1)) << s) + k;
       const double2 u = a[i], v = Cmul(a[i + half], W(k));
       a[i] = Add(u, v);
       a[i + half] = Sub(u, v);
@@ -168,16 +172,17 @@ __device__ void FftDit(double2 *a, int logM, const Twiddles &W) {
 }
 
 // Decimation in frequency: natural input, bit-reversed output.
-__device__ void FftDif(double2 *a, int logM, const Twiddles &W) {
+__device__ void FftDif(double2 *a, int logM, const Twiddles &W, int count = 1) {
   const int M = 1 << logM;
   int s = logM;
   for (; s - 1 >= 1; s -= 2) {
     // stages s (len2 = 2^s) then s-1 (len = 2^(s-1)), half = 2^(s-2)
     const int half = 1 << (s - 2);
     const int sh2 = logM - s, sh1 = logM - s + 1;
-    for (int idx = threadIdx.x; idx < M / 4; idx += blockDim.x) {
+    for (int t = threadIdx.x; t < count * (M / 4); t += blockDim.x) {
+      const int idx = t & (M / 4 - 1);
       const int k = idx & (half - 1);
-      const int base = ((idx >> (s - 2)) << s) + k;
+      const int base = ((t >> (logM - 2)) << logM) + ((idx >> (s - 2)) << s) + k;
       const double2 e0 = a[base], e1 = a[base + half], e2 = a[base + 2 * half],
                     e3 = a[base + 3 * half];
       const double2 w2 = W(k << sh2);
@@ -193,7 +198,7 @@ __device__ void FftDif(double2 *a, int logM, const Twiddles &W) {
     __syncthreads();
   }
   if (s == 1) {
-    for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < count * (M / 2); idx += blockDim.x) {
       const int i = 2 * idx;
       const double2 u = a[i], v = a[i + 1];
       a[i] = Add(u, v);
@@ -214,10 +219,10 @@ __device__ __forceinline__ double2 Chirp(int64_t m, int32_t B) {
 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) k_build_twiddles(double2 *tw) {
-  const int half = kMaxM / 2;
+  const int half = kTwM / 2;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < half; k += gridDim.x * blockDim.x) {
     double s, c;
-    sincospi(-2.0 * double(k) / double(kMaxM), &s, &c);
+    sincospi(-2.0 * double(k) / double(kTwM), &s, &c);
     tw[k] = make_double2(c, s);
   }
 }
@@ -243,7 +248,7 @@ __global__ void __launch_bounds__(kThreads)
     lds[BitRev(j, bp.logM)] = v;
   }
   __syncthreads();
-  FftDit(lds, bp.logM, Twiddles{tw, kMaxM / M, false});
+  FftDit(lds, bp.logM, Twiddles{tw, kTwM / M, false});
   const double inv = 1.0 / double(M);
   for (int k = threadIdx.x; k < M; k += blockDim.x) filt[k] = Scale(lds[k], inv);
 }
@@ -587,13 +592,13 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
 __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   extern __shared__ double2 lds[];
   constexpr int T = kFftThreads;
-  constexpr int kPerB = kMaxBins / T;            // fold / chirp entries per thread
+  constexpr int kPerB = kMaxLdsBins / T;         // fold / chirp entries per thread
   constexpr int kPerM = kMaxM / T;               // filter entries per thread
   constexpr int kPerTw = (kLdsTwMaxM / 2) / T;   // twiddles staged per thread
   constexpr int kPerTerm = 2;                    // BinInFreq terms prefetched per thread
   const DevState *st = c.st;
   const IterParams &ip = st->P[p];
-  if (ip.done) return;
+  if (ip.done || ip.logM > kMaxLogM) return;
   const int32_t trials = st->trials, rows = ip.rows;
   const int32_t trial = blockIdx.x / c.rows_cap;
   const int32_t row = blockIdx.x - trial * c.rows_cap;
@@ -631,7 +636,7 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
 #pragma unroll
   for (int r = 0; r < kPerTw; ++r) {
     const int j = tid + r * T;
-    twv[r] = (tw_lds && j < M / 2) ? c.tw[j * (kMaxM / M)] : make_double2(0.0, 0.0);
+    twv[r] = (tw_lds && j < M / 2) ? c.tw[j * (kTwM / M)] : make_double2(0.0, 0.0);
   }
 #pragma unroll
   for (int r = 0; r < kPerTerm; ++r) {
@@ -647,7 +652,7 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
 
   double2 *a = lds;
-  Twiddles Wf{c.tw, kMaxM / M, false};
+  Twiddles Wf{c.tw, kTwM / M, false};
   if (tw_lds) {
     double2 *twl = lds + M;
 #pragma unroll
@@ -707,6 +712,184 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   __syncthreads();
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
   for (int k = tid; k < B; k += T) A[k] = a[k];
+}
+
+// ---------------------------------------------------------------------------
+// The same Bluestein DFT for B > kMaxLdsBins, where the M-point FFT does not
+// fit one workgroup's LDS.  M = N1 * N2 is split four-step style:
+//   forward  X[k1 + N1 k2] = sum_n2 W_N2^(n2 k2) W_M^(n2 k1)
+//                            sum_n1 W_N1^(n1 k1) u[N2 n1 + n2]
+//   inverse  y[N2 n1 + n2] = sum_k1 W_N1^(-n1 k1) W_M^(-n2 k1)
+//                            sum_k2 W_N2^(-n2 k2) U[k1 + N1 k2]
+// k_big_cols : per column group, the N1-point FFTs over n1 and the W_M^(n2 k1)
+//              twist, stored k1-major ([k1][n2]) in the scratch slot;
+// k_big_rows : per k1 group, the N2-point FFT over n2, the product with the
+//              filter spectrum, the inverse N2-point FFT and the conjugate
+//              twist, in place;
+// k_big_out  : per column group, the inverse N1-point FFTs over k1, then as
+//              k_bin_fft: chirp, row phase, BinInFreq subtraction, store A.
+// The filter spectrum H = FFT(h)/M is not tabulated per B: k_big_cols
+// transforms h as one more row (the last scratch slot) and every k_big_rows
+// workgroup finishes the transform of the k1 group it needs.  Each workgroup
+// holds kBigTile elements; the scratch slot of a row holds M of them.
+struct BigGeom {
+  int logN1, logN2, N1, N2, M;
+  __device__ explicit BigGeom(int logM)
+      : logN1(logM / 2), logN2(logM - logM / 2), N1(1 << (logM / 2)),
+        N2(1 << (logM - logM / 2)), M(1 << logM) {}
+};
+
+// W_M^j = e^{-2 pi i j / M} for 0 <= j < M from the half-period table.
+__device__ __forceinline__ double2 TwFull(const double2 *tw, int logM, int j) {
+  const int half = 1 << (logM - 1);
+  const int stride = kTwM >> logM;
+  if (j < half) return tw[j * stride];
+  const double2 w = tw[(j - half) * stride];
+  return make_double2(-w.x, -w.y);
+}
+
+// Resolves blockIdx.x into (trial, row, group) of a four-step kernel;
+// false when the workgroup has nothing to do this iteration.
+__device__ __forceinline__ bool BigSlot(const Ctx &c, const IterParams &ip, int32_t trials,
+                                        bool with_filter, int32_t *slot, int32_t *trial,
+                                        int32_t *row, int32_t *g) {
+  const int32_t gpr = 1 << (ip.logM - 10);  // kBigTile = 2^10
+  const int32_t s = blockIdx.x / c.big_gpr;
+  *g = blockIdx.x - s * c.big_gpr;
+  *slot = s;
+  if (*g >= gpr) return false;
+  if (s == c.trials_cap * c.rows_cap) {  // the filter row
+    *trial = -1;
+    *row = -1;
+    return with_filter;
+  }
+  *trial = s / c.rows_cap;
+  *row = s - *trial * c.rows_cap;
+  return *trial < trials && *row < ip.rows;
+}
+
+__global__ void __launch_bounds__(kThreads) k_big_cols(Ctx c, int p) {
+  __shared__ double2 a[kBigTile];
+  const DevState *st = c.st;
+  const IterParams &ip = st->P[p];
+  if (ip.done || ip.logM <= kMaxLogM) return;
+  int32_t slot, trial, row, g;
+  if (!BigSlot(c, ip, st->trials, true, &slot, &trial, &row, &g)) return;
+  const BigGeom G(ip.logM);
+  const int32_t B = ip.B;
+  const int logC = 10 - G.logN1;  // columns per workgroup = kBigTile / N1
+  const int C = 1 << logC;
+  const int n2_0 = g * C;
+  const double2 *fold =
+      trial >= 0 ? c.fold + (int64_t(trial) * c.rows_cap + row) * c.B_cap : nullptr;
+  const double inv_m = 1.0 / double(G.M);
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int col = e & (C - 1), n1 = e >> logC;
+    const int m = (n1 << G.logN2) + n2_0 + col;
+    double2 v = make_double2(0.0, 0.0);
+    if (fold) {
+      if (m < B) v = Cmul(fold[m], Chirp(m, B));
+    } else if (m < B) {
+      v = Scale(Conj(Chirp(m, B)), inv_m);
+    } else if (m > G.M - B) {
+      v = Scale(Conj(Chirp(G.M - m, B)), inv_m);
+    }
+    a[(col << G.logN1) + BitRev(n1, G.logN1)] = v;
+  }
+  __syncthreads();
+  FftDit(a, G.logN1, Twiddles{c.tw, kTwM >> G.logN1, false}, C);
+  double2 *out = c.big + int64_t(slot) * c.big_gpr * kBigTile;
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int col = e & (C - 1), k1 = e >> logC;
+    const int n2 = n2_0 + col;
+    out[(k1 << G.logN2) + n2] = Cmul(a[(col << G.logN1) + k1], TwFull(c.tw, ip.logM, n2 * k1));
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_big_rows(Ctx c, int p) {
+  __shared__ double2 a[2 * kBigTile];  // [data rows | filter rows]
+  const DevState *st = c.st;
+  const IterParams &ip = st->P[p];
+  if (ip.done || ip.logM <= kMaxLogM) return;
+  int32_t slot, trial, row, g;
+  if (!BigSlot(c, ip, st->trials, false, &slot, &trial, &row, &g)) return;
+  const BigGeom G(ip.logM);
+  const int logC = 10 - G.logN2;  // k1 rows per workgroup = kBigTile / N2
+  const int C = 1 << logC;
+  const int64_t stride = int64_t(c.big_gpr) * kBigTile;
+  double2 *y = c.big + slot * stride + g * kBigTile;
+  const double2 *h = c.big + int64_t(c.trials_cap) * c.rows_cap * stride + g * kBigTile;
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    a[e] = y[e];
+    a[kBigTile + e] = h[e];
+  }
+  __syncthreads();
+  const Twiddles Wf{c.tw, kTwM >> G.logN2, false};
+  FftDif(a, G.logN2, Wf, 2 * C);
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) a[e] = Cmul(a[e], a[kBigTile + e]);
+  __syncthreads();
+  Twiddles Wi = Wf;
+  Wi.inverse = true;
+  FftDit(a, G.logN2, Wi, C);
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int k1 = g * C + (e >> G.logN2), n2 = e & (G.N2 - 1);
+    y[e] = Cmul(a[e], Conj(TwFull(c.tw, ip.logM, n2 * k1)));
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_big_out(Ctx c, int p) {
+  __shared__ double2 a[kBigTile];
+  const DevState *st = c.st;
+  const IterParams &ip = st->P[p];
+  if (ip.done || ip.logM <= kMaxLogM) return;
+  int32_t slot, trial, row, g;
+  if (!BigSlot(c, ip, st->trials, false, &slot, &trial, &row, &g)) return;
+  const BigGeom G(ip.logM);
+  const int32_t B = ip.B;
+  const int logC = 10 - G.logN1;
+  const int C = 1 << logC;
+  const int n2_0 = g * C;
+  const int64_t n = c.n, mask = c.mask;
+  const double2 *z = c.big + int64_t(slot) * c.big_gpr * kBigTile;
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int col = e & (C - 1), k1 = e >> logC;
+    a[(col << G.logN1) + BitRev(k1, G.logN1)] = z[(k1 << G.logN2) + n2_0 + col];
+  }
+  __syncthreads();
+  FftDit(a, G.logN1, Twiddles{c.tw, kTwM >> G.logN1, true}, C);
+  const int64_t tau = ModN(ip.q[trial] + RowOffset(row) * B, n, mask);
+  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int col = e & (C - 1), n1 = e >> logC;
+    const int m = (n1 << G.logN2) + n2_0 + col;
+    if (m < B) {
+      const int i = (col << G.logN1) + n1;
+      a[i] = Cmul(Cmul(a[i], Chirp(m, B)), rp);
+    }
+  }
+  __syncthreads();
+  // BinInFreqV1 (src/binner.cc:565-591): the found modes whose bin lies in
+  // this workgroup's columns.
+  const int32_t cnt = min(st->count, c.cap);
+  const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
+  const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
+  for (int i = threadIdx.x; i < cnt; i += kThreads) {
+    const int bin = tb[i];
+    const int col = (bin & (G.N2 - 1)) - n2_0;
+    if (col >= 0 && col < C) {
+      const double2 v = tv[i];
+      double *dst = &a[(col << G.logN1) + (bin >> G.logN2)].x;
+      atomicAdd(dst, -v.x);
+      atomicAdd(dst + 1, -v.y);
+    }
+  }
+  __syncthreads();
+  double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
+  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
+    const int col = e & (C - 1), n1 = e >> logC;
+    const int m = (n1 << G.logN2) + n2_0 + col;
+    if (m < B) A[m] = a[(col << G.logN1) + n1];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -916,7 +1099,7 @@ int32_t Blocks(int64_t work) {
 }  // namespace
 
 hipError_t LaunchBuildTwiddles(double2 *tw, hipStream_t s) {
-  k_build_twiddles<<<Blocks(kMaxM / 2), kThreads, 0, s>>>(tw);
+  k_build_twiddles<<<Blocks(kTwM / 2), kThreads, 0, s>>>(tw);
   return hipGetLastError();
 }
 
@@ -948,6 +1131,15 @@ hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hi
 
 hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s) {
   k_bin_fft<<<c.trials_cap * c.rows_cap, kFftThreads, lds, s>>>(c, parity);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s) {
+  if (c.big_gpr <= 0) return hipSuccess;
+  const int32_t rows = c.trials_cap * c.rows_cap;
+  k_big_cols<<<(rows + 1) * c.big_gpr, kThreads, 0, s>>>(c, parity);
+  k_big_rows<<<rows * c.big_gpr, kThreads, 0, s>>>(c, parity);
+  k_big_out<<<rows * c.big_gpr, kThreads, 0, s>>>(c, parity);
   return hipGetLastError();
 }
 

@@ -76,7 +76,7 @@ def test_argument_errors_need_no_gpu():
     assert L.mpsft_plan_create(1109, 4, 1e-6, 2, 0, C.byref(h)) == 1   # even bins
     assert L.mpsft_plan_create(1109, 5, 1e-6, 0, 0, C.byref(h)) == 1   # bits < 1
     assert L.mpsft_plan_create(1109, 5, 1e-6, 2, 9, C.byref(h)) == 3   # unknown precision
-    assert L.mpsft_plan_create(1 << 20, 4097, 1e-6, 2, 0, C.byref(h)) == 3  # B too big
+    assert L.mpsft_plan_create(1 << 20, 32769, 1e-6, 2, 0, C.byref(h)) == 3  # B > 32767
     opt = mpsft_amd.solve_options(10, 1e-6)
     assert L.mpsft_solver_create(1 << 16, C.byref(opt), 9, 0, C.byref(h)) == 3
     assert L.mpsft_solve(None, None, 0, None, None) == 1

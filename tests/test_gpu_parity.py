@@ -69,7 +69,10 @@ def test_bin_in_time_reference_cases(gpu, orc, kats, case):
 @pytest.mark.parametrize("n,bins,delta", [(1 << 16, 101, 1e-6), (1 << 18, 2001, 1e-6),
                                           (1 << 14, 4095, 1e-6), (1048573, 2001, 1e-6),
                                           (1 << 20, 101, 1e-5), (1 << 12, 1, 1e-6),
-                                          (4099, 3, 1e-6)])
+                                          (4099, 3, 1e-6),
+                                          # B > 4096: the four-step bin FFT
+                                          (1 << 16, 4097, 1e-6), (1 << 18, 8001, 1e-6),
+                                          (1 << 17, 12001, 1e-5), (1 << 18, 16385, 1e-6)])
 def test_bin_in_time_random(gpu, orc, n, bins, delta):
     rng = np.random.default_rng(n ^ bins)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -164,6 +167,19 @@ def test_iterate_parity(gpu, orc, trials):
               window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
     _iterate_both(gpu, orc, x, n, gpu.iterate_options(**kw), orc.IterateOptions(**kw), 3,
                   seed=5 + trials)
+
+
+def test_iterate_parity_large_bins(gpu, orc):
+    """B = 5001 (M = 16384, the four-step bin FFT); the second step
+    subtracts the modes the first found (BinInFreq inside k_big_out)."""
+    n, k, sigma = 1 << 18, 2500, 1e-6
+    x, _ = make_signal(n, k, sigma, seed=23)
+    bins = 2 * k + 1
+    kw = dict(bins=bins, window_delta=1e-6, trials=1, bin_threshold=3 * sigma / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
+    mm = _iterate_both(gpu, orc, x, n, gpu.iterate_options(**kw), orc.IterateOptions(**kw), 2,
+                       seed=41)
+    assert len(mm.to_dict()) > k // 2
 
 
 def test_reference_iterate_once_on_gpu(gpu, orc, kats):
@@ -278,6 +294,33 @@ def test_solve_trials3_prime_n(gpu, orc):
     x, _ = make_signal(n, k, sigma, seed=3)
     got, ref, _, _ = _solve_both(gpu, orc, x, x, n, k, sigma, seed=13, trials=3)
     assert_coefs_close(got, ref, FP64_TOL)
+
+
+def test_solve_parity_large_bins(gpu, orc):
+    """S = 2200: the first B is 4401 (four-step bin FFT), later ones fit
+    k_bin_fft; both kinds run in one captured graph."""
+    n, k, sigma = 1 << 18, 2200, 1e-6
+    x, truth = make_signal(n, k, sigma, seed=4)
+    got, ref, solver, _ = _solve_both(gpu, orc, x, x, n, k, sigma, seed=17)
+    assert_coefs_close(got, ref, FP64_TOL)
+    assert solver.log()[0][0] == 2 * k + 1
+    assert len([key for key in truth if key in got]) >= 0.95 * k
+
+
+@pytest.mark.timeout(400)  # 20-40 s of host-side signal synthesis at n = 2^26
+def test_solve_parity_config3_mixed(gpu, orc):
+    """N=2^26 complex64, S=4000 (1000 in 10 clusters of +-64 bins), unit
+    magnitudes, sigma=1e-6: first B = 8001, bits = 14, 29 rows."""
+    n, k, sigma = 1 << 26, 4000, 1e-6
+    x, truth = make_signal(n, k, sigma, seed=3, clusters=(10, 0.25, 64))
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    del x
+    x_orc = x64.astype(np.complex128)
+    got, ref, solver, _ = _solve_both(gpu, orc, x64, x_orc, n, k, sig_eff, seed=19, precision=1)
+    assert_coefs_close(got, ref, MIXED_TOL)
+    assert solver.log()[0][0] == 8001
+    assert len([key for key in truth if key in got]) >= 0.95 * k
 
 
 # ---------------------------------------------------------------- edge cases
