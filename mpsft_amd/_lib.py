@@ -80,7 +80,12 @@ def lib():
         raise ImportError(
             f"{LIB_PATH} is not built; run `python -m mpsft_amd.build` "
             "(the HIP library is required, there is no CPU fallback)")
-    L = C.CDLL(LIB_PATH)
+    _lib = declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def declare(L):
+    """Sets the ctypes prototype of every include/mpsft.h symbol on L."""
     vp, i32, i64, f64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_size_t
     pi32, pi64, pf64 = C.POINTER(i32), C.POINTER(i64), C.POINTER(f64)
     sig = {
@@ -122,7 +127,6 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
     return L
 
 

@@ -193,6 +193,8 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
 }
 
 Engine::~Engine() {
+  // Work a caller left queued may still use these buffers.
+  (void)hipDeviceSynchronize();
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   (void)hipFree(d_state_);
   (void)hipFree(d_btab_);
@@ -370,16 +372,19 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
 }
 
 // Sizes the transform / q arrays for count iterations; *grown is set when
-// the arrays were reallocated (their content is then gone).
+// the arrays were reallocated (their device content is then gone).  A
+// captured graph holds their addresses, so a caller with graph launches in
+// flight drains its stream first; the new addresses bump shape_gen_, which
+// makes EnsureGraph capture again.
 int Engine::GrowTransforms(int32_t count, bool *grown) {
   *grown = false;
   if (count <= tf_cap_ && tf_qstride_ == trials_cap_) return MPSFT_OK;
-  const int32_t cap = std::max(count, std::max(256, 2 * tf_cap_));
+  const int32_t cap = std::max(count, std::max(1024, 2 * tf_cap_));
   if (Alloc(&d_tfs_, cap) != MPSFT_OK || Alloc(&d_qs_, size_t(cap) * trials_cap_) != MPSFT_OK)
     return MPSFT_ERR_NOMEM;
   tf_cap_ = cap;
+  if (tf_qstride_ != trials_cap_) tf_valid_ = false;  // the host copies' q layout changed
   tf_qstride_ = trials_cap_;
-  tf_valid_ = false;
   *grown = true;
   ++shape_gen_;
   return MPSFT_OK;
@@ -389,7 +394,9 @@ int Engine::GrowTransforms(int32_t count, bool *grown) {
 // reference's order: Transform(n) draws a, b, c (src/integer.cc:44-49),
 // then Iterate draws RandomInt32() % n per trial (src/iterate.cc:103).
 // For n = 2^k the multiplier is drawn odd, a = 2 (r mod n/2) + 1; for other
-// composite n the draw of a repeats until gcd(a, n) = 1.
+// composite n the draw of a repeats until gcd(a, n) = 1.  Draws already made
+// for the same seed are kept (host copies) and only the new ones are
+// uploaded, unless the device arrays were reallocated.
 int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s) {
   bool grown = false;
   const int rc = GrowTransforms(count, &grown);
@@ -403,8 +410,7 @@ int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipSt
     tf_seed_ = seed;
     tf_trials_ = trials;
   }
-  if (!fresh && count <= tf_count_) return MPSFT_OK;
-  const int32_t first = fresh ? 0 : tf_count_;
+  if (!fresh && !grown && count <= tf_count_) return MPSFT_OK;
   const int32_t target = std::max(count, tf_count_);
   const int64_t n = n_;
   while (int32_t(h_tfs_.size()) < target) {
@@ -421,12 +427,13 @@ int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipSt
     for (int32_t t = 0; t < trials_cap_; ++t)
       h_qs_.push_back(t < trials ? int64_t(RandomInt32() % n) : 0);
   }
-  const int32_t from = first;
+  const int32_t from = (fresh || grown) ? 0 : tf_count_;
   HIP_OK(hipMemcpyAsync(d_tfs_ + from, h_tfs_.data() + from,
                         size_t(target - from) * sizeof(DevTf), hipMemcpyHostToDevice, s));
   HIP_OK(hipMemcpyAsync(d_qs_ + size_t(from) * trials_cap_, h_qs_.data() + size_t(from) * trials_cap_,
                         size_t(target - from) * trials_cap_ * sizeof(int64_t),
                         hipMemcpyHostToDevice, s));
+  // The host vectors may move on the next draw: the copies finish first.
   HIP_OK(hipStreamSynchronize(s));
   tf_count_ = target;
   tf_valid_ = true;
@@ -598,14 +605,23 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       if (out != MPSFT_OK) break;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    if (out != MPSFT_OK) return out;
+    if (out != MPSFT_OK) {
+      (void)hipStreamSynchronize(s);
+      return out;
+    }
   } else {
     // One graph per two iterations, launched ahead of the device by at most
     // `lookahead` iterations.  The first kernel of every iteration reports
     // {iterations done, stopped} in pinned host memory, so the host paces
     // launches without waiting on the stream and stops once the loop stops.
+    // Every return below first waits for what was launched: the kernels
+    // read and write this engine's buffers and pinned counters.
+    auto fail = [&](int code) {
+      (void)hipStreamSynchronize(s);
+      return code;
+    };
     rc = EnsureGraph(2, s);
-    if (rc != MPSFT_OK) return rc;
+    if (rc != MPSFT_OK) return fail(rc);
     const int32_t lookahead = chunk < 2 ? 2 : chunk;
     auto last_change = std::chrono::steady_clock::now();
     int32_t seen = 0, synced = 0;
@@ -617,9 +633,18 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       // stops the loop (logging the last iteration).
       if (h_progress_[1] || launched > max_iter) break;
       if (launched - progressed < lookahead) {
-        rc = EnsureTransforms(seed, std::min(max_iter, launched + 3), opt.trials, s);
-        if (rc != MPSFT_OK) return rc;
-        HIP_OK(hipGraphLaunch(graph_exec_, s));
+        const int32_t need = std::min(max_iter, launched + 3);
+        if (need > tf_count_) {
+          // Draw ahead in doubling batches.  Growing the arrays moves them,
+          // and the graph holds their addresses: drain the stream first, so
+          // no launched graph can read the freed arrays, then capture again.
+          const int32_t want = std::min(max_iter, std::max(need, 2 * tf_count_));
+          if (want > tf_cap_ && hipStreamSynchronize(s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+          rc = EnsureTransforms(seed, want, opt.trials, s);
+          if (rc == MPSFT_OK) rc = EnsureGraph(2, s);
+          if (rc != MPSFT_OK) return fail(rc);
+        }
+        if (hipGraphLaunch(graph_exec_, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
         launched += 2;
         continue;
       }
@@ -631,14 +656,14 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
         // No progress report for a while (a long iteration): wait on the
         // stream instead of spinning, then continue from the device state.
         rc = ReadState(s);
-        if (rc != MPSFT_OK) return rc;
+        if (rc != MPSFT_OK) return fail(rc);
         synced = std::max(h_state_->L[0].it, h_state_->L[1].it);
         done = (h_state_->L[0].done | h_state_->L[1].done) != 0;
         last_change = std::chrono::steady_clock::now();
       }
     }
     rc = ReadState(s);
-    if (rc != MPSFT_OK) return rc;
+    if (rc != MPSFT_OK) return fail(rc);
   }
   const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
   if (!stopped) {

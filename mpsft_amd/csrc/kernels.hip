@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kThreads) k_build_twiddles(double2 *tw) {
 __global__ void __launch_bounds__(kThreads)
     k_build_tables(const int32_t *blist, const BParams *btab, double2 *table,
                    const double2 *tw) {
-  extern __shared__ double2 lds[];
+  HIP_DYNAMIC_SHARED(double2, lds)
   const int32_t B = blist[blockIdx.x];
   const BParams bp = btab[B];
   double2 *chirp = table + bp.off;
@@ -460,7 +460,7 @@ struct Lags {
 // found mode for this iteration.
 template <typename Tx>
 __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
-  extern __shared__ double2 lds[];
+  HIP_DYNAMIC_SHARED(double2, lds)
   const DevState *st = c.st;
   IterParams ip;
   LoopCounters lc;
@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
 // (terms from k_bin_time).  The twiddle table is staged in LDS when
 // M <= kLdsTwMaxM.
 __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
-  extern __shared__ double2 lds[];
+  HIP_DYNAMIC_SHARED(double2, lds)
   constexpr int T = kFftThreads;
   constexpr int kPerB = kMaxLdsBins / T;         // fold / chirp entries per thread
   constexpr int kPerM = kMaxM / T;               // filter entries per thread
@@ -1099,7 +1099,7 @@ int32_t Blocks(int64_t work) {
 }  // namespace
 
 hipError_t LaunchBuildTwiddles(double2 *tw, hipStream_t s) {
-  k_build_twiddles<<<Blocks(kTwM / 2), kThreads, 0, s>>>(tw);
+  hipLaunchKernelGGL(k_build_twiddles, dim3(Blocks(kTwM / 2)), dim3(kThreads), 0, s, tw);
   return hipGetLastError();
 }
 
@@ -1107,56 +1107,56 @@ hipError_t LaunchBuildTables(const int32_t *blist, int32_t nb, const BParams *bt
                              double2 *table, const double2 *tw, int32_t M_cap,
                              hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  k_build_tables<<<nb, kThreads, size_t(M_cap) * sizeof(double2), s>>>(blist, btab, table, tw);
+  hipLaunchKernelGGL(k_build_tables, dim3(nb), dim3(kThreads), size_t(M_cap) * sizeof(double2), s, blist, btab, table, tw);
   return hipGetLastError();
 }
 
 hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s) {
-  k_clear_modes<<<1024, kThreads, 0, s>>>(c, full);
+  hipLaunchKernelGGL(k_clear_modes, dim3(1024), dim3(kThreads), 0, s, c, full);
   return hipGetLastError();
 }
 
 hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s) {
-  k_init<<<1, 64, 0, s>>>(c, a);
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, c, a);
   return hipGetLastError();
 }
 
 hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s) {
   if (c.mixed)
-    k_bin_time<float2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c, parity);
+    hipLaunchKernelGGL(k_bin_time<float2>, dim3(kPrepBlocks + tiles), dim3(kThreads), lds, s, c, parity);
   else
-    k_bin_time<double2><<<kPrepBlocks + tiles, kThreads, lds, s>>>(c, parity);
+    hipLaunchKernelGGL(k_bin_time<double2>, dim3(kPrepBlocks + tiles), dim3(kThreads), lds, s, c, parity);
   return hipGetLastError();
 }
 
 hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s) {
-  k_bin_fft<<<c.trials_cap * c.rows_cap, kFftThreads, lds, s>>>(c, parity);
+  hipLaunchKernelGGL(k_bin_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds, s, c, parity);
   return hipGetLastError();
 }
 
 hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s) {
   if (c.big_gpr <= 0) return hipSuccess;
   const int32_t rows = c.trials_cap * c.rows_cap;
-  k_big_cols<<<(rows + 1) * c.big_gpr, kThreads, 0, s>>>(c, parity);
-  k_big_rows<<<rows * c.big_gpr, kThreads, 0, s>>>(c, parity);
-  k_big_out<<<rows * c.big_gpr, kThreads, 0, s>>>(c, parity);
+  hipLaunchKernelGGL(k_big_cols, dim3((rows + 1) * c.big_gpr), dim3(kThreads), 0, s, c, parity);
+  hipLaunchKernelGGL(k_big_rows, dim3(rows * c.big_gpr), dim3(kThreads), 0, s, c, parity);
+  hipLaunchKernelGGL(k_big_out, dim3(rows * c.big_gpr), dim3(kThreads), 0, s, c, parity);
   return hipGetLastError();
 }
 
 hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s) {
-  k_identify<<<blocks, kThreads, 0, s>>>(c, parity);
+  hipLaunchKernelGGL(k_identify, dim3(blocks), dim3(kThreads), 0, s, c, parity);
   return hipGetLastError();
 }
 
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s) {
-  k_set_modes<<<Blocks(m), kThreads, 0, s>>>(c, keys, coefs, m);
+  hipLaunchKernelGGL(k_set_modes, dim3(Blocks(m)), dim3(kThreads), 0, s, c, keys, coefs, m);
   return hipGetLastError();
 }
 
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  k_get_coefs<<<Blocks(m), kThreads, 0, s>>>(c, out, m);
+  hipLaunchKernelGGL(k_get_coefs, dim3(Blocks(m)), dim3(kThreads), 0, s, c, out, m);
   return hipGetLastError();
 }
 
@@ -1165,14 +1165,14 @@ hipError_t LaunchBinInFreqDirect(int64_t n, int64_t mask, const BParams &bp,
                                  const double2 *coef, int64_t m, double2 *out,
                                  hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  k_bin_in_freq_direct<<<Blocks(m), kThreads, 0, s>>>(n, mask, bp, tf, q, k0, coef, m, out);
+  hipLaunchKernelGGL(k_bin_in_freq_direct, dim3(Blocks(m)), dim3(kThreads), 0, s, n, mask, bp, tf, q, k0, coef, m, out);
   return hipGetLastError();
 }
 
 hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma, uint8_t *vote,
                            hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  k_mat_pencil<<<Blocks(m), kThreads, 0, s>>>(u, m, sigma, vote);
+  hipLaunchKernelGGL(k_mat_pencil, dim3(Blocks(m)), dim3(kThreads), 0, s, u, m, sigma, vote);
   return hipGetLastError();
 }
 

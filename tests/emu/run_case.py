@@ -1,0 +1,88 @@
+"""Runs one scenario of the library on the host emulator (tests/emu) and
+prints a JSON summary.  Started by tests/test_emu.py in a child process
+with AddressSanitizer preloaded; the kernels' memory accesses are checked
+by ASan, the results against the oracle."""
+import ctypes as C
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from mpsft_amd import _lib  # noqa: E402
+
+_lib._lib = _lib.declare(C.CDLL(os.path.join(HERE, "_build", "libmpsft_emu.so")))
+
+import mpsft_amd as gpu  # noqa: E402
+from oracle import pyoracle as orc  # noqa: E402
+
+
+def signal(n, k, sigma, seed):
+    rng = np.random.default_rng(seed)
+    locs = rng.choice(n, size=k, replace=False)
+    xh = np.zeros(n, np.complex128)
+    xh[locs] = np.exp(2j * np.pi * rng.uniform(0, 1, k))
+    xh += (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * sigma / np.sqrt(2 * n)
+    return np.fft.ifft(xh) * n
+
+
+def solve(n, k, sigma, seed, precision=0, repeat=1, x_path=None, **kw):
+    x = np.load(x_path) if x_path else signal(n, k, sigma, seed)
+    xg = x.astype(np.complex64) if precision else x
+    xo = xg.astype(np.complex128)
+    s = gpu.Solver(n, gpu.solve_options(k, sigma, **kw), precision)
+    xd = s.upload(xg)
+    out = {}
+    for r in range(repeat):
+        it = s.solve(xd, seed)
+        bins, found = s.log()
+        got = s.modes_dict()
+        out[f"iters{r}"] = it
+    orc.seed(seed)
+    ref, rb, rf = orc.demo1_run(xo, k, sigma, orc.Demo1Options(**kw))
+    ref = ref.to_dict()
+    err = max((abs(got[key] - ref[key]) / max(abs(ref[key]), 1e-3) for key in ref if key in got),
+              default=0.0)
+    out.update(iters=it, ref_iters=len(rb), bins=bins.tolist(), ref_bins=list(map(int, rb)),
+               same_support=set(got) == set(ref), n_found=len(got), max_rel_err=err)
+    return out
+
+
+def iterate(n, k, sigma, seed, bins, steps=2):
+    x = signal(n, k, sigma, seed)
+    kw = dict(bins=bins, window_delta=1e-6, trials=1, bin_threshold=3 * sigma / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
+    s = gpu.Solver(n, gpu.solve_options(1, 0.0))
+    xd = s.upload(x)
+    mm = orc.ModeMap()
+    orc.seed(seed)
+    errs = []
+    for _ in range(steps):
+        tf = orc.transform_draw(n)
+        q = orc.rand_int32() % n
+        orc.iterate(x, orc.IterateOptions(**kw), mm, tf=tf, qs=np.array([q], np.int32))
+        s.iterate(xd, gpu.iterate_options(**kw), gpu.transform_make(n, *map(int, tf[:3])), [q])
+        got, ref = s.modes_dict(), mm.to_dict()
+        errs.append(max((abs(got[key] - ref[key]) for key in ref if key in got), default=0.0))
+    return dict(same_support=set(got) == set(ref), n_found=len(got), max_abs_err=max(errs))
+
+
+def plan(n, bins, bits, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    orc.seed(seed)
+    tf = orc.transform_draw(n)
+    q = int(rng.integers(0, n))
+    ref = orc.bin_in_time(x, bins, 1e-6, bits, tf, q, version=0)
+    got = gpu.Plan(n, bins, 1e-6, bits).bin_in_time(x, gpu.transform_make(n, *map(int, tf[:3])), q)
+    return dict(rel_err=float(np.abs(got - ref).max() / np.abs(ref).max()))
+
+
+if __name__ == "__main__":
+    case = json.loads(sys.argv[1])
+    fn = {"solve": solve, "iterate": iterate, "plan": plan}[case.pop("kind")]
+    print("RESULT " + json.dumps(fn(**case)), flush=True)
