@@ -84,6 +84,32 @@ def cpu_baseline(x, sample_solves):
     return dt * 1e6, iters
 
 
+def copy_calibration():
+    """A plain copy of known size: 256 MiB read and 256 MiB written, for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of other kernels."""
+    import torch
+    src = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+
+
+def traffic_from_profile():
+    """HBM bytes per k_bin_time<float2> launch measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), as committed
+    in profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles", "k_bin_time_traffic_mpsft_solve_n22_s1000_mixed.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return int(float(ln.split(":", 1)[1]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,11 +118,18 @@ def main():
     ap.add_argument("--cpu-solves", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the instrumented MIXED solve (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one signal's solve per device); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import mpsft_amd as M
     from mpsft_amd.signals import make_signal, quantization_sigma
@@ -107,6 +140,15 @@ def main():
     sig_eff = float(np.hypot(SIGMA, quantization_sigma(x, x64)))
 
     mixed = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_MIXED)
+    if args.roofline_only:
+        xd64 = mixed.upload(x64)
+        it_p, ms = mixed.solve_profiled(xd64, SOLVE_SEED)
+        line_bytes, logical, n_launch = gather_bytes(M, mixed, 8)
+        print(json.dumps({"k_bin_time_avg_launch_us": ms[0] * 1000.0 / n_launch,
+                          "launches": n_launch, "timed_repeats_per_iteration": 8,
+                          "algorithmic_bytes_per_launch": line_bytes / n_launch,
+                          "kernel_ms_per_solve": ms}), flush=True)
+        return
     fp64q = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
     fp64 = M.Solver(N, M.solve_options(K, SIGMA, window_delta=DELTA), M.MPSFT_FP64)
     xd64 = mixed.upload(x64)
@@ -125,6 +167,8 @@ def main():
     rel = [abs(gm[k] - gq[k]) / abs(gq[k]) for k in gq if k in gm]
     max_rel = max(rel) if rel else None
     hits = sum(1 for k in truth if k in gm)
+    g64 = fp64.modes_dict()
+    hits64 = sum(1 for k in truth if k in g64)
     truth_err = max(abs(gm[k] - v) / abs(v) for k, v in truth.items() if k in gm)
 
     line = {
@@ -150,6 +194,10 @@ def main():
         "mixed_speedup_vs_fp64": round(us_fp64q / us_mixed, 3),
         "fp64_c128_sigma_us": round(us_fp64, 2),
         "fp64_c128_sigma_iterations": it_fp64,
+        "mixed_speedup_vs_plain_fp64": round(us_fp64 / us_mixed, 3),
+        "true_modes_recovered_plain_fp64": hits64,
+        "true_support_identical_mixed_vs_plain_fp64":
+            {k for k in truth if k in gm} == {k for k in truth if k in g64},
         "support_identical_mixed_vs_fp64": same_support,
         "max_rel_coef_diff_mixed_vs_fp64": max_rel,
         "true_modes_recovered": hits,
@@ -164,7 +212,7 @@ def main():
         achieved = line_bytes / t_bt / 1e9
         line["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
             "kernel": "k_bin_time<float2>", "launches": n_launch,
             "avg_launch_us": round(ms[0] * 1000.0 / n_launch, 2),
             "algorithmic_bytes_per_launch": int(line_bytes / n_launch),

@@ -160,9 +160,12 @@ int mpsft_solve(mpsft_solver *s, const void *x_dev, uint32_t seed,
                 int32_t *iters, void *stream);
 
 /* Instrumentation, no reference counterpart: the same solve with each
- * kernel bracketed by HIP events on `stream` (no graph), returning the
- * summed milliseconds of the gather-fold, bin-FFT and identify kernels in
- * kernel_ms[0..2].  Results are identical to mpsft_solve. */
+ * kernel bracketed by HIP events on `stream` (no graph), returning per
+ * solve the summed milliseconds of the gather-fold, bin-FFT and identify
+ * kernels in kernel_ms[0..2].  The gather-fold and the bin FFT rewrite
+ * their whole output from unchanged inputs, so each is launched 8 times
+ * back to back per iteration and its mean launch time is counted (keeping
+ * event overhead out).  Results are identical to mpsft_solve. */
 int mpsft_solve_profiled(mpsft_solver *s, const void *x_dev, uint32_t seed,
                          int32_t *iters, double *kernel_ms, void *stream);
 

@@ -139,7 +139,8 @@ class Engine {
   int status() const { return status_; }
 
   // kernel_ms (3 entries, or null): when given, the solve runs without the
-  // graph and accumulates the time of k_bin_time, k_bin_fft, k_identify.
+  // graph and accumulates the time of k_bin_time, the bin FFT and
+  // k_identify (the first two as the mean of repeated launches).
   int Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
             int32_t *iters, hipStream_t s, double *kernel_ms = nullptr);
   int IterateOnce(const void *x, const mpsft_iterate_options &opt,
