@@ -322,7 +322,10 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     lbt = std::max(lbt, (size_t(NumLags(bits, bp.f) + bp.f) * bp.R + kThreads) * sizeof(double2));
     if (bp.logM <= kMaxLogM) {
       small = true;
-      lfft = std::max(lfft, (size_t(bp.M) + (bp.M <= kLdsTwMaxM ? bp.M / 2 : 0)) * sizeof(double2));
+      // k_bin_fft: M <= 1024 array + M/2 twiddles; larger M the padded
+      // array, then the chirp
+      const size_t el = bp.logM <= kFftSmallLogM ? size_t(bp.M + bp.M / 2) : size_t(bp.M + bp.M / 16 + B);
+      lfft = std::max(lfft, el * sizeof(double2));
     } else {
       big = true;
       Mbig = std::max(Mbig, bp.M);

@@ -32,10 +32,10 @@
 namespace mpsft {
 
 constexpr int kThreads = 256;      // workgroup size of most kernels
-constexpr int kFftThreads = 512;   // workgroup size of k_bin_fft
-constexpr int kMaxLogM = 13;       // k_bin_fft (one LDS FFT) up to M = 8192, B <= 4096
-constexpr int kMaxLdsBins = 4096;  // largest B of k_bin_fft
-constexpr int kLdsTwMaxM = 4096;   // k_bin_fft stages twiddles in LDS up to this M
+constexpr int kFftThreads = 256;   // workgroup size of k_bin_fft
+constexpr int kMaxLogM = 12;       // k_bin_fft (one LDS FFT) up to M = 4096, B <= 2048
+constexpr int kMaxLdsBins = 2048;  // largest B of k_bin_fft
+constexpr int kFftSmallLogM = 10;  // k_bin_fft's in-place radix-4 path up to M = 1024
 constexpr int kMaxLogMBig = 16;    // four-step Bluestein FFT up to M = 65536
 constexpr int kTwLogM = kMaxLogMBig;  // twiddle table e^{-2 pi i j / 2^kTwLogM}
 constexpr int kBigTile = 1024;     // elements of one four-step workgroup

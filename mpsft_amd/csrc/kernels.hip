@@ -20,7 +20,6 @@ namespace mpsft {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kMaxM = 1 << kMaxLogM;   // largest k_bin_fft FFT
 constexpr int kTwM = 1 << kTwLogM;     // twiddle table period
 
 __device__ __forceinline__ int64_t ModN(int64_t v, int64_t n, int64_t mask) {
@@ -58,39 +57,35 @@ __device__ __forceinline__ double2 CisFrac(int64_t r, int64_t n) {
   return CisTurns(double(r) / double(n));
 }
 
-template <typename Tx>
-__device__ __forceinline__ double2 LoadX(const Tx *x, int64_t j);
-template <>
-__device__ __forceinline__ double2 LoadX<double2>(const double2 *x, int64_t j) {
-  return x[j];
-}
-template <>
-__device__ __forceinline__ double2 LoadX<float2>(const float2 *x, int64_t j) {
-  const float2 v = x[j];
-  return make_double2(double(v.x), double(v.y));
-}
+// A sample of x (double2 or float2) in double precision.
+__device__ __forceinline__ double2 Widen(double2 v) { return v; }
+__device__ __forceinline__ double2 Widen(float2 v) { return make_double2(double(v.x), double(v.y)); }
 
-// SincPi (src/sincos.cc:69-88): sin(x)/x with Taylor terms near 0.
-__device__ double SincPiRef(double x) {
+// Window::SampleInTime (src/window.cc:71-75): w sinc(pi t w) e^{-2 (pi t
+// sigma_f)^2}, with SincPi (src/sincos.cc:69-88) = sin(x)/x and its Taylor
+// terms near 0.  sin(pi |t| w) is evaluated as sinpi(|t| w): the same value
+// up to rounding, with exact argument reduction (the generic sin carries a
+// large-argument path that needs private memory).
+__device__ __forceinline__ double SampleInTime(int64_t i, double width, double sigma_f) {
   const double taylor_0_bound = 2.220446049250313080847e-16;
   const double taylor_2_bound = 1.490116119384765625e-08;  // sqrt(eps)
   const double taylor_n_bound = 1.220703125e-04;           // eps^(1/4)
-  if (x < 0) x = -x;
-  if (x >= taylor_n_bound) return sin(x) / x;
-  double r = 1.0;
-  if (x >= taylor_0_bound) {
-    const double x2 = x * x;
-    r -= x2 / 6.0;
-    if (x >= taylor_2_bound) r += (x2 * x2) / 120.0;
-  }
-  return r;
-}
-
-// Window::SampleInTime (src/window.cc:71-75).
-__device__ __forceinline__ double SampleInTime(int64_t i, double width, double sigma_f) {
   const double t = double(i);
+  const double at = fabs(t);
+  const double x = at * kPi * width;
+  double sinc;
+  if (x >= taylor_n_bound) {
+    sinc = sinpi(at * width) / x;
+  } else {
+    sinc = 1.0;
+    if (x >= taylor_0_bound) {
+      const double x2 = x * x;
+      sinc -= x2 / 6.0;
+      if (x >= taylor_2_bound) sinc += (x2 * x2) / 120.0;
+    }
+  }
   const double u = t * kPi * sigma_f;
-  return width * SincPiRef(t * kPi * width) * exp(-2.0 * u * u);
+  return width * sinc * exp(-2.0 * u * u);
 }
 
 // Window::SampleInFreq (src/window.cc:77-81).
@@ -422,7 +417,8 @@ __global__ void __launch_bounds__(kThreads)
 // union, nlam lags, is staged once per residue.
 struct Lags {
   int K, len0, nlam, f;
-  __device__ Lags(int bits, int f_) : f(f_) {
+  float inv_f;
+  __device__ __forceinline__ Lags(int bits, int f_) : f(f_), inv_f(1.0f / float(f_)) {
     K = 0;
     while (K < bits && (1 << K) < f) ++K;
     len0 = (K ? (1 << K) : 0) + f;
@@ -432,7 +428,10 @@ struct Lags {
   __device__ __forceinline__ int64_t Lambda(int li) const {
     if (li < len0) return K ? li - (1 << (K - 1)) : li;
     const int h = li - len0;
-    const int seg = h / f;
+    // h / f through the float reciprocal, corrected to the exact quotient
+    int seg = int(float(h) * inv_f);
+    if ((seg + 1) * f <= h) ++seg;
+    else if (seg * f > h) --seg;
     const int kk = K + seg / 2;
     const int64_t lo = (seg & 1) ? -(int64_t(1) << kk) : (int64_t(1) << kk);
     return lo + (h - seg * f);
@@ -518,32 +517,48 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
   double2 *T = lds + L.nlam * R;  // [f][R]
   double2 *red = T + f * R;       // [kThreads]
 
-  // Phase A: gather every distinct sample of the tile once.
+  // Phase A: gather every distinct sample of the tile once, and phase T:
+  // the taps tap(t), t = k*B + rho - p2 (Window::SampleInTime and the phase
+  // of src/binner.cc:100-104 with tau factored out).  Each thread issues
+  // the loads of up to kGather samples into registers, computes its taps
+  // while they are in flight, and only then writes the samples to LDS;
+  // storing each sample as it arrives would serialise the loads.
+  constexpr int kGather = 8;
+  const int logR = R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : 3;
+  const int total = L.nlam << logR;
   const int64_t ubase = q - p2 + rho0;
-  for (int idx = threadIdx.x; idx < L.nlam * R; idx += blockDim.x) {
-    const int li = idx / R;
-    const int ri = idx - li * R;
-    if (ri < nrho) {
-      const int64_t u = ubase + L.Lambda(li) * B + ri;
-      X[idx] = LoadX<Tx>(x, ModN(tf.a * u + tf.c, n, mask));
-    }
-  }
-  // Phase T: taps tap(t), t = k*B + rho - p2 (Window::SampleInTime and the
-  // phase of src/binner.cc:100-104 with tau factored out).
-  {
-    const double width = ip.width, sigma_f = ip.sigma_f;
-    const double delta = -0.5 / double(B);
-    for (int idx = threadIdx.x; idx < f * R; idx += blockDim.x) {
-      const int k = idx / R;
-      const int ri = idx - k * R;
-      if (ri < nrho) {
-        const int64_t t = int64_t(k) * B + rho0 + ri - p2;
-        const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
-        const int64_t r = ModN(tf.b * t, n, mask);
-        const double d = delta * double(t);
-        const double turns = double(r) / double(n) + (d - floor(d));
-        T[idx] = Scale(CisTurns(turns), w);
+  for (int base = 0; base < total; base += kGather * kThreads) {
+    Tx g[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; ++k) {
+      const int idx = base + int(threadIdx.x) + k * kThreads;
+      const int li = idx >> logR, ri = idx & (R - 1);
+      g[k] = Tx{};
+      if (idx < total && ri < nrho) {
+        const int64_t u = ubase + L.Lambda(li) * B + ri;
+        g[k] = x[ModN(tf.a * u + tf.c, n, mask)];
       }
+    }
+    if (base == 0) {
+      const double width = ip.width, sigma_f = ip.sigma_f;
+      const double delta = -0.5 / double(B);
+      for (int idx = threadIdx.x; idx < (f << logR); idx += blockDim.x) {
+        const int k = idx >> logR;
+        const int ri = idx & (R - 1);
+        if (ri < nrho) {
+          const int64_t t = int64_t(k) * B + rho0 + ri - p2;
+          const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
+          const int64_t r = ModN(tf.b * t, n, mask);
+          const double d = delta * double(t);
+          const double turns = double(r) / double(n) + (d - floor(d));
+          T[idx] = Scale(CisTurns(turns), w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGather; ++k) {
+      const int idx = base + int(threadIdx.x) + k * kThreads;
+      if (idx < total && (idx & (R - 1)) < nrho) X[idx] = Widen(g[k]);
     }
   }
   __syncthreads();
@@ -583,42 +598,196 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident Stockham FFT, the large-M path of k_bin_fft (M = 2048,
+// 4096): one first pass of radix R0 = 2^(logM mod 4) (16 when 4 divides
+// logM), then radix-16 passes.  In a pass of radix R with Ns = product of
+// the earlier radices, butterfly j < M/R takes x[j + r M/R] (r < R) into
+// registers, multiplies input r by W_{Ns R}^{k r} (k = j mod Ns), transforms
+// them, and writes output r to (j/Ns) Ns R + k + r Ns (natural order in and
+// out).  A thread holds at most 16 values (M <= 16 kFftThreads).  Between
+// passes the array sits in LDS with one element of padding per 16, which
+// spreads the early passes' strided writes over the banks.  The
+// 4096-point transform is 3 LDS round trips instead of 6 radix-4 passes.
+__device__ __forceinline__ int Pad(int i) { return i + (i >> 4); }
+
+
+constexpr double kCos8 = 0.92387953251128675613;   // cos(pi/8)
+constexpr double kSin8 = 0.38268343236508977173;   // sin(pi/8)
+constexpr double kHalfR2 = 0.70710678118654752440;  // cos(pi/4)
+
+// cos and sin of 2 pi m/16, for a compile-time m.
+__device__ __forceinline__ constexpr double Cos16(int m) {
+  switch (m & 15) {
+    case 0: return 1.0;
+    case 1: case 15: return kCos8;
+    case 2: case 14: return kHalfR2;
+    case 3: case 13: return kSin8;
+    case 4: case 12: return 0.0;
+    case 5: case 11: return -kSin8;
+    case 6: case 10: return -kHalfR2;
+    case 7: case 9: return -kCos8;
+    default: return -1.0;
+  }
+}
+__device__ __forceinline__ constexpr double Sin16(int m) { return Cos16(m - 4); }
+
+// a * e^{-/+ 2 pi i m/16} (forward / inverse); m is a compile-time constant
+// after unrolling.
+template <bool INV>
+__device__ __forceinline__ double2 Rot16(double2 a, int m) {
+  const double cs = Cos16(m), sn = INV ? Sin16(m) : -Sin16(m);
+  return make_double2(a.x * cs - a.y * sn, a.x * sn + a.y * cs);
+}
+
+template <bool INV>
+__device__ __forceinline__ void Dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+  const double2 t0 = Add(a0, a2), t1 = Sub(a0, a2), t2 = Add(a1, a3);
+  const double2 d = Sub(a1, a3);
+  const double2 t3 = INV ? MulPosI(d) : MulNegI(d);
+  a0 = Add(t0, t2);
+  a2 = Sub(t0, t2);
+  a1 = Add(t1, t3);
+  a3 = Sub(t1, t3);
+}
+
+// In-place R-point DFT of v[0..R-1], as R = R1 R2 with n = n1 + R1 n2,
+// k = k1 + R2 k2:
+//   X[k1 + R2 k2] = sum_n1 W_R1^(n1 k2) W_R^(n1 k1) sum_n2 W_R2^(n2 k1) x[n1 + R1 n2].
+// Inputs in natural order; output k sits in register DftSlot<R>(k).
+template <int R>
+__device__ __forceinline__ constexpr int DftSlot(int k) {
+  if constexpr (R == 8) return 2 * (k & 3) + (k >> 2);   // X[k1 + 4 k2] in v[2 k1 + k2]
+  if constexpr (R == 16) return 4 * (k & 3) + (k >> 2);  // X[k1 + 4 k2] in v[4 k1 + k2]
+  return k;
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void Dft(double2 *v) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = Add(a, b);
+    v[1] = Sub(a, b);
+  } else if constexpr (R == 4) {
+    Dft4<INV>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {  // R1 = 2, R2 = 4
+    Dft4<INV>(v[0], v[2], v[4], v[6]);
+    Dft4<INV>(v[1], v[3], v[5], v[7]);
+    v[3] = Rot16<INV>(v[3], 2);  // n1 = 1: W_8^k1 = W_16^(2 k1)
+    v[5] = Rot16<INV>(v[5], 4);
+    v[7] = Rot16<INV>(v[7], 6);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      const double2 x0 = v[2 * k1], x1 = v[2 * k1 + 1];
+      v[2 * k1] = Add(x0, x1);
+      v[2 * k1 + 1] = Sub(x0, x1);
+    }
+  } else {  // R == 16: R1 = R2 = 4
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) Dft4<INV>(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+#pragma unroll
+    for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) v[n1 + 4 * k1] = Rot16<INV>(v[n1 + 4 * k1], n1 * k1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) Dft4<INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+  }
+}
+
+// Loads the inputs of this thread's butterflies of a radix-R pass:
+// v[b R + r] = a[j + r M/R], j = tid + b T.
+template <int R>
+__device__ __forceinline__ void PassLoad(const double2 *a, double2 (&v)[16], int logM) {
+  constexpr int NB = 16 / R;
+  const int nb = (1 << logM) / R;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = int(threadIdx.x) + b * kFftThreads;
+    if (j < nb)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b * R + r] = a[Pad(j + r * nb)];
+  }
+}
+
+// Twist (input r by w^r, w = W_{Ns R}^k, given for b = 0 in w1), transform
+// and store this thread's butterflies of a radix-R pass.  Passes with
+// Ns > 1 have one butterfly per thread.
+template <int R, bool INV>
+__device__ __forceinline__ void PassComputeStore(double2 *a, double2 (&v)[16], int logM, int Ns,
+                                                 double2 w1) {
+  constexpr int NB = 16 / R;
+  const int nb = (1 << logM) / R;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = int(threadIdx.x) + b * kFftThreads;
+    if (j < nb) {
+      double2 *u = v + b * R;
+      if (Ns > 1) {
+        // w^r by a squaring tree (depth <= 4 products) rather than a chain
+        double2 wp[R];
+        wp[1] = INV ? Conj(w1) : w1;
+#pragma unroll
+        for (int r = 2; r < R; ++r) wp[r] = (r & 1) ? Cmul(wp[r - 1], wp[1]) : Cmul(wp[r / 2], wp[r / 2]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = Cmul(u[r], wp[r]);
+      }
+      Dft<R, INV>(u);
+      const int k = j & (Ns - 1);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[Pad(base + r * Ns)] = u[DftSlot<R>(r)];
+    }
+  }
+}
+
+// Runtime radix to template dispatch: load, or twist + transform + store.
+__device__ __forceinline__ void LoadR(const double2 *a, double2 (&v)[16], int logM, int R) {
+  switch (R) {
+    case 2: PassLoad<2>(a, v, logM); break;
+    case 4: PassLoad<4>(a, v, logM); break;
+    case 8: PassLoad<8>(a, v, logM); break;
+    default: PassLoad<16>(a, v, logM); break;
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void ComputeStoreR(double2 *a, double2 (&v)[16], int logM, int R, int Ns,
+                                              double2 w1) {
+  switch (R) {
+    case 2: PassComputeStore<2, INV>(a, v, logM, Ns, w1); break;
+    case 4: PassComputeStore<4, INV>(a, v, logM, Ns, w1); break;
+    case 8: PassComputeStore<8, INV>(a, v, logM, Ns, w1); break;
+    default: PassComputeStore<16, INV>(a, v, logM, Ns, w1); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_bin_fft: per (trial, row) the forward B-point DFT of the fold by
-// Bluestein (X_k = c_k sum_j (v_j c_j) conj(c_{k-j})) over two LDS FFTs of
-// M = 2^logM >= 2B-1 points, then the row phase e^{2 pi i b tau/n} and the
+// Bluestein (X_k = c_k sum_j (v_j c_j) conj(c_{k-j})) over two M-point
+// FFTs (M = 2^logM >= 2B-1), then the row phase e^{2 pi i b tau/n} and the
 // BinInFreqV1 subtraction (src/binner.cc:565-591) of every found mode
-// (terms from k_bin_time).  The twiddle table is staged in LDS when
-// M <= kLdsTwMaxM.
-__global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
-  HIP_DYNAMIC_SHARED(double2, lds)
+// (terms from k_bin_time).  Every global load is issued up front.  Two
+// paths, chosen by M for the whole launch:
+//   M <= 1024 (B <= 512): in-place radix-4 passes in LDS (bit-reversed
+//     DIT forward, DIF inverse), twiddles staged in LDS; up to 256
+//     butterflies per pass keep every thread busy at small M;
+//   M = 2048, 4096: the register Stockham passes above, the forward
+//     transform starting from registers and the filter product applied as
+//     the inverse transform's first pass loads its inputs.
+// Either path leaves X_k (chirp and row phase applied) in LDS, at k on the
+// small path and at Pad(k) on the large one.
+
+__device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, const double2 *fold,
+                                            double2 rp, double2 *a) {
   constexpr int T = kFftThreads;
-  constexpr int kPerB = kMaxLdsBins / T;         // fold / chirp entries per thread
-  constexpr int kPerM = kMaxM / T;               // filter entries per thread
-  constexpr int kPerTw = (kLdsTwMaxM / 2) / T;   // twiddles staged per thread
-  constexpr int kPerTerm = 2;                    // BinInFreq terms prefetched per thread
-  const DevState *st = c.st;
-  const IterParams &ip = st->P[p];
-  if (ip.done || ip.logM > kMaxLogM) return;
-  const int32_t trials = st->trials, rows = ip.rows;
-  const int32_t trial = blockIdx.x / c.rows_cap;
-  const int32_t row = blockIdx.x - trial * c.rows_cap;
-  if (trial >= trials || row >= rows) return;
+  constexpr int kPerB = 512 / T;   // fold / chirp entries per thread (B <= 512)
+  constexpr int kPerM = 1024 / T;  // filter entries per thread
+  constexpr int kPerTw = 512 / T;  // twiddles staged per thread
   const int32_t B = ip.B, logM = ip.logM;
   const int M = 1 << logM;
   const int tid = threadIdx.x;
-  const int64_t n = c.n, mask = c.mask;
   const double2 *chirp = c.table + ip.tab_off;
   const double2 *filt = chirp + B;
-  const double2 *fold = c.fold + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  const int32_t cnt = min(st->count, c.cap);
-  const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
-  const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
-  const bool tw_lds = M <= kLdsTwMaxM;
-
-  // Every global load the workgroup needs is issued here, before the first
-  // barrier, so the kernel waits on memory once instead of once per stage.
-  double2 fv[kPerB], cv[kPerB], hv[kPerM], twv[kPerTw], tval[kPerTerm];
-  int tbin[kPerTerm];
+  double2 fv[kPerB], cv[kPerB], hv[kPerM], twv[kPerTw];
 #pragma unroll
   for (int r = 0; r < kPerB; ++r) {
     const int k = tid + r * T;
@@ -636,32 +805,15 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
 #pragma unroll
   for (int r = 0; r < kPerTw; ++r) {
     const int j = tid + r * T;
-    twv[r] = (tw_lds && j < M / 2) ? c.tw[j * (kTwM / M)] : make_double2(0.0, 0.0);
+    twv[r] = j < M / 2 ? c.tw[j << (kTwLogM - logM)] : make_double2(0.0, 0.0);
   }
+  double2 *twl = a + M;
 #pragma unroll
-  for (int r = 0; r < kPerTerm; ++r) {
-    const int i = tid + r * T;
-    tbin[r] = 0;
-    tval[r] = make_double2(0.0, 0.0);
-    if (i < cnt) {
-      tbin[r] = tb[i];
-      tval[r] = tv[i];
-    }
+  for (int r = 0; r < kPerTw; ++r) {
+    const int j = tid + r * T;
+    if (j < M / 2) twl[j] = twv[r];
   }
-  const int64_t tau = ModN(ip.q[trial] + RowOffset(row) * B, n, mask);
-  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
-
-  double2 *a = lds;
-  Twiddles Wf{c.tw, kTwM / M, false};
-  if (tw_lds) {
-    double2 *twl = lds + M;
-#pragma unroll
-    for (int r = 0; r < kPerTw; ++r) {
-      const int j = tid + r * T;
-      if (j < M / 2) twl[j] = twv[r];
-    }
-    Wf = Twiddles{twl, 1, false};
-  }
+  const Twiddles Wf{twl, 1, false};
 #pragma unroll
   for (int r = 0; r < kPerM; ++r) {
     const int m = tid + r * T;
@@ -682,7 +834,6 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   Twiddles Wi = Wf;
   Wi.inverse = true;
   FftDif(a, logM, Wi);
-
   double2 vals[kPerB];
 #pragma unroll
   for (int r = 0; r < kPerB; ++r) {
@@ -695,23 +846,147 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
     const int k = tid + r * T;
     if (k < B) a[k] = vals[r];
   }
+}
+
+__device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, const double2 *fold,
+                                            double2 rp, double2 *a) {
+  constexpr int T = kFftThreads;
+  const int32_t B = ip.B, logM = ip.logM;
+  const int M = 1 << logM;
+  const int tid = threadIdx.x;
+  const double2 *chirp = c.table + ip.tab_off;
+  const double2 *filt = chirp + B;
+  const int r0 = logM & 3;
+  const int R0 = r0 ? (1 << r0) : 16;
+  const int lr0 = r0 ? r0 : 4;
+  const int nb0 = M / R0, NB0 = 16 / R0;
+  const int npass = (logM >> 2) + (r0 ? 1 : 0);  // passes after the first are radix 16
+
+  // Prefetch, in the first pass's layout (element b R0 + r is index
+  // j + r nb0, j = tid + b T): fold x chirp and the filter spectrum.  The
+  // chirp itself waits in LDS (after the padded array) for the last step.
+  double2 *lchirp = a + Pad(M);
+  double2 v[16], hv[16], wv[2];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int b = e >> lr0, r = e & (R0 - 1);
+    const int j = tid + b * T;
+    const int m = j + r * nb0;
+    v[e] = hv[e] = make_double2(0.0, 0.0);
+    if (b < NB0 && j < nb0) {
+      if (m < B) {
+        const double2 cm = chirp[m];
+        lchirp[m] = cm;
+        v[e] = Cmul(fold[m], cm);
+      }
+      hv[e] = filt[m];
+    }
+  }
+  // w1 of the radix-16 passes 1, 2 for butterfly j = tid:
+  // W_{Ns 16}^(j mod Ns), Ns = R0 16^(q-1).
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    wv[q] = make_double2(1.0, 0.0);
+    const int logNs = lr0 + 4 * q;  // pass q + 1
+    if (q + 1 < npass && tid < M / 16) {
+      const int k = tid & ((1 << logNs) - 1);
+      wv[q] = c.tw[k << (kTwLogM - logNs - 4)];
+    }
+  }
+  // Forward transform.
+  ComputeStoreR<false>(a, v, logM, R0, 1, make_double2(1.0, 0.0));
+  __syncthreads();
+#pragma unroll
+  for (int q = 1; q < 3; ++q) {
+    if (q < npass) {
+      PassLoad<16>(a, v, logM);
+      __syncthreads();
+      PassComputeStore<16, false>(a, v, logM, R0 << (4 * (q - 1)), wv[q - 1]);
+      __syncthreads();
+    }
+  }
+  // Inverse transform of (spectrum x filter); the filter is pre-scaled by 1/M.
+  LoadR(a, v, logM, R0);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = Cmul(v[e], hv[e]);
+  __syncthreads();
+  ComputeStoreR<true>(a, v, logM, R0, 1, make_double2(1.0, 0.0));
+  __syncthreads();
+#pragma unroll
+  for (int q = 1; q < 3; ++q) {
+    if (q < npass) {
+      PassLoad<16>(a, v, logM);
+      __syncthreads();
+      PassComputeStore<16, true>(a, v, logM, R0 << (4 * (q - 1)), wv[q - 1]);
+      __syncthreads();
+    }
+  }
+  // X_k = c_k y_k, times the row phase, in place (each thread its own k's).
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int b = e >> lr0, r = e & (R0 - 1);
+    const int j = tid + b * T;
+    const int m = j + r * nb0;
+    if (b < NB0 && j < nb0 && m < B) a[Pad(m)] = Cmul(Cmul(a[Pad(m)], lchirp[m]), rp);
+  }
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  constexpr int T = kFftThreads;
+  constexpr int kPerTerm = 2;  // BinInFreq terms prefetched per thread
+  const DevState *st = c.st;
+  const IterParams &ip = st->P[p];
+  if (ip.done || ip.logM > kMaxLogM) return;
+  const int32_t trials = st->trials, rows = ip.rows;
+  const int32_t trial = blockIdx.x / c.rows_cap;
+  const int32_t row = blockIdx.x - trial * c.rows_cap;
+  if (trial >= trials || row >= rows) return;
+  const int32_t B = ip.B;
+  const int tid = threadIdx.x;
+  const int64_t n = c.n, mask = c.mask;
+  const double2 *fold = c.fold + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
+  const int32_t cnt = min(st->count, c.cap);
+  const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
+  const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
+  double2 tval[kPerTerm];
+  int tbin[kPerTerm];
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) {
+    const int i = tid + r * T;
+    tbin[r] = 0;
+    tval[r] = make_double2(0.0, 0.0);
+    if (i < cnt) {
+      tbin[r] = tb[i];
+      tval[r] = tv[i];
+    }
+  }
+  const int64_t tau = ModN(ip.q[trial] + RowOffset(row) * B, n, mask);
+  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+  double2 *a = lds;
+  const bool small = ip.logM <= kFftSmallLogM;
+  if (small)
+    BinFftSmall(c, ip, fold, rp, a);
+  else
+    BinFftLarge(c, ip, fold, rp, a);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kPerTerm; ++r) {
     if (tid + r * T < cnt) {
-      atomicAdd(&a[tbin[r]].x, -tval[r].x);
-      atomicAdd(&a[tbin[r]].y, -tval[r].y);
+      const int i = small ? tbin[r] : Pad(tbin[r]);
+      atomicAdd(&a[i].x, -tval[r].x);
+      atomicAdd(&a[i].y, -tval[r].y);
     }
   }
   for (int i = tid + kPerTerm * T; i < cnt; i += T) {
-    const double2 v = tv[i];
-    const int bin = tb[i];
-    atomicAdd(&a[bin].x, -v.x);
-    atomicAdd(&a[bin].y, -v.y);
+    const double2 tvi = tv[i];
+    const int bin = small ? tb[i] : Pad(tb[i]);
+    atomicAdd(&a[bin].x, -tvi.x);
+    atomicAdd(&a[bin].y, -tvi.y);
   }
   __syncthreads();
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  for (int k = tid; k < B; k += T) A[k] = a[k];
+  for (int k = tid; k < B; k += T) A[k] = a[small ? k : Pad(k)];
 }
 
 // ---------------------------------------------------------------------------

@@ -78,6 +78,7 @@ inline unsigned __brev(unsigned v) {
   return r;
 }
 
+inline double sinpi(double x) { return std::sin(M_PI * std::fmod(x, 2.0)); }
 inline void sincospi(double x, double *s, double *c) {
   const double r = std::fmod(x, 2.0);
   *s = std::sin(M_PI * r);
