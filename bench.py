@@ -43,29 +43,69 @@ def time_solves(M, solver, xd, steps, warmup):
     return (time.perf_counter() - t0) / steps * 1e6, iters
 
 
-def gather_bytes(M, solver, elem_bytes):
+def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
     """Algorithmic bytes of every k_bin_time launch of the last solve: 64 B
     per distinct 64-B line of x its gathers touch, plus the fold it writes
     (rows x B double2).  Also the reference's logical gather bytes."""
     bins_log, _ = solver.log()
     tfs, qs = solver.transforms()
     line_bytes = logical = 0
+    per_line = 64 // elem_bytes
+    seen = np.zeros(n // per_line, dtype=bool)
     for it, B in enumerate(bins_log):
         B = int(B)
-        P = M.lib().mpsft_window_p(N, B, DELTA)
+        P = M.lib().mpsft_window_p(n, B, delta)
         p2 = (P - 1) // 2
-        bits = M.num_bits(N, B)
+        bits = M.num_bits(n, B)
         a, c = int(tfs[it, 0]), int(tfs[it, 2])
+        rows = 1 + 2 * bits
         for q in qs[it]:
             offs = [0] + [s * B * (1 << k) for k in range(bits) for s in (1, -1)]
-            u = np.unique(np.concatenate([np.arange(o - p2, o + p2 + 1, dtype=np.int64) + int(q)
-                                          for o in offs]))
-            j = (a * (u % N) + c) % N
-            lines = np.unique(j * elem_bytes // 64).size
-            rows = 1 + 2 * bits
-            line_bytes += 64 * lines + rows * B * 16
+            seen[:] = False
+            for o in offs:
+                u = np.arange(o - p2, o + p2 + 1, dtype=np.int64) + int(q)
+                seen[((a * (u % n) + c) % n) // per_line] = True
+            line_bytes += 64 * int(seen.sum()) + rows * B * 16
             logical += rows * P * elem_bytes
     return line_bytes, logical, len(bins_log)
+
+
+def large_workload(M, steps, warmup):
+    """N=2^26 complex64, S=4000 unit modes (1000 within +-64 bins of 10
+    centres), sigma=1e-6, MIXED: us per solve and the gather-fold's bytes/s."""
+    from mpsft_amd.signals import make_signal, quantization_sigma
+    n, k, sigma = 1 << 26, 4000, 1e-6
+    log("generating the N=2^26, S=4000 signal")
+    x, truth = make_signal(n, k, sigma, seed=3, clusters=(10, 0.25, 64))
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    del x
+    s = M.Solver(n, M.solve_options(k, sig_eff, window_delta=DELTA), M.MPSFT_MIXED)
+    xd = s.upload(x64)
+    M.synchronize()
+    log("timing the N=2^26 solve")
+    us, iters = time_solves(M, s, xd, steps, warmup)
+    got = s.modes_dict()
+    hits = sum(1 for key in truth if key in got)
+    err = max(abs(got[key] - v) / abs(v) for key, v in truth.items() if key in got)
+    _, ms = s.solve_profiled(xd, SOLVE_SEED)
+    bins_log, _ = s.log()
+    line_bytes, logical, n_launch = gather_bytes(M, s, 8, n=n)
+    t = ms[0] / 1000.0
+    return {
+        "workload": "MPSFT solve N=2^26 S=4000 (25% in 10 clusters of +-64) unit |c| sigma=1e-6, "
+                    "complex64 MIXED",
+        "us_per_solve": round(us, 1), "solves_per_sec": round(1e6 / us, 1), "iterations": iters,
+        "first_bins": int(bins_log[0]), "sigma_eff": sig_eff, "true_modes_recovered": hits,
+        "max_rel_coef_err_vs_truth": err,
+        "kernel_ms_per_solve": {"k_bin_time": round(ms[0], 4), "bin_fft": round(ms[1], 4),
+                                "k_identify": round(ms[2], 4)},
+        "bin_in_time": {"algorithmic_bytes_per_solve": line_bytes,
+                        "effective_GBs": round(line_bytes / t / 1e9, 1),
+                        "frac_of_8TBs": round(line_bytes / t / 1e9 / HBM_PEAK_GBS, 4),
+                        "logical_gather_GBs": round(logical / t / 1e9, 1),
+                        "launches": n_launch},
+    }
 
 
 def cpu_baseline(x, sample_solves):
@@ -122,6 +162,8 @@ def main():
                     help="run only the instrumented MIXED solve (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
+    ap.add_argument("--config3", action="store_true",
+                    help="only the N=2^26 S=4000 clustered MIXED workload (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one signal's solve per device); "
@@ -133,6 +175,10 @@ def main():
 
     import mpsft_amd as M
     from mpsft_amd.signals import make_signal, quantization_sigma
+
+    if args.config3:
+        print(json.dumps(large_workload(M, args.steps, args.warmup)), flush=True)
+        return
 
     log("generating the N=2^22, S=1000 signal")
     x, truth = make_signal(N, K, SIGMA, seed=SIGNAL_SEED, magnitude="loguniform")

@@ -174,14 +174,14 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
     return;
   }
   int32_t *hp = nullptr;
-  if (hipHostMalloc(reinterpret_cast<void **>(&hp), 2 * sizeof(int32_t),
+  if (hipHostMalloc(reinterpret_cast<void **>(&hp), 4 * sizeof(int32_t),
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer(reinterpret_cast<void **>(&d_progress_), hp, 0) != hipSuccess) {
     status_ = MPSFT_ERR_NOMEM;
     return;
   }
   h_progress_ = hp;
-  hp[0] = hp[1] = 0;
+  hp[0] = hp[1] = hp[2] = hp[3] = 0;
   std::memset(h_state_, 0, sizeof(DevState));
   if (hipMemset(d_state_, 0, sizeof(DevState)) != hipSuccess ||
       hipMemset(d_coef_, 0, size_t(n_) * sizeof(double2)) != hipSuccess ||
@@ -195,7 +195,8 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
 Engine::~Engine() {
   // Work a caller left queued may still use these buffers.
   (void)hipDeviceSynchronize();
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  for (auto e : graph_exec_)
+    if (e) (void)hipGraphExecDestroy(e);
   (void)hipFree(d_state_);
   (void)hipFree(d_btab_);
   (void)hipFree(d_table_);
@@ -482,45 +483,57 @@ int32_t Engine::IdentifyBlocks() const {
 }
 
 // The bin FFT of every B the current call may use: k_bin_fft for
-// B <= kMaxLdsBins, the four-step kernels above; each exits at once for an
-// iteration whose B is the other kind.
-int Engine::LaunchFft(const Ctx &c, int parity, hipStream_t s) {
+// B <= kMaxLdsBins, the four-step kernels above (unless big is false); each
+// exits at once for an iteration whose B is the other kind.
+int Engine::LaunchFft(const Ctx &c, int parity, hipStream_t s, bool big) {
   if (fft_small_) HIP_OK(LaunchBinFft(c, parity, LdsFft(), s));
-  if (fft_big_) HIP_OK(LaunchBigFft(c, parity, s));
+  if (fft_big_ && big) HIP_OK(LaunchBigFft(c, parity, s));
   return MPSFT_OK;
 }
 
-int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s) {
+int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big) {
   HIP_OK(LaunchBinTime(c, parity, TilesCap(), LdsBinTime(), s));
-  const int rc = LaunchFft(c, parity, s);
+  const int rc = LaunchFft(c, parity, s, big);
   if (rc != MPSFT_OK) return rc;
   HIP_OK(LaunchIdentify(c, parity, IdentifyBlocks(), s));
   return MPSFT_OK;
 }
 
+// Captures `iters` iterations (parities 0, 1, ...) into graph_exec_[0] and,
+// when the call reserves both bin-FFT kinds, the same without the four-step
+// kernels into graph_exec_[1]: a solve's B never grows, so once the device
+// reports B <= kMaxLdsBins the host launches the shorter graph.
 int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
-  if (graph_exec_ && graph_key_ == shape_gen_ && graph_iters_ == iters) return MPSFT_OK;
-  if (graph_exec_) {
-    (void)hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
+  const int want = (fft_big_ && fft_small_) ? 2 : 1;
+  if (graph_exec_[0] && graph_key_ == shape_gen_ && graph_iters_ == iters &&
+      (want == 1 || graph_exec_[1]))
+    return MPSFT_OK;
+  for (auto &e : graph_exec_) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
   }
-  hipStream_t cs;
-  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   const Ctx c = MakeCtx();
-  hipGraph_t g;
-  HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  int rc = MPSFT_OK;
-  // iters is even: the graph starts at an even iteration, so node i runs
-  // with parity i & 1.
-  for (int32_t i = 0; i < iters && rc == MPSFT_OK; ++i) rc = LaunchIteration(c, i & 1, cs);
-  const hipError_t ec = hipStreamEndCapture(cs, &g);
-  (void)hipStreamDestroy(cs);
-  if (rc != MPSFT_OK || ec != hipSuccess) return MPSFT_ERR_HIP;
-  const hipError_t ei = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (ei != hipSuccess) {
-    graph_exec_ = nullptr;
-    return MPSFT_ERR_HIP;
+  for (int k = 0; k < want; ++k) {
+    hipStream_t cs;
+    HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return MPSFT_ERR_HIP;
+    }
+    int rc = MPSFT_OK;
+    // iters is even: the graph starts at an even iteration, so node i runs
+    // with parity i & 1.
+    for (int32_t i = 0; i < iters && rc == MPSFT_OK; ++i) rc = LaunchIteration(c, i & 1, cs, k == 0);
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc != MPSFT_OK || ec != hipSuccess) return MPSFT_ERR_HIP;
+    const hipError_t ei = hipGraphInstantiate(&graph_exec_[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      graph_exec_[k] = nullptr;
+      return MPSFT_ERR_HIP;
+    }
   }
   graph_iters_ = iters;
   graph_key_ = shape_gen_;
@@ -561,6 +574,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   Ctx c = MakeCtx();
   h_progress_[0] = 0;  // the previous solve has finished; the device writes
   h_progress_[1] = 0;  // these again only after k_init below
+  h_progress_[2] = 0;
   HIP_OK(LaunchClearModes(c, 0, s));
   InitArgs a{};
   a.x = x;
@@ -655,7 +669,11 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
           if (rc == MPSFT_OK) rc = EnsureGraph(2, s);
           if (rc != MPSFT_OK) return fail(rc);
         }
-        if (hipGraphLaunch(graph_exec_, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+        // progress[2] is the B of the latest iteration the device started
+        const int32_t bnow = h_progress_[2];
+        hipGraphExec_t ge = (graph_exec_[1] && bnow > 0 && bnow <= kMaxLdsBins) ? graph_exec_[1]
+                                                                                 : graph_exec_[0];
+        if (hipGraphLaunch(ge, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
         launched += 2;
         continue;
       }

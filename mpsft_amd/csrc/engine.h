@@ -124,7 +124,7 @@ struct Ctx {
   double2 *coef;         // n (dense found coefficients)
   uint32_t *present;     // ceil(n/32) bits
   int32_t *log;          // 2 x log_cap
-  int32_t *progress;     // host-mapped {iterations done, done flag}, or null
+  int32_t *progress;     // host-mapped {iteration started, stopped, its B, -}, or null
   int32_t log_cap;
   int32_t cap;
   int32_t trials_cap, rows_cap, B_cap;
@@ -168,8 +168,8 @@ class Engine {
   int EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s);
   int GrowTransforms(int32_t count, bool *grown);
   Ctx MakeCtx() const;
-  int LaunchIteration(const Ctx &c, int parity, hipStream_t s);
-  int LaunchFft(const Ctx &c, int parity, hipStream_t s);
+  int LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big = true);
+  int LaunchFft(const Ctx &c, int parity, hipStream_t s, bool big = true);
   int EnsureGraph(int32_t iters, hipStream_t s);
   int ReadState(hipStream_t s);
   size_t LdsBinTime() const;
@@ -248,7 +248,7 @@ class Engine {
   std::vector<int64_t> h_qs_;
 
   // graph of graph_iters_ iterations for the current shapes
-  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // [1]: without the four-step FFT
   int32_t graph_iters_ = 0;
   int64_t graph_key_ = -1;
   int64_t shape_gen_ = 0;

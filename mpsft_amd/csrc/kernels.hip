@@ -317,6 +317,7 @@ __global__ void k_init(Ctx c, InitArgs a) {
   if (c.progress) {
     __hip_atomic_store(&c.progress[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&c.progress[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c.progress[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -385,6 +386,8 @@ __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
     st->stop_stale = lc.stale;
   }
   if (c.progress) {
+    if (!lc.done)
+      __hip_atomic_store(&c.progress[2], ip.B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&c.progress[1], lc.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -1177,18 +1180,46 @@ __global__ void __launch_bounds__(kThreads) k_big_out(Ctx c, int p) {
 // estimate reduced across the lanes; un-permutation and mm[k0] += coef.  FP
 // contraction is off so the decision arithmetic rounds as the reference's
 // does.
-__device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v) {
+// mm[k0] += v.  A key not yet present is appended to the workgroup's list
+// (LDS); AppendKeys later reserves the list's slots in c.keys with one
+// global atomic per workgroup instead of one per key (a single counter
+// updated by every inserting wave serialises them: ~1500 inserts took
+// ~30 us at B = 8001).
+constexpr int kIdentWaves = kThreads / 64;
+__device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v, int32_t *ncnt,
+                                           int64_t *nkeys) {
   const uint32_t bit = 1u << (k0 & 31);
   const uint32_t old = atomicOr(&c.present[k0 >> 5], bit);
   if (!(old & bit)) {
-    const int32_t idx = atomicAdd(&c.st->count, 1);
-    if (idx < c.cap)
-      c.keys[idx] = k0;
-    else
-      atomicOr(&c.st->flags, kFlagOverflow);
+    const int32_t li = atomicAdd(ncnt, 1);  // LDS
+    if (li < kIdentWaves) {
+      nkeys[li] = k0;
+    } else {  // more new keys than waves (a wave that handled several bins)
+      const int32_t idx = atomicAdd(&c.st->count, 1);
+      if (idx < c.cap)
+        c.keys[idx] = k0;
+      else
+        atomicOr(&c.st->flags, kFlagOverflow);
+    }
   }
   atomicAdd(&c.coef[k0].x, v.x);
   atomicAdd(&c.coef[k0].y, v.y);
+}
+
+// After the workgroup's bins: reserve slots for its new keys and store them.
+__device__ __forceinline__ void AppendKeys(const Ctx &c, int32_t *ncnt, int32_t *nbase,
+                                           const int64_t *nkeys) {
+  __syncthreads();
+  const int32_t m = min(*ncnt, kIdentWaves);
+  if (threadIdx.x == 0 && m > 0) *nbase = atomicAdd(&c.st->count, m);
+  __syncthreads();
+  if (int(threadIdx.x) < m) {
+    const int32_t idx = *nbase + int(threadIdx.x);
+    if (idx < c.cap)
+      c.keys[idx] = nkeys[threadIdx.x];
+    else
+      atomicOr(&c.st->flags, kFlagOverflow);
+  }
 }
 
 __device__ __forceinline__ double WaveSum(double v) {
@@ -1224,7 +1255,13 @@ __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
   const int64_t rstride = c.B_cap, tstride = int64_t(c.rows_cap) * c.B_cap;
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
-  bool any = false;
+  __shared__ int32_t ncnt, nbase, nany;
+  __shared__ int64_t nkeys[kIdentWaves];
+  if (threadIdx.x == 0) {
+    ncnt = 0;
+    nany = 0;
+  }
+  __syncthreads();
 
   for (int32_t b = blockIdx.x * waves + (threadIdx.x >> 6); b < B; b += gridDim.x * waves) {
     const double2 *Ab = c.A + b;
@@ -1311,11 +1348,12 @@ __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
       const int64_t k0 = ModN(tf.a_inv * ModN(k1 - tf.b, n, mask), n, mask);
       coef = Cmul(coef, Conj(CisFrac(ModN(tf.c * k0, n, mask), n)));
       coef = make_double2(coef.x / wf, coef.y / wf);
-      InsertMode(c, k0, coef);
-      any = true;
+      InsertMode(c, k0, coef, &ncnt, nkeys);
+      nany = 1;
     }
   }
-  if (any) atomicOr(&st->found_iter[p], 1);
+  AppendKeys(c, &ncnt, &nbase, nkeys);
+  if (threadIdx.x == 0 && nany) atomicOr(&st->found_iter[p], 1);
 }
 
 // BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix: one
