@@ -108,6 +108,71 @@ def large_workload(M, steps, warmup):
     }
 
 
+class _DevPtr:
+    """A torch cuda tensor seen through the C-ABI's device-pointer argument."""
+
+    def __init__(self, t):
+        import ctypes
+        self.t = t
+        self.ptr = ctypes.c_void_p(t.data_ptr())
+
+
+def batch_workload(M, steps, warmup, count=256):
+    """Batched uneven instances: `count` complex64 signals, n = 2^k with k
+    uniform in [16, 22], S uniform in [10, 500], unit coefficients with
+    uniform phases, sigma = 1e-6, synthesised on the GPU.  Solves/s of one
+    batched solve (tile width from the autotuner) against a serial loop of
+    single-signal solves of the same signals."""
+    import torch
+    from mpsft_amd.signals import make_signal_device
+    rng = np.random.default_rng(4)
+    ns = [1 << int(v) for v in rng.integers(16, 23, count)]
+    ks = [int(v) for v in rng.integers(10, 501, count)]
+    seeds = [int(v) for v in rng.integers(1, 1 << 30, count)]
+    sigma = 1e-6
+    log(f"synthesising {count} signals on the GPU (total {sum(ns) / 1e6:.0f} M samples)")
+    xs, opts, truths = [], [], []
+    for i, (n, k) in enumerate(zip(ns, ks)):
+        x, sq, truth = make_signal_device(n, k, sigma, seed=7000 + i)
+        xs.append(_DevPtr(x))
+        truths.append(truth)
+        opts.append(M.solve_options(k, float(np.hypot(sigma, sq)), window_delta=DELTA))
+    torch.cuda.synchronize()
+    b = M.Batch(ns, opts, M.MPSFT_MIXED)
+    log("autotuning the batch")
+    tile = b.autotune(xs, seeds)
+    for _ in range(warmup):
+        b.solve(xs, seeds)
+    M.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        iters, stats = b.solve(xs, seeds)
+    M.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    log("serial loop of single-signal solves")
+    solvers = [M.Solver(n, o, M.MPSFT_MIXED) for n, o in zip(ns, opts)]
+    for s_, x, sd in zip(solvers, xs, seeds):  # warm: tables, graphs
+        s_.solve(x, sd)
+    M.synchronize()
+    t0 = time.perf_counter()
+    for s_, x, sd in zip(solvers, xs, seeds):
+        s_.solve(x, sd)
+    M.synchronize()
+    t_serial = time.perf_counter() - t0
+    same = sum(1 for i in range(count) if set(b.modes_dict(i)) == set(solvers[i].modes_dict()))
+    hits = sum(sum(1 for key in truths[i] if key in b.modes_dict(i)) for i in range(count))
+    return {
+        "workload": f"{count} complex64 signals, n=2^U[16,22], S=U[10,500], unit |c|, "
+                    "sigma=1e-6, MIXED, one batched solve",
+        "signals": count, "total_samples": int(sum(ns)), "total_modes": int(sum(ks)),
+        "batched_ms": round(t_batch * 1e3, 3), "batched_solves_per_sec": round(count / t_batch, 1),
+        "serial_ms": round(t_serial * 1e3, 3), "serial_solves_per_sec": round(count / t_serial, 1),
+        "speedup_vs_serial": round(t_serial / t_batch, 2), "tile_residues": tile,
+        "max_iterations": int(max(iters)), "status_ok": int(sum(1 for v in stats if v == 0)),
+        "signals_identical_to_serial": same, "true_modes_recovered": hits,
+    }
+
+
 def cpu_baseline(x, sample_solves):
     """The FP64 CPU restatement of the reference (single thread, V4-order
     binning) on the plain FP64 workload."""
@@ -164,6 +229,8 @@ def main():
                     help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
     ap.add_argument("--config3", action="store_true",
                     help="only the N=2^26 S=4000 clustered MIXED workload (its own JSON line)")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the batched workload: 256 uneven signals (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one signal's solve per device); "
@@ -178,6 +245,9 @@ def main():
 
     if args.config3:
         print(json.dumps(large_workload(M, args.steps, args.warmup)), flush=True)
+        return
+    if args.batch:
+        print(json.dumps(batch_workload(M, args.steps, args.warmup)), flush=True)
         return
 
     log("generating the N=2^22, S=1000 signal")

@@ -193,6 +193,37 @@ int mpsft_solver_log_get(mpsft_solver *s, int32_t *bins, int32_t *found,
  * mpsft_solve drew, for its first min(cap, iterations) iterations. */
 int mpsft_solver_transforms_get(mpsft_solver *s, mpsft_transform *tfs,
                                 int64_t *qs, int32_t cap, int32_t *count);
+/* No reference counterpart: fixes the gather-fold's tile width (residues
+ * per workgroup: 1, 2, 4 or 8; 0 = the default choice).  It changes how
+ * the work is split, not what is computed. */
+int mpsft_solver_set_tile_residues(mpsft_solver *s, int32_t residues);
+
+/* ---- The batched form: no reference counterpart.  Demo1::Run
+ * (src/demo1.cc:36-59) over many independent signals at once, for callers
+ * with many small signals: every stage of an iteration is one launch over
+ * all signals, each signal keeping its own loop state and stopping on its
+ * own.  Signal i has its own n[i], options opts[i] (first bins
+ * 2 k + 1 <= 2047) and seed.  All signals share one precision. */
+typedef struct mpsft_batch mpsft_batch;
+int mpsft_batch_create(int32_t count, const int64_t *n, const mpsft_solve_options *opts,
+                       int precision, mpsft_batch **batch);
+int mpsft_batch_destroy(mpsft_batch *batch);
+/* x_dev[i]: signal i in device memory (complex128 or complex64 as the
+ * precision).  iters[i], status[i] (host, may be null): signal i's
+ * iteration count and mpsft_status; the return value is the first non-OK
+ * status, else MPSFT_OK. */
+int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
+                      int32_t *iters, int32_t *status, void *stream);
+/* Times short batched solves of these signals at tile widths 1, 2, 4, 8
+ * and keeps the fastest for later solves; *residues gets it. */
+int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
+                         int32_t *residues, void *stream);
+/* Results of signal i of the last batched solve, as the solver calls. */
+int mpsft_batch_modes_count(mpsft_batch *batch, int32_t i, int64_t *count);
+int mpsft_batch_modes_get(mpsft_batch *batch, int32_t i, int64_t *keys, double *coefs,
+                          int64_t cap, int64_t *count);
+int mpsft_batch_log_get(mpsft_batch *batch, int32_t i, int32_t *bins, int32_t *found,
+                        int32_t cap, int32_t *n_iter);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,9 @@ EXPORTS = [
     "mpsft_bin_in_freq", "mpsft_mat_pencil", "mpsft_solver_create",
     "mpsft_solver_destroy", "mpsft_solve", "mpsft_solve_profiled", "mpsft_iterate", "mpsft_solver_modes_count",
     "mpsft_solver_modes_get", "mpsft_solver_modes_set", "mpsft_solver_bins_get",
-    "mpsft_solver_log_get", "mpsft_solver_transforms_get",
+    "mpsft_solver_log_get", "mpsft_solver_transforms_get", "mpsft_solver_set_tile_residues",
+    "mpsft_batch_create", "mpsft_batch_destroy", "mpsft_batch_solve", "mpsft_batch_autotune",
+    "mpsft_batch_modes_count", "mpsft_batch_modes_get", "mpsft_batch_log_get",
 ]
 
 
@@ -122,6 +124,16 @@ def declare(L):
         "mpsft_solver_bins_get": (C.c_int, [vp, pf64, i64, pi32, pi32, pi32]),
         "mpsft_solver_log_get": (C.c_int, [vp, pi32, pi32, i32, pi32]),
         "mpsft_solver_transforms_get": (C.c_int, [vp, C.POINTER(Transform), pi64, i32, pi32]),
+        "mpsft_solver_set_tile_residues": (C.c_int, [vp, i32]),
+        "mpsft_batch_create": (C.c_int, [i32, pi64, C.POINTER(SolveOptions), C.c_int,
+                                         C.POINTER(vp)]),
+        "mpsft_batch_destroy": (C.c_int, [vp]),
+        "mpsft_batch_solve": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_uint32), pi32, pi32,
+                                        vp]),
+        "mpsft_batch_autotune": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_uint32), pi32, vp]),
+        "mpsft_batch_modes_count": (C.c_int, [vp, i32, pi64]),
+        "mpsft_batch_modes_get": (C.c_int, [vp, i32, pi64, pf64, i64, pi64]),
+        "mpsft_batch_log_get": (C.c_int, [vp, i32, pi32, pi32, i32, pi32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

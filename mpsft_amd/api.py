@@ -261,6 +261,10 @@ class Solver:
     def upload(self, x):
         return _as_device(x, self.precision)
 
+    def set_tile_residues(self, residues):
+        """Gather-fold tile width (1, 2, 4, 8; 0 = default)."""
+        check(lib().mpsft_solver_set_tile_residues(self.h, residues), "set_tile_residues")
+
     def solve(self, x_dev, seed, stream=None):
         """Demo1::Run; returns the iteration count."""
         it = C.c_int32(0)
@@ -336,3 +340,75 @@ class Solver:
         m = cnt.value
         tf = np.array([(t.a, t.b, t.c, t.a_inv) for t in tfs[:m]], np.int64).reshape(m, 4)
         return tf, qs[:m * self.opt.trials].reshape(m, self.opt.trials)
+
+
+class Batch:
+    """The batched form of Demo1::Run (no reference counterpart): many
+    independent signals, each with its own n, options and seed, recovered
+    together with one launch per stage over all of them."""
+
+    def __init__(self, ns, opts, precision=MPSFT_FP64):
+        self.ns = [int(n) for n in ns]
+        self.opts = list(opts)
+        self.precision = precision
+        self.count = len(self.ns)
+        narr = (C.c_int64 * self.count)(*self.ns)
+        oarr = (SolveOptions * self.count)(*self.opts)
+        self.h = C.c_void_p()
+        check(lib().mpsft_batch_create(self.count, narr, oarr, precision, C.byref(self.h)),
+              "mpsft_batch_create")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().mpsft_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    def upload(self, xs):
+        return [_as_device(x, self.precision) for x in xs]
+
+    def _ptrs(self, xds, seeds):
+        ptrs = (C.c_void_p * self.count)(*[d.ptr.value for d in xds])
+        sd = (C.c_uint32 * self.count)(*[int(v) for v in seeds])
+        return ptrs, sd
+
+    def solve(self, xds, seeds, stream=None):
+        """Returns (iterations, statuses) per signal; raises on a failure
+        other than a per-signal status."""
+        ptrs, sd = self._ptrs(xds, seeds)
+        it = (C.c_int32 * self.count)()
+        stv = (C.c_int32 * self.count)()
+        rc = lib().mpsft_batch_solve(self.h, ptrs, sd, it, stv, stream)
+        if rc not in (0, 6, 7):  # per-signal capacity / iteration-cap statuses are reported
+            check(rc, "mpsft_batch_solve")
+        return list(it), list(stv)
+
+    def autotune(self, xds, seeds, stream=None):
+        ptrs, sd = self._ptrs(xds, seeds)
+        r = C.c_int32(0)
+        check(lib().mpsft_batch_autotune(self.h, ptrs, sd, C.byref(r), stream),
+              "mpsft_batch_autotune")
+        return r.value
+
+    def modes_dict(self, i):
+        cnt = C.c_int64(0)
+        check(lib().mpsft_batch_modes_count(self.h, i, C.byref(cnt)), "batch_modes_count")
+        m = cnt.value
+        keys = np.zeros(max(m, 1), np.int64)
+        coefs = np.zeros(2 * max(m, 1), np.float64)
+        check(lib().mpsft_batch_modes_get(self.h, i, keys.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          coefs.ctypes.data_as(C.POINTER(C.c_double)), m,
+                                          C.byref(cnt)), "batch_modes_get")
+        return {int(a): complex(b) for a, b in zip(keys[:m], coefs.view(np.complex128)[:m])}
+
+    def log(self, i):
+        cap = 1 << 16
+        bins = np.zeros(cap, np.int32)
+        found = np.zeros(cap, np.int32)
+        n_it = C.c_int32()
+        check(lib().mpsft_batch_log_get(self.h, i, bins.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        found.ctypes.data_as(C.POINTER(C.c_int32)), cap,
+                                        C.byref(n_it)), "batch_log_get")
+        return bins[:n_it.value].copy(), found[:n_it.value].copy()
+

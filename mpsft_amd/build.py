@@ -12,7 +12,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpsft_hip.so")
-SOURCES = ["kernels.hip", "engine.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "capi.cpp"]
 ARCH = os.environ.get("MPSFT_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions stay off (they are produced by hipcc -O3 from

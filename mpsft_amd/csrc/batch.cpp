@@ -1,0 +1,312 @@
+// batch.cpp -- host side of the batched form: one Engine per signal (its
+// tables, buffers, transforms and loop state), one launch per stage over
+// all of them.
+#include "batch.h"
+
+#include <algorithm>
+#include <chrono>
+
+namespace mpsft {
+
+namespace {
+
+#define HIP_OK(expr)                                 \
+  do {                                               \
+    if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
+  } while (0)
+
+template <typename T>
+int AllocDev(T **p, size_t count) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T)) !=
+      hipSuccess) {
+    *p = nullptr;
+    return MPSFT_ERR_NOMEM;
+  }
+  return MPSFT_OK;
+}
+
+}  // namespace
+
+Batch::Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, int precision)
+    : precision_(precision) {
+  if (count <= 0 || !n || !opts || (precision != MPSFT_FP64 && precision != MPSFT_MIXED)) {
+    status_ = MPSFT_ERR_INVALID_ARG;
+    return;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    if (opts[i].k < 0 || n[i] < 2) {
+      status_ = MPSFT_ERR_INVALID_ARG;
+      return;
+    }
+    int64_t cap = 8 * int64_t(opts[i].k) + 4096;
+    if (cap > n[i]) cap = n[i];
+    eng_.emplace_back(new Engine(n[i], precision, cap));
+    if (eng_.back()->status() != MPSFT_OK) {
+      status_ = eng_.back()->status();
+      return;
+    }
+    opts_.push_back(opts[i]);
+  }
+  args_.resize(count);
+  built_gen_.assign(count, -1);
+  int32_t *hp = nullptr;
+  if (AllocDev(&d_ctxs_, count) != MPSFT_OK || AllocDev(&d_args_, count) != MPSFT_OK ||
+      AllocDev(&d_counters_, 2) != MPSFT_OK || AllocDev(&d_results_, count) != MPSFT_OK ||
+      hipHostMalloc(reinterpret_cast<void **>(&h_results_), count * sizeof(BatchResultDev)) !=
+          hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&hp), 2 * sizeof(int32_t),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void **>(&d_host_), hp, 0) != hipSuccess) {
+    status_ = MPSFT_ERR_NOMEM;
+    return;
+  }
+  h_host_ = hp;
+  hp[0] = hp[1] = 0;
+}
+
+Batch::~Batch() {
+  (void)hipDeviceSynchronize();
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  (void)hipFree(d_ctxs_);
+  (void)hipFree(d_args_);
+  (void)hipFree(d_map_);
+  (void)hipFree(d_nblk_ident_);
+  (void)hipFree(d_counters_);
+  (void)hipFree(d_results_);
+  if (h_results_) (void)hipHostFree(h_results_);
+  if (h_host_) (void)hipHostFree(const_cast<int32_t *>(h_host_));
+}
+
+// Per-signal set-up (Engine::PrepareSolve) and the Ctx array the batched
+// kernels read.  The batched kernels cover bin FFTs of one workgroup per
+// row only (B <= kMaxLdsBins).
+int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!x || !seeds) return MPSFT_ERR_INVALID_ARG;
+  seeds_.assign(seeds, seeds + eng_.size());
+  std::vector<Ctx> ctxs(eng_.size());
+  for (size_t i = 0; i < eng_.size(); ++i) {
+    Engine &e = *eng_[i];
+    int rc = e.PrepareSolve(x[i], opts_[i], seeds[i], s, &args_[i]);
+    if (rc != MPSFT_OK) return rc;
+    if (e.fft_big_) return MPSFT_ERR_UNSUPPORTED;
+    if ((e.precision_ == MPSFT_MIXED) != (precision_ == MPSFT_MIXED)) return MPSFT_ERR_INVALID_ARG;
+    ctxs[i] = e.MakeCtx();
+    ctxs[i].batch_done = d_counters_ + 1;
+    ctxs[i].batch_host = d_host_;
+  }
+  HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_args_, args_.data(), args_.size() * sizeof(InitArgs),
+                        hipMemcpyHostToDevice, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return Build(s);
+}
+
+// Block maps of the three batched launches and the graph of two batched
+// iterations; redone only when an engine's shapes changed.
+int Batch::Build(hipStream_t s) {
+  bool same = graph_ != nullptr;
+  for (size_t i = 0; i < eng_.size() && same; ++i) same = built_gen_[i] == eng_[i]->shape_gen_;
+  if (same) return MPSFT_OK;
+  std::vector<int2> map_time, map_fft, map_ident;
+  std::vector<int32_t> nblk_ident(eng_.size());
+  size_t lds_time = 0, lds_fft = 0;
+  for (size_t i = 0; i < eng_.size(); ++i) {
+    Engine &e = *eng_[i];
+    const int32_t si = int32_t(i);
+    const int32_t nt = kPrepBlocks + e.TilesCap();
+    for (int32_t b = 0; b < nt; ++b) map_time.push_back(make_int2(si, b));
+    const int32_t nf = e.trials_cap_ * e.rows_cap_;
+    for (int32_t b = 0; b < nf; ++b) map_fft.push_back(make_int2(si, b));
+    nblk_ident[i] = e.IdentifyBlocks();
+    for (int32_t b = 0; b < nblk_ident[i]; ++b) map_ident.push_back(make_int2(si, b));
+    lds_time = std::max(lds_time, e.LdsBinTime());
+    lds_fft = std::max(lds_fft, e.LdsFft());
+    built_gen_[i] = e.shape_gen_;
+  }
+  const size_t total = map_time.size() + map_fft.size() + map_ident.size();
+  if (AllocDev(&d_map_, total) != MPSFT_OK || AllocDev(&d_nblk_ident_, eng_.size()) != MPSFT_OK)
+    return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemcpy(d_map_, map_time.data(), map_time.size() * sizeof(int2),
+                   hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_map_ + map_time.size(), map_fft.data(), map_fft.size() * sizeof(int2),
+                   hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_map_ + map_time.size() + map_fft.size(), map_ident.data(),
+                   map_ident.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_nblk_ident_, nblk_ident.data(), nblk_ident.size() * sizeof(int32_t),
+                   hipMemcpyHostToDevice));
+  bl_.ctxs = d_ctxs_;
+  bl_.map_time = d_map_;
+  bl_.map_fft = d_map_ + map_time.size();
+  bl_.map_ident = d_map_ + map_time.size() + map_fft.size();
+  bl_.nblk_ident = d_nblk_ident_;
+  bl_.nb_time = int32_t(map_time.size());
+  bl_.nb_fft = int32_t(map_fft.size());
+  bl_.nb_ident = int32_t(map_ident.size());
+  bl_.lds_time = lds_time;
+  bl_.lds_fft = lds_fft;
+  bl_.counters = d_counters_;
+  bl_.host = d_host_;
+  bl_.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
+
+  if (graph_) {
+    (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+  }
+  hipStream_t cs;
+  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return MPSFT_ERR_HIP;
+  }
+  const hipError_t e0 = LaunchBatchIteration(bl_, 0, cs);
+  const hipError_t e1 = LaunchBatchIteration(bl_, 1, cs);
+  hipGraph_t g;
+  const hipError_t ec = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) return MPSFT_ERR_HIP;
+  const hipError_t ei = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    graph_ = nullptr;
+    return MPSFT_ERR_HIP;
+  }
+  (void)s;
+  return MPSFT_OK;
+}
+
+// Launches graphs of two batched iterations, at most `lookahead` ahead of
+// the device, until every signal has stopped (or max_iter).  Transforms are
+// drawn ahead for all signals; when an engine's arrays must grow, the stream
+// is drained first and the Ctx array re-uploaded (the graph reads it).
+int Batch::Run(hipStream_t s, int32_t max_iter) {
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(s);
+    return code;
+  };
+  const int32_t K = count();
+  const int32_t lookahead = 2;
+  int32_t launched = 0;
+  auto last_change = std::chrono::steady_clock::now();
+  int32_t seen = 0;
+  for (uint64_t spin = 0;; ++spin) {
+    const int32_t started = h_host_[0];
+    if (h_host_[1] >= K || launched > max_iter) break;
+    if (launched - started < lookahead) {
+      const int32_t need = std::min(max_iter, launched + 3);
+      int32_t have = max_iter;
+      for (auto &e : eng_) have = std::min(have, e->tf_count_);
+      if (need > have) {
+        bool grow = false;
+        const int32_t want = std::min(max_iter, std::max(need, 2 * have));
+        for (auto &e : eng_) grow = grow || want > e->tf_cap_;
+        if (grow && hipStreamSynchronize(s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+        std::vector<Ctx> ctxs(eng_.size());
+        for (size_t i = 0; i < eng_.size(); ++i) {
+          const int rc = eng_[i]->EnsureTransforms(seeds_[i], want, opts_[i].trials, s);
+          if (rc != MPSFT_OK) return fail(rc);
+          ctxs[i] = eng_[i]->MakeCtx();
+          ctxs[i].batch_done = d_counters_ + 1;
+          ctxs[i].batch_host = d_host_;
+        }
+        if (grow &&
+            hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
+                           hipMemcpyHostToDevice, s) != hipSuccess)
+          return fail(MPSFT_ERR_HIP);
+      }
+      if (hipGraphLaunch(graph_, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+      launched += 2;
+      continue;
+    }
+    if (started != seen) {
+      seen = started;
+      last_change = std::chrono::steady_clock::now();
+    } else if ((spin & 4095) == 0 &&
+               std::chrono::steady_clock::now() - last_change > std::chrono::milliseconds(200)) {
+      // No report for a while: wait for the queued work, then go on.
+      if (hipStreamSynchronize(s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+      last_change = std::chrono::steady_clock::now();
+    }
+  }
+  return MPSFT_OK;
+}
+
+int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, int32_t *status,
+                 hipStream_t s) {
+  int rc = Prepare(x, seeds, s);
+  if (rc != MPSFT_OK) return rc;
+  int32_t max_iter = 0;
+  for (auto &a : args_) max_iter = std::max(max_iter, a.max_iter);
+  h_host_[0] = 0;
+  h_host_[1] = 0;
+  HIP_OK(LaunchBatchReset(d_ctxs_, d_args_, count(), d_counters_, s));
+  rc = Run(s, max_iter);
+  if (rc != MPSFT_OK) return rc;
+  HIP_OK(LaunchBatchResults(d_ctxs_, count(), d_results_, s));
+  HIP_OK(hipMemcpyAsync(h_results_, d_results_, count() * sizeof(BatchResultDev),
+                        hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  int worst = MPSFT_OK;
+  for (int32_t i = 0; i < count(); ++i) {
+    const BatchResultDev &r = h_results_[i];
+    int st = MPSFT_OK;
+    int32_t it = r.iters;
+    if (!r.stopped) {
+      st = MPSFT_ERR_NOT_CONVERGED;
+      it = r.last_it + 1;
+    } else if (r.flags & kFlagOverflow) {
+      st = MPSFT_ERR_CAPACITY;
+    } else if (r.stop_stale < opts_[i].max_stale_iter) {
+      st = MPSFT_ERR_NOT_CONVERGED;
+    }
+    if (iters) iters[i] = it;
+    if (status) status[i] = st;
+    if (st != MPSFT_OK && worst == MPSFT_OK) worst = st;
+  }
+  return worst;
+}
+
+int Batch::Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen,
+                    hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  // A short solve (3 iterations per signal) per candidate width, timed with
+  // events on the launch stream; the first run of each also builds tables.
+  std::vector<mpsft_solve_options> saved = opts_;
+  for (auto &o : opts_) o.max_iter = 3;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  int best_R = 0, rc = MPSFT_OK;
+  float best_ms = 0;
+  for (int R = 1; R <= 8 && rc == MPSFT_OK; R *= 2) {
+    for (auto &e : eng_)
+      if ((rc = e->SetTileResidues(R)) != MPSFT_OK) break;
+    if (rc != MPSFT_OK) break;
+    float ms = 0;
+    for (int rep = 0; rep < 2 && rc == MPSFT_OK; ++rep) {  // first run warms up
+      if (hipEventRecord(e0, s) != hipSuccess) rc = MPSFT_ERR_HIP;
+      const int r = Solve(x, seeds, nullptr, nullptr, s);
+      if (r != MPSFT_OK && r != MPSFT_ERR_NOT_CONVERGED) rc = r;
+      if (rc == MPSFT_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                             hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = MPSFT_ERR_HIP;
+    }
+    if (rc == MPSFT_OK && (best_R == 0 || ms < best_ms)) {
+      best_R = R;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  opts_ = saved;
+  if (rc != MPSFT_OK) return rc;
+  for (auto &e : eng_)
+    if ((rc = e->SetTileResidues(best_R)) != MPSFT_OK) return rc;
+  if (chosen) *chosen = best_R;
+  return MPSFT_OK;
+}
+
+}  // namespace mpsft

@@ -1,0 +1,60 @@
+// batch.h -- the batched form of the MPSFT solve: many independent signals,
+// each with its own n, options and seed, recovered together.  Every stage
+// of an iteration (gather-fold, bin FFT, identify) is one launch over all
+// signals; each signal keeps its own loop state (Demo1::Run,
+// src/demo1.cc:36-59) and stops on its own, and a graph of two batched
+// iterations is replayed until every signal has stopped.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace mpsft {
+
+class Batch {
+ public:
+  Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, int precision);
+  ~Batch();
+  int status() const { return status_; }
+  int32_t count() const { return int32_t(eng_.size()); }
+  Engine &engine(int32_t i) { return *eng_[i]; }
+
+  // Demo1::Run for every signal: x[i] is signal i (device, n[i] samples),
+  // seeds[i] its transform seed.  iters[i] / status[i] (host, may be null)
+  // get each signal's iteration count and mpsft_status.
+  int Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, int32_t *status,
+            hipStream_t s);
+  // Tries k_bin_time tile widths 1, 2, 4, 8 on a short batched solve of
+  // these signals and keeps the fastest; *chosen (may be null) gets it.
+  int Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen, hipStream_t s);
+
+ private:
+  int Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s);
+  int Build(hipStream_t s);
+  int Run(hipStream_t s, int32_t max_iter);
+
+  int status_ = MPSFT_OK;
+  int precision_;
+  std::vector<std::unique_ptr<Engine>> eng_;
+  std::vector<mpsft_solve_options> opts_;
+  std::vector<InitArgs> args_;
+  std::vector<uint32_t> seeds_;
+  std::vector<int64_t> built_gen_;  // engines' shape generations the maps were built for
+
+  Ctx *d_ctxs_ = nullptr;
+  InitArgs *d_args_ = nullptr;
+  int2 *d_map_ = nullptr;           // map_time | map_fft | map_ident
+  int32_t *d_nblk_ident_ = nullptr;
+  int32_t *d_counters_ = nullptr;   // {iterations started, signals stopped}
+  BatchResultDev *d_results_ = nullptr;
+  BatchResultDev *h_results_ = nullptr;  // pinned
+  volatile int32_t *h_host_ = nullptr;  // pinned, mapped: device copy of the counters
+  int32_t *d_host_ = nullptr;
+  BatchLaunch bl_{};
+  hipGraphExec_t graph_ = nullptr;
+};
+
+}  // namespace mpsft

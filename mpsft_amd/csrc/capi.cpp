@@ -5,6 +5,7 @@
 #include <new>
 
 #include "../../include/mpsft.h"
+#include "batch.h"
 #include "engine.h"
 #include "kernels.h"
 
@@ -17,6 +18,10 @@ struct mpsft_plan {
   int32_t bins = 0, bits = 0;
   double delta = 0;
   mpsft::BParams bp{};
+};
+
+struct mpsft_batch {
+  mpsft::Batch *b;
 };
 
 struct mpsft_solver {
@@ -252,6 +257,69 @@ int mpsft_solver_transforms_get(mpsft_solver *s, mpsft_transform *tfs, int64_t *
                                 int32_t *count) {
   if (!s || !count || (cap > 0 && (!tfs || !qs))) return MPSFT_ERR_INVALID_ARG;
   return s->eng->TransformsGet(reinterpret_cast<DevTf *>(tfs), qs, cap, s->opt.trials, count);
+}
+
+int mpsft_solver_set_tile_residues(mpsft_solver *s, int32_t residues) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->SetTileResidues(residues);
+}
+
+int mpsft_batch_create(int32_t count, const int64_t *n, const mpsft_solve_options *opts,
+                       int precision, mpsft_batch **batch) {
+  if (!batch) return MPSFT_ERR_INVALID_ARG;
+  *batch = nullptr;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  if (count <= 0 || !n || !opts) return MPSFT_ERR_INVALID_ARG;
+  for (int32_t i = 0; i < count; ++i)
+    if (2 * int64_t(opts[i].k) + 1 > mpsft::kMaxLdsBins) return MPSFT_ERR_UNSUPPORTED;
+  mpsft_batch *b = new (std::nothrow) mpsft_batch;
+  if (!b) return MPSFT_ERR_NOMEM;
+  b->b = new (std::nothrow) mpsft::Batch(count, n, opts, precision);
+  if (!b->b || b->b->status() != MPSFT_OK) {
+    const int rc = b->b ? b->b->status() : MPSFT_ERR_NOMEM;
+    delete b->b;
+    delete b;
+    return rc;
+  }
+  *batch = b;
+  return MPSFT_OK;
+}
+
+int mpsft_batch_destroy(mpsft_batch *batch) {
+  if (!batch) return MPSFT_ERR_INVALID_ARG;
+  delete batch->b;
+  delete batch;
+  return MPSFT_OK;
+}
+
+int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
+                      int32_t *iters, int32_t *status, void *stream) {
+  if (!batch || !x_dev || !seeds) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->Solve(x_dev, seeds, iters, status, S(stream));
+}
+
+int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
+                         int32_t *residues, void *stream) {
+  if (!batch || !x_dev || !seeds) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->Autotune(x_dev, seeds, residues, S(stream));
+}
+
+int mpsft_batch_modes_count(mpsft_batch *batch, int32_t i, int64_t *count) {
+  if (!batch || !count || i < 0 || i >= batch->b->count()) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->engine(i).ModesCount(count);
+}
+
+int mpsft_batch_modes_get(mpsft_batch *batch, int32_t i, int64_t *keys, double *coefs,
+                          int64_t cap, int64_t *count) {
+  if (!batch || !count || i < 0 || i >= batch->b->count() || (cap > 0 && (!keys || !coefs)))
+    return MPSFT_ERR_INVALID_ARG;
+  return batch->b->engine(i).ModesGet(keys, coefs, cap, count);
+}
+
+int mpsft_batch_log_get(mpsft_batch *batch, int32_t i, int32_t *bins, int32_t *found,
+                        int32_t cap, int32_t *n_iter) {
+  if (!batch || !n_iter || i < 0 || i >= batch->b->count()) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->engine(i).LogGet(bins, found, cap, n_iter);
 }
 
 }  // extern "C"

@@ -62,8 +62,15 @@ int32_t NumLags(int32_t bits, int32_t f) {
 }
 
 // Residues per k_bin_time tile: the largest power of two <= 8 that still
-// gives kTargetTiles tiles, and keeps the tile's LDS under 64 KiB.
-int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f) {
+// gives kTargetTiles tiles, and keeps the tile's LDS under 64 KiB.  A
+// forced width (forced > 0, from the autotuner) is kept within the LDS of
+// one workgroup.
+int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f, int32_t forced) {
+  if (forced > 0) {
+    int32_t R = forced;
+    while (R > 1 && (size_t(nlam + f) * R + kThreads) * sizeof(double2) > 160 * 1024) R /= 2;
+    return R;
+  }
   int32_t R = 8;
   while (R > 1 && ((B + R - 1) / R < kTargetTiles ||
                    size_t(nlam + f) * R * sizeof(double2) > 65536))
@@ -246,7 +253,8 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     BParams bp;
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
-    bp.R = ResiduesPerTile(B, NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f), bp.f);
+    bp.R = ResiduesPerTile(B, NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f), bp.f,
+                           forced_R_);
     bp.off = 0;
     if (bp.logM <= kMaxLogM) {
       bp.off = table_used_ + need;
@@ -305,6 +313,26 @@ int Engine::SetThresholds(double sigma) {
   HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
                    hipMemcpyHostToDevice));
   tab_sigma_ = sigma;
+  return MPSFT_OK;
+}
+
+// Fixes the k_bin_time tile width (residues per tile) of every B, or back
+// to the default heuristic for R <= 0.  Tiling changes only how the work
+// is split, not what is summed; the buffers are re-sized by Reserve.
+int Engine::SetTileResidues(int32_t R) {
+  if (status_ != MPSFT_OK) return status_;
+  if (R > 8 || (R > 0 && (R & (R - 1)))) return MPSFT_ERR_INVALID_ARG;
+  forced_R_ = R > 0 ? R : 0;
+  for (size_t B = 0; B < hbtab_.size(); ++B) {
+    if (!built_[B]) continue;
+    BParams &bp = hbtab_[B];
+    const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
+    bp.R = ResiduesPerTile(int32_t(B), NumLags(bits, bp.f), bp.f, forced_R_);
+  }
+  if (d_btab_)
+    HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
+                     hipMemcpyHostToDevice));
+  ++shape_gen_;
   return MPSFT_OK;
 }
 
@@ -471,6 +499,8 @@ Ctx Engine::MakeCtx() const {
   c.big = d_big_;
   c.big_gpr = fft_big_ ? big_M_cap_ / kBigTile : 0;
   c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
+  c.batch_done = nullptr;
+  c.batch_host = nullptr;
   return c;
 }
 
@@ -547,12 +577,16 @@ int Engine::ReadState(hipStream_t s) {
   return MPSFT_OK;
 }
 
-int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
-                  int32_t *iters, hipStream_t s, double *kernel_ms) {
+// Everything a solve needs before its first launch: argument checks, the
+// window tables of every reachable B, thresholds, buffers and the first
+// transform draws; *a gets the device-side settings for k_init.
+int Engine::PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
+                         hipStream_t s, InitArgs *a) {
   if (status_ != MPSFT_OK) return status_;
   if (!x || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
-      !(opt.window_threshold > 0) || opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 || opt.max_stale_iter <= 0 ||
-      !(opt.window_delta > 0) || !(opt.window_delta < 1) || opt.sigma < 0)
+      !(opt.window_threshold > 0) || opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 ||
+      opt.max_stale_iter <= 0 || !(opt.window_delta > 0) || !(opt.window_delta < 1) ||
+      opt.sigma < 0)
     return MPSFT_ERR_INVALID_ARG;
   const int64_t want0 = 2 * int64_t(opt.k) + 1;
   const int32_t B0 = int32_t(std::max<int64_t>(want0, opt.min_bins));
@@ -567,26 +601,48 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   rc = Reserve(opt.trials, bs);
   if (rc != MPSFT_OK) return rc;
   const int32_t max_iter = opt.max_iter > 0 ? opt.max_iter : 100000;
-  const int32_t chunk = ChunkIters();
-  rc = EnsureTransforms(seed, std::min(max_iter, std::max(256, chunk)), opt.trials, s);
+  rc = EnsureTransforms(seed, std::min(max_iter, std::max(256, ChunkIters())), opt.trials, s);
   if (rc != MPSFT_OK) return rc;
+  *a = InitArgs{};
+  a->x = x;
+  a->mode = kModeSolve;
+  a->B = B0;
+  a->trials = opt.trials;
+  a->k = opt.k;
+  a->min_bins = opt.min_bins;
+  a->max_stale = opt.max_stale_iter;
+  a->max_iter = max_iter;
+  a->bits_override = 0;
+  a->win_thr = opt.window_threshold;
+  return MPSFT_OK;
+}
 
+// The result of a finished solve from the device state.
+int Engine::SolveResult(const mpsft_solve_options &opt, int32_t *iters) {
+  const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
+  if (!stopped) {
+    // Only when max_iter stops the launches before the device saw it.
+    *iters = std::max(h_state_->L[0].it, h_state_->L[1].it) + 1;
+    return MPSFT_ERR_NOT_CONVERGED;
+  }
+  *iters = h_state_->iters;
+  if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
+  if (h_state_->stop_stale < opt.max_stale_iter) return MPSFT_ERR_NOT_CONVERGED;
+  return MPSFT_OK;
+}
+
+int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
+                  int32_t *iters, hipStream_t s, double *kernel_ms) {
+  InitArgs a;
+  int rc = PrepareSolve(x, opt, seed, s, &a);
+  if (rc != MPSFT_OK) return rc;
+  const int32_t max_iter = a.max_iter;
+  const int32_t chunk = ChunkIters();
   Ctx c = MakeCtx();
   h_progress_[0] = 0;  // the previous solve has finished; the device writes
   h_progress_[1] = 0;  // these again only after k_init below
   h_progress_[2] = 0;
   HIP_OK(LaunchClearModes(c, 0, s));
-  InitArgs a{};
-  a.x = x;
-  a.mode = kModeSolve;
-  a.B = B0;
-  a.trials = opt.trials;
-  a.k = opt.k;
-  a.min_bins = opt.min_bins;
-  a.max_stale = opt.max_stale_iter;
-  a.max_iter = max_iter;
-  a.bits_override = 0;
-  a.win_thr = opt.window_threshold;
   HIP_OK(LaunchInit(c, a, s));
 
   int32_t launched = 0;
@@ -694,16 +750,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
     rc = ReadState(s);
     if (rc != MPSFT_OK) return fail(rc);
   }
-  const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
-  if (!stopped) {
-    // Only when max_iter stops the launches before the device saw it.
-    *iters = std::max(h_state_->L[0].it, h_state_->L[1].it) + 1;
-    return MPSFT_ERR_NOT_CONVERGED;
-  }
-  *iters = h_state_->iters;
-  if (h_state_->flags & kFlagOverflow) return MPSFT_ERR_CAPACITY;
-  if (h_state_->stop_stale < opt.max_stale_iter) return MPSFT_ERR_NOT_CONVERGED;
-  return MPSFT_OK;
+  return SolveResult(opt, iters);
 }
 
 int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const DevTf &tf,

@@ -31,6 +31,8 @@
 
 namespace mpsft {
 
+struct InitArgs;
+
 constexpr int kThreads = 256;      // workgroup size of most kernels
 constexpr int kFftThreads = 256;   // workgroup size of k_bin_fft
 constexpr int kMaxLogM = 12;       // k_bin_fft (one LDS FFT) up to M = 4096, B <= 2048
@@ -130,6 +132,8 @@ struct Ctx {
   int32_t trials_cap, rows_cap, B_cap;
   int32_t big_gpr;       // four-step workgroups per row (M_cap / kBigTile), 0 if unused
   int32_t mixed;
+  int32_t *batch_done;   // batched form: device count of stopped signals, or null
+  int32_t *batch_host;   // batched form: host-mapped {iterations, stopped}, or null
 };
 
 class Engine {
@@ -160,8 +164,15 @@ class Engine {
 
   int64_t n() const { return n_; }
   int precision() const { return precision_; }
+  // k_bin_time tile width: residues per tile (1, 2, 4, 8), or 0 for the
+  // default choice.
+  int SetTileResidues(int32_t R);
 
  private:
+  friend class Batch;
+  int PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t seed, hipStream_t s,
+                   InitArgs *a);
+  int SolveResult(const mpsft_solve_options &opt, int32_t *iters);
   int EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t bits_override);
   int SetThresholds(double sigma);
   int Reserve(int32_t trials, const std::vector<int32_t> &bs);
@@ -252,6 +263,7 @@ class Engine {
   int32_t graph_iters_ = 0;
   int64_t graph_key_ = -1;
   int64_t shape_gen_ = 0;
+  int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
 };
 
 // Host-side helpers shared with the C-ABI.

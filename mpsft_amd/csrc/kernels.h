@@ -29,6 +29,25 @@ hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);
 // The four-step bin FFT (B > kMaxLdsBins): three kernels.
 hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s);
 hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s);
+
+// The batched form: per-stage launches over every signal of a batch.
+struct BatchLaunch {
+  const Ctx *ctxs;                      // device, one per signal
+  const int2 *map_time, *map_fft, *map_ident;  // device, {signal, local block} per block
+  const int32_t *nblk_ident;            // device, k_identify blocks per signal
+  int32_t nb_time, nb_fft, nb_ident;    // blocks per launch
+  size_t lds_time, lds_fft;             // dynamic LDS (max over signals)
+  int32_t *counters;                    // device {iterations started, signals stopped}
+  int32_t *host;                        // host-mapped copy of counters
+  int mixed;
+};
+hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count,
+                            int32_t *counters, hipStream_t s);
+hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s);
+struct BatchResultDev {
+  int32_t iters, stop_stale, flags, stopped, last_it, pad;
+};
+hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

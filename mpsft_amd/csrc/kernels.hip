@@ -250,10 +250,10 @@ __global__ void __launch_bounds__(kThreads)
 
 // Clears the found-mode map: only the listed keys (their coefficient and
 // presence word), or everything after an overflow.
-__global__ void __launch_bounds__(kThreads) k_clear_modes(Ctx c, int32_t full) {
+// tid / gsize: this thread's index and the thread count over the map.
+__device__ __forceinline__ void ClearModesBody(const Ctx &c, int32_t full, int64_t tid,
+                                               int64_t gsize) {
   const DevState *st = c.st;
-  const int64_t gsize = int64_t(gridDim.x) * blockDim.x;
-  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (full || (st->flags & kFlagOverflow)) {
     for (int64_t i = tid; i < c.n; i += gsize) c.coef[i] = make_double2(0.0, 0.0);
     for (int64_t i = tid; i < (c.n + 31) / 32; i += gsize) c.present[i] = 0u;
@@ -265,6 +265,11 @@ __global__ void __launch_bounds__(kThreads) k_clear_modes(Ctx c, int32_t full) {
     c.coef[k] = make_double2(0.0, 0.0);
     c.present[k >> 5] = 0u;
   }
+}
+
+__global__ void __launch_bounds__(kThreads) k_clear_modes(Ctx c, int32_t full) {
+  ClearModesBody(c, full, int64_t(blockIdx.x) * blockDim.x + threadIdx.x,
+                 int64_t(gridDim.x) * blockDim.x);
 }
 
 __device__ __forceinline__ void FillParams(IterParams *ip, const BParams &bp, int32_t bits) {
@@ -285,8 +290,7 @@ __device__ __forceinline__ void FillParams(IterParams *ip, const BParams &bp, in
 
 // Fixed per-call settings; iteration -1 sits in slot 1 as "found", so the
 // first iteration (slot 0) starts with stale = 0.
-__global__ void k_init(Ctx c, InitArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void InitBody(const Ctx &c, const InitArgs &a) {
   DevState *st = c.st;
   st->x = a.x;
   st->mode = a.mode;
@@ -319,6 +323,11 @@ __global__ void k_init(Ctx c, InitArgs a) {
     __hip_atomic_store(&c.progress[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&c.progress[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+__global__ void k_init(Ctx c, InitArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  InitBody(c, a);
 }
 
 // The next iteration's parameters, as Demo1::Run derives them
@@ -384,6 +393,10 @@ __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
   if (lc.done) {
     st->iters = lc.it;
     st->stop_stale = lc.stale;
+    if (c.batch_done) {  // a batch counts its stopped signals for the host
+      const int32_t d = atomicAdd(c.batch_done, 1) + 1;
+      __hip_atomic_store(&c.batch_host[1], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
   if (c.progress) {
     if (!lc.done)
@@ -460,28 +473,28 @@ struct Lags {
 //   phase B forms every (row, rho) sum, split over `parts` threads.
 // Workgroups below kPrepBlocks instead compute the BinInFreq terms of every
 // found mode for this iteration.
+// blk: this workgroup's block index within the signal's launch.
 template <typename Tx>
-__global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
-  HIP_DYNAMIC_SHARED(double2, lds)
+__device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double2 *lds) {
   const DevState *st = c.st;
   IterParams ip;
   LoopCounters lc;
   NextIteration(c, p, &ip, &lc);
-  if (blockIdx.x == 0 && threadIdx.x == 0) PublishIteration(c, p, ip, lc);
+  if (blk == 0 && threadIdx.x == 0) PublishIteration(c, p, ip, lc);
   if (ip.done) return;
   const int32_t trials = st->trials;
   const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
   const DevTf tf = ip.tf;
   const int64_t n = c.n, mask = c.mask;
 
-  if (blockIdx.x < kPrepBlocks) {
+  if (blk < kPrepBlocks) {
     // BinInFreqV1 (src/binner.cc:565-591) per found mode and trial: bin,
     // base = wf * coef * e^{2 pi i (q k1 + c k0)/n}, and per bit
     // g = e^{2 pi i (B 2^bit k1)/n} for rows 2bit+1 (base g) and 2bit+2
     // (base conj(g)).
     const int32_t cnt = min(st->count, c.cap);
     const double width = ip.width, dfreq = ip.dfreq;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
+    for (int64_t i = int64_t(blk) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
          i += int64_t(kPrepBlocks) * blockDim.x) {
       const int32_t trial = int32_t(i / cnt);
       const int32_t mi = int32_t(i - int64_t(trial) * cnt);
@@ -507,7 +520,7 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
 
   const int32_t R = ip.R;
   const int32_t nblk = (B + R - 1) / R;
-  int32_t tile = blockIdx.x - kPrepBlocks;
+  int32_t tile = blk - kPrepBlocks;
   if (tile >= trials * nblk) return;
   const int32_t trial = tile / nblk;
   const int32_t rho0 = (tile - trial * nblk) * R;
@@ -598,6 +611,12 @@ __global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
       fold_t[int64_t(row) * c.B_cap + m] = acc;
     }
   }
+}
+
+template <typename Tx>
+__global__ void __launch_bounds__(kThreads) k_bin_time(Ctx c, int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  BinTimeBody<Tx>(c, p, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -934,16 +953,15 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
   }
 }
 
-__global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
-  HIP_DYNAMIC_SHARED(double2, lds)
+__device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2 *lds) {
   constexpr int T = kFftThreads;
   constexpr int kPerTerm = 2;  // BinInFreq terms prefetched per thread
   const DevState *st = c.st;
   const IterParams &ip = st->P[p];
   if (ip.done || ip.logM > kMaxLogM) return;
   const int32_t trials = st->trials, rows = ip.rows;
-  const int32_t trial = blockIdx.x / c.rows_cap;
-  const int32_t row = blockIdx.x - trial * c.rows_cap;
+  const int32_t trial = blk / c.rows_cap;
+  const int32_t row = blk - trial * c.rows_cap;
   if (trial >= trials || row >= rows) return;
   const int32_t B = ip.B;
   const int tid = threadIdx.x;
@@ -990,6 +1008,11 @@ __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   __syncthreads();
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
   for (int k = tid; k < B; k += T) A[k] = a[small ? k : Pad(k)];
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  BinFftBody(c, p, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1240,7 +1263,9 @@ __device__ __forceinline__ double2 ColFetch(const double2 (&col)[kColSlices], in
   return r;
 }
 
-__global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
+// blk, nblk: this workgroup's block index and the block count of the
+// signal's launch (bins are distributed over nblk x waves wavefronts).
+__device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int nblk) {
 #pragma clang fp contract(off)
   DevState *st = c.st;
   const IterParams &ip = st->P[p];
@@ -1263,7 +1288,7 @@ __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
   }
   __syncthreads();
 
-  for (int32_t b = blockIdx.x * waves + (threadIdx.x >> 6); b < B; b += gridDim.x * waves) {
+  for (int32_t b = blk * waves + (threadIdx.x >> 6); b < B; b += nblk * waves) {
     const double2 *Ab = c.A + b;
     double2 col[kColSlices];
 #pragma unroll
@@ -1356,6 +1381,10 @@ __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
   if (threadIdx.x == 0 && nany) atomicOr(&st->found_iter[p], 1);
 }
 
+__global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
+  IdentifyBody(c, p, blockIdx.x, gridDim.x);
+}
+
 // BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix: one
 // thread per mode, subtracting from every row with device atomics.
 __global__ void __launch_bounds__(kThreads)
@@ -1402,6 +1431,65 @@ __global__ void __launch_bounds__(kThreads)
   sigma[2 * i] = s1;
   sigma[2 * i + 1] = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
   vote[i] = beta.y > 0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// The batched form: one launch per stage covers every signal of a batch.
+// map[blockIdx.x] = {signal, block within that signal's share}; each
+// workgroup runs the single-signal body on its signal's Ctx.  Signals that
+// have stopped exit at once.
+__global__ void __launch_bounds__(kThreads)
+    k_batch_reset(const Ctx *ctxs, const InitArgs *args, int32_t *counters) {
+  const Ctx &c = ctxs[blockIdx.x];
+  ClearModesBody(c, 0, threadIdx.x, blockDim.x);
+  __syncthreads();
+  if (threadIdx.x == 0) InitBody(c, args[blockIdx.x]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counters[0] = 0;  // batched iterations started
+    counters[1] = 0;  // signals stopped
+  }
+}
+
+template <typename Tx>
+__global__ void __launch_bounds__(kThreads)
+    k_bin_time_batch(const Ctx *ctxs, const int2 *map, int p, int32_t *counters,
+                     int32_t *host) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t v = atomicAdd(&counters[0], 1) + 1;
+    __hip_atomic_store(&host[0], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  const int2 m = map[blockIdx.x];
+  BinTimeBody<Tx>(ctxs[m.x], p, m.y, lds);
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_bin_fft_batch(const Ctx *ctxs, const int2 *map,
+                                                               int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int2 m = map[blockIdx.x];
+  BinFftBody(ctxs[m.x], p, m.y, lds);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_identify_batch(const Ctx *ctxs, const int2 *map, const int32_t *nblk, int p) {
+  const int2 m = map[blockIdx.x];
+  IdentifyBody(ctxs[m.x], p, m.y, nblk[m.x]);
+}
+
+// Per signal: what the host needs to report the batched solve.
+__global__ void __launch_bounds__(kThreads)
+    k_batch_results(const Ctx *ctxs, int32_t count, BatchResultDev *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const DevState *st = ctxs[i].st;
+  BatchResultDev r;
+  r.iters = st->iters;
+  r.stop_stale = st->stop_stale;
+  r.flags = st->flags;
+  r.stopped = (st->L[0].done | st->L[1].done) ? 1 : 0;
+  r.last_it = max(st->L[0].it, st->L[1].it);
+  r.pad = 0;
+  out[i] = r;
 }
 
 int32_t Blocks(int64_t work) {
@@ -1486,6 +1574,32 @@ hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma, uint8_t *
                            hipStream_t s) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_mat_pencil, dim3(Blocks(m)), dim3(kThreads), 0, s, u, m, sigma, vote);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count,
+                            int32_t *counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(count), dim3(kThreads), 0, s, ctxs, args, counters);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s) {
+  if (b.mixed)
+    hipLaunchKernelGGL(k_bin_time_batch<float2>, dim3(b.nb_time), dim3(kThreads), b.lds_time, s,
+                       b.ctxs, b.map_time, parity, b.counters, b.host);
+  else
+    hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.nb_time), dim3(kThreads), b.lds_time, s,
+                       b.ctxs, b.map_time, parity, b.counters, b.host);
+  hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
+                     b.map_fft, parity);
+  hipLaunchKernelGGL(k_identify_batch, dim3(b.nb_ident), dim3(kThreads), 0, s, b.ctxs,
+                     b.map_ident, b.nblk_ident, parity);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_results, dim3(Blocks(count)), dim3(kThreads), 0, s, ctxs, count, out);
   return hipGetLastError();
 }
 

@@ -52,6 +52,28 @@ def make_signal(n, k, sigma, seed, magnitude="unit", clusters=None, dtype=np.com
     return x.astype(dtype), dict(zip(locs.tolist(), coef.tolist()))
 
 
+def make_signal_device(n, k, sigma, seed):
+    """make_signal's construction (unit magnitudes, uniform phases, distinct
+    uniform locations) computed on the GPU with torch: returns the signal as
+    a complex64 torch tensor on cuda, the rms rounding error of storing it
+    in complex64 (sigma_q), and the modes {location: coefficient}."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    locs = torch.randperm(n, generator=g, device="cuda")[:k]
+    phase = torch.rand(k, generator=g, device="cuda", dtype=torch.float64)
+    coef = torch.polar(torch.ones(k, device="cuda", dtype=torch.float64), 2 * np.pi * phase)
+    scale = sigma / np.sqrt(2.0 * n)
+    xh = torch.complex(torch.randn(n, generator=g, device="cuda", dtype=torch.float64),
+                       torch.randn(n, generator=g, device="cuda", dtype=torch.float64)) * scale
+    xh[locs] += coef
+    x = torch.fft.ifft(xh) * n
+    x64 = x.to(torch.complex64)
+    sigma_q = float((x - x64.to(torch.complex128)).abs().pow(2).mean().sqrt())
+    truth = dict(zip(locs.cpu().tolist(), coef.cpu().numpy().tolist()))
+    return x64, sigma_q, truth
+
+
 def quantization_sigma(x128, x64):
     """RMS per-sample error of storing x in complex64: the noise the mixed
     form sees on top of sigma."""

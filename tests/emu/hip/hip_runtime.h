@@ -38,6 +38,10 @@ struct double2 {
 struct float2 {
   float x, y;
 };
+struct int2 {
+  int x, y;
+};
+inline int2 make_int2(int x, int y) { return int2{x, y}; }
 inline double2 make_double2(double x, double y) { return double2{x, y}; }
 inline float2 make_float2(float x, float y) { return float2{x, y}; }
 

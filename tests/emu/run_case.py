@@ -82,7 +82,31 @@ def plan(n, bins, bits, seed):
     return dict(rel_err=float(np.abs(got - ref).max() / np.abs(ref).max()))
 
 
+def batch(ns, ks, sigma, seeds, precision=0, autotune=False):
+    xs = [signal(n, k, sigma, sd) for n, k, sd in zip(ns, ks, seeds)]
+    xg = [x.astype(np.complex64) if precision else x for x in xs]
+    opts = [gpu.solve_options(k, sigma) for k in ks]
+    b = gpu.Batch(ns, opts, precision)
+    xd = b.upload(xg)
+    tuned = b.autotune(xd, seeds) if autotune else None
+    iters, stats = b.solve(xd, seeds)
+    out = {"iters": iters, "status": stats, "tuned": tuned, "same_as_single": [], "same_as_oracle": []}
+    for i, (n, k, sd) in enumerate(zip(ns, ks, seeds)):
+        got = b.modes_dict(i)
+        s = gpu.Solver(n, opts[i], precision)
+        xs1 = s.upload(xg[i])
+        it1 = s.solve(xs1, sd)
+        one = s.modes_dict()
+        orc.seed(sd)
+        ref, rb, _ = orc.demo1_run(xg[i].astype(np.complex128), k, sigma, orc.Demo1Options())
+        ref = ref.to_dict()
+        e1 = max((abs(got[key] - one[key]) for key in one if key in got), default=0.0)
+        out["same_as_single"].append(bool(set(got) == set(one) and it1 == iters[i] and e1 < 1e-12))
+        out["same_as_oracle"].append(bool(set(got) == set(ref) and len(rb) == iters[i]))
+    return out
+
+
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
-    fn = {"solve": solve, "iterate": iterate, "plan": plan}[case.pop("kind")]
+    fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch}[case.pop("kind")]
     print("RESULT " + json.dumps(fn(**case)), flush=True)

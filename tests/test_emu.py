@@ -71,6 +71,15 @@ def test_emu_solve_graph(emu_env, precision):
     assert r["max_rel_err"] < 1e-10
 
 
+def test_emu_batch_matches_single_and_oracle(emu_env):
+    """The batched form on 4 signals of different n and S: each signal's
+    result equals its own single-signal solve and the oracle."""
+    r = run_case(emu_env, kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8],
+                 sigma=1e-6, seeds=[3, 4, 5, 6])
+    assert r["status"] == [0, 0, 0, 0]
+    assert all(r["same_as_single"]) and all(r["same_as_oracle"])
+
+
 @pytest.mark.skipif(os.environ.get("MPSFT_SLOW_TESTS") != "1",
                     reason="about 15 minutes; set MPSFT_SLOW_TESTS=1")
 def test_emu_solve_past_initial_transform_draw(emu_env, tmp_path):

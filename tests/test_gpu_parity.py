@@ -326,6 +326,53 @@ def test_solve_parity_config3_mixed(gpu, orc):
     assert len([key for key in truth if key in got]) >= 0.95 * k
 
 
+# ---------------------------------------------------------------- batched form
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_single_and_oracle(gpu, orc, precision):
+    """12 signals, n = 2^14..2^18, S = 10..300: the batched solve gives each
+    signal the result of its own single-signal solve, and the oracle's
+    support and iteration log."""
+    rng = np.random.default_rng(40 + precision)
+    ns = [1 << int(v) for v in rng.integers(14, 19, 12)]
+    ks = [int(v) for v in rng.integers(10, 301, 12)]
+    seeds = [int(v) for v in rng.integers(1, 1000, 12)]
+    sigma = 1e-6
+    xs, sig = [], []
+    for i, (n, k) in enumerate(zip(ns, ks)):
+        x, _ = make_signal(n, k, sigma, seed=100 + i)
+        if precision:
+            x64 = x.astype(np.complex64)
+            sig.append(float(np.hypot(sigma, quantization_sigma(x, x64))))
+            x = x64
+        else:
+            sig.append(sigma)
+        xs.append(x)
+    opts = [gpu.solve_options(k, s_) for k, s_ in zip(ks, sig)]
+    b = gpu.Batch(ns, opts, precision)
+    xd = b.upload(xs)
+    iters, stats = b.solve(xd, seeds)
+    assert stats == [0] * 12
+    before = [b.modes_dict(i) for i in range(12)]
+    tol = FP64_TOL if precision == 0 else MIXED_TOL
+    for i in range(12):
+        got = b.modes_dict(i)
+        s = gpu.Solver(ns[i], opts[i], precision)
+        it1 = s.solve(s.upload(xs[i]), seeds[i])
+        assert it1 == iters[i]
+        assert_coefs_close(got, s.modes_dict(), 1e-12, f"single {i}")
+        orc.seed(seeds[i])
+        ref, rb, _ = orc.demo1_run(xs[i].astype(np.complex128), ks[i], sig[i], orc.Demo1Options())
+        assert np.array_equal(b.log(i)[0], rb)
+        assert_coefs_close(got, ref.to_dict(), tol, f"oracle {i}")
+    # the tile width chosen by the autotuner changes nothing but speed
+    r = b.autotune(xd, seeds)
+    assert r in (1, 2, 4, 8)
+    iters2, _ = b.solve(xd, seeds)
+    assert iters2 == iters
+    for i in range(12):  # a different tile width only re-groups the fold's sums
+        assert_coefs_close(b.modes_dict(i), before[i], 1e-12, f"retuned {i}")
+
+
 # ---------------------------------------------------------------- edge cases
 def test_solve_zero_modes_and_errors(gpu):
     n = 1 << 12
