@@ -173,6 +173,69 @@ def batch_workload(M, steps, warmup, count=256):
     }
 
 
+def adjoint_workload(M, steps, warmup, count=32):
+    """Differentiable recovery: `count` complex64 signals, n = 2^20, S = 100
+    unit modes, sigma = 1e-6; loss = sum |c_hat - c_target|^2 over the
+    recovered coefficients with c_target = truth (1 + 0.01 CN(0,1)).  A step
+    is the forward (recorded solve of every signal, torch.ops.mpsft.recover),
+    the loss, and the backward (the adjoint of every signal's recovery)."""
+    import torch
+    from mpsft_amd import autograd as ag
+    from mpsft_amd.signals import loss_targets, make_signal
+    n, k, sigma = 1 << 20, 100, 1e-6
+    dev = torch.device("cuda:0")
+    log(f"synthesising {count} complex64 signals n=2^20 S={k}")
+    xs, dense_t, solvers, seeds = [], [], [], []
+    for i in range(count):
+        x, truth = make_signal(n, k, sigma, seed=100 + i, dtype=np.complex64)
+        t = loss_targets(truth, seed=200 + i)
+        td = torch.zeros(n, dtype=torch.complex128)
+        td[torch.tensor(list(t.keys()))] = torch.tensor(list(t.values()), dtype=torch.complex128)
+        dense_t.append(td.to(dev))
+        xs.append(torch.from_numpy(x).to(dev).requires_grad_(True))
+        solvers.append(M.Solver(n, M.solve_options(k, sigma, window_delta=DELTA), M.MPSFT_MIXED))
+        seeds.append(300 + i)
+    ag.load()
+
+    def step():
+        loss = torch.zeros((), dtype=torch.float64, device=dev)
+        t0 = time.perf_counter()
+        for x, s_, td, sd in zip(xs, solvers, dense_t, seeds):
+            keys, coefs = ag.recover(x, s_, sd)
+            loss = loss + (coefs - td[keys]).abs().pow(2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(loss)
+
+    for _ in range(warmup):
+        step()
+        for x in xs:
+            x.grad = None
+    fw = bw = 0.0
+    for _ in range(steps):
+        a, b, loss = step()
+        fw += a
+        bw += b
+        for x in xs:
+            x.grad = None
+    fw, bw = fw / steps, bw / steps
+    found = sum(s_.modes()[0].size for s_ in solvers)
+    return {
+        "metric": "ms per differentiable-recovery step (forward + backward), "
+                  f"{count} x complex64 N=2^20 S=100",
+        "value": round((fw + bw) * 1e3, 3), "unit": "ms", "higher_is_better": False,
+        "steps": steps, "warmup": warmup, "dtype": "c64 input, f64 arithmetic",
+        "data": "synthetic",
+        "config": {"workload": f"{count} complex64 signals, n=2^20, S=100, unit |c|, sigma=1e-6, "
+                               "loss sum |c_hat - c_target|^2, c_target = truth (1 + 0.01 CN(0,1))"},
+        "forward_ms": round(fw * 1e3, 3), "backward_ms": round(bw * 1e3, 3),
+        "signals_per_sec": round(count / (fw + bw), 1), "loss": loss, "modes_found": int(found),
+    }
+
+
 def cpu_baseline(x, sample_solves):
     """The FP64 CPU restatement of the reference (single thread, V4-order
     binning) on the plain FP64 workload."""
@@ -231,6 +294,9 @@ def main():
                     help="only the N=2^26 S=4000 clustered MIXED workload (its own JSON line)")
     ap.add_argument("--batch", action="store_true",
                     help="only the batched workload: 256 uneven signals (its own JSON line)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only the differentiable recovery: 32 signals forward + backward "
+                         "(its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one signal's solve per device); "
@@ -248,6 +314,9 @@ def main():
         return
     if args.batch:
         print(json.dumps(batch_workload(M, args.steps, args.warmup)), flush=True)
+        return
+    if args.adjoint:
+        print(json.dumps(adjoint_workload(M, args.steps, args.warmup)), flush=True)
         return
 
     log("generating the N=2^22, S=1000 signal")

@@ -198,6 +198,35 @@ int mpsft_solver_transforms_get(mpsft_solver *s, mpsft_transform *tfs,
  * the work is split, not what is computed. */
 int mpsft_solver_set_tile_residues(mpsft_solver *s, int32_t residues);
 
+/* ---- The differentiable form: no reference counterpart.  For callers
+ * that train through the recovery (loss on the recovered coefficients,
+ * gradient with respect to the signal).  With the decisions of a solve
+ * held fixed (its transforms, q's, bins per iteration and the k1 each bin
+ * identified), Iterate (src/iterate.cc:80-177) is linear in x: BinInTime,
+ * the bin DFT, the BinInFreq subtraction of earlier estimates and the
+ * coefficient estimate.  The recovered coefficients are then c = F x.
+ *
+ * set_record(on): while on, every mpsft_solve keeps those decisions in
+ * device memory (4096 iterations of at most the first iteration's bins).
+ * replay: the last recorded solve's map F applied to x_dev (n samples,
+ * complex128 for MPSFT_FP64 or complex64 for MPSFT_MIXED, whatever the
+ * solver's own precision); the found modes are replaced by F x.  Returns
+ * MPSFT_ERR_SHAPE when no recorded solve is held (none ran, it did not
+ * stop, or the map or tables changed since).
+ * adjoint: out_dev (n samples of `precision`) = F^H g, where g holds, for
+ * each of the m keys (device int64), the complex128 gradient dL/dconj(c)
+ * of a real loss L -- the convention of torch, whose x.grad is then
+ * F^H g.  Keys repeated add; keys outside [0, n) or not found carry
+ * nothing.  Bins above 2047 in the recorded solve: MPSFT_ERR_UNSUPPORTED. */
+int mpsft_solver_set_record(mpsft_solver *s, int32_t on);
+/* The solver's n and precision, and an identifier of the recording it
+ * holds (a new value per recorded solve, 0 when none is held); any output
+ * may be null. */
+int mpsft_solver_info(mpsft_solver *s, int64_t *n, int32_t *precision, int64_t *record_id);
+int mpsft_solver_replay(mpsft_solver *s, const void *x_dev, int precision, void *stream);
+int mpsft_solver_adjoint(mpsft_solver *s, const int64_t *keys_dev, const void *grad_dev,
+                         int64_t m, void *out_dev, int precision, void *stream);
+
 /* ---- The batched form: no reference counterpart.  Demo1::Run
  * (src/demo1.cc:36-59) over many independent signals at once, for callers
  * with many small signals: every stage of an iteration is one launch over

@@ -29,6 +29,8 @@ EXPORTS = [
     "mpsft_solver_destroy", "mpsft_solve", "mpsft_solve_profiled", "mpsft_iterate", "mpsft_solver_modes_count",
     "mpsft_solver_modes_get", "mpsft_solver_modes_set", "mpsft_solver_bins_get",
     "mpsft_solver_log_get", "mpsft_solver_transforms_get", "mpsft_solver_set_tile_residues",
+    "mpsft_solver_set_record", "mpsft_solver_info", "mpsft_solver_replay",
+    "mpsft_solver_adjoint",
     "mpsft_batch_create", "mpsft_batch_destroy", "mpsft_batch_solve", "mpsft_batch_autotune",
     "mpsft_batch_modes_count", "mpsft_batch_modes_get", "mpsft_batch_log_get",
 ]
@@ -125,6 +127,10 @@ def declare(L):
         "mpsft_solver_log_get": (C.c_int, [vp, pi32, pi32, i32, pi32]),
         "mpsft_solver_transforms_get": (C.c_int, [vp, C.POINTER(Transform), pi64, i32, pi32]),
         "mpsft_solver_set_tile_residues": (C.c_int, [vp, i32]),
+        "mpsft_solver_set_record": (C.c_int, [vp, i32]),
+        "mpsft_solver_info": (C.c_int, [vp, pi64, pi32, pi64]),
+        "mpsft_solver_replay": (C.c_int, [vp, vp, C.c_int, vp]),
+        "mpsft_solver_adjoint": (C.c_int, [vp, vp, vp, i64, vp, C.c_int, vp]),
         "mpsft_batch_create": (C.c_int, [i32, pi64, C.POINTER(SolveOptions), C.c_int,
                                          C.POINTER(vp)]),
         "mpsft_batch_destroy": (C.c_int, [vp]),

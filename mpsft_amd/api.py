@@ -108,6 +108,15 @@ class Window:
         return self.p_
 
 
+def _ptr(d):
+    """Device address of a DeviceArray, a torch tensor or a plain int."""
+    if isinstance(d, DeviceArray):
+        return d.ptr
+    if hasattr(d, "data_ptr"):
+        return C.c_void_p(d.data_ptr())
+    return C.c_void_p(int(d))
+
+
 def _as_device(x, precision):
     """numpy complex array -> (DeviceArray, keepalive)."""
     if isinstance(x, DeviceArray):
@@ -287,6 +296,34 @@ class Solver:
         check(lib().mpsft_iterate(self.h, x_dev.ptr, C.byref(iopt), C.byref(tf), q,
                                   C.byref(found), stream), "mpsft_iterate")
         return bool(found.value)
+
+    def set_record(self, on=True):
+        """While on, solves keep their decisions for replay / adjoint."""
+        check(lib().mpsft_solver_set_record(self.h, 1 if on else 0), "set_record")
+
+    def replay(self, x_dev, precision=None, stream=None):
+        """The last recorded solve's map (decisions fixed, linear in x) on
+        x_dev; the found modes become its output."""
+        prec = self.precision if precision is None else precision
+        check(lib().mpsft_solver_replay(self.h, _ptr(x_dev), prec, stream), "mpsft_solver_replay")
+
+    def adjoint_dev(self, keys_dev, grad_dev, m, out_dev, precision=None, stream=None):
+        """out_dev = F^H g for the last recorded solve's map F; keys_dev
+        (int64) and grad_dev (complex128) hold m device entries."""
+        prec = self.precision if precision is None else precision
+        check(lib().mpsft_solver_adjoint(self.h, _ptr(keys_dev), _ptr(grad_dev), int(m),
+                                         _ptr(out_dev), prec, stream), "mpsft_solver_adjoint")
+
+    def adjoint(self, keys, grad, precision=MPSFT_FP64):
+        """Host form of adjoint_dev: numpy keys / complex gradients in,
+        numpy complex array of n out."""
+        keys = np.ascontiguousarray(keys, np.int64)
+        grad = np.ascontiguousarray(grad, np.complex128)
+        kd, gd = DeviceArray.from_numpy(keys), DeviceArray.from_numpy(grad)
+        esz = 16 if precision == MPSFT_FP64 else 8
+        out = DeviceArray(esz * self.n)
+        self.adjoint_dev(kd, gd, keys.size, out, precision)
+        return out.to_numpy(np.complex128 if precision == MPSFT_FP64 else np.complex64, (self.n,))
 
     def modes(self):
         """Found-mode map as (keys int64[m], coefs complex128[m])."""

@@ -1,7 +1,9 @@
-"""Builds libmpsft_hip.so in-tree with hipcc for gfx950.
+"""Builds libmpsft_hip.so in-tree with hipcc for gfx950, and beside it
+libmpsft_torch.so, the torch operator of the differentiable recovery (host
+C++ against libtorch, calling libmpsft_hip.so through its C-ABI).
 
 Run as ``python -m mpsft_amd.build`` (or through ``__graft_entry__.build``).
-The library is always rebuilt from source.
+Both libraries are always rebuilt from source.
 """
 import os
 import shutil
@@ -12,6 +14,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpsft_hip.so")
+TORCH_LIB = os.path.join(HERE, "libmpsft_torch.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "capi.cpp"]
 ARCH = os.environ.get("MPSFT_OFFLOAD_ARCH", "gfx950")
 
@@ -51,7 +55,8 @@ def build(verbose=False):
                 print(err, file=sys.stderr)
             objs.append(obj)
         out_tmp = LIB + ".tmp"
-        cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", out_tmp] + objs
+        cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC",
+               "-Wl,-soname,libmpsft_hip.so", "-o", out_tmp] + objs
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stderr}")
@@ -61,5 +66,31 @@ def build(verbose=False):
     return LIB
 
 
+def build_torch(verbose=False):
+    """Compile csrc/torch_recover.cpp with g++ against the installed torch
+    (its headers, C++ ABI and libraries) and link it to libmpsft_hip.so,
+    found at run time beside it.  Returns its path."""
+    import torch
+    import torch.utils.cpp_extension as ext
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-Wall", "-Wno-unused-variable"]
+    cmd += [f"-I{p}" for p in ext.include_paths()] + [f"-I{INCLUDE}"]
+    cmd += [os.path.join(CSRC, "torch_recover.cpp")]
+    for p in ext.library_paths():
+        cmd += [f"-L{p}", f"-Wl,-rpath,{p}"]
+    out_tmp = TORCH_LIB + ".tmp"
+    cmd += ["-ltorch", "-ltorch_cpu", "-lc10", f"-L{HERE}", "-l:libmpsft_hip.so",
+            "-Wl,-rpath,$ORIGIN", "-o", out_tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on torch_recover.cpp:\n{r.stderr[-4000:]}")
+    os.replace(out_tmp, TORCH_LIB)
+    return TORCH_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_torch(verbose=True))

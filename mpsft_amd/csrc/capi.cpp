@@ -264,6 +264,30 @@ int mpsft_solver_set_tile_residues(mpsft_solver *s, int32_t residues) {
   return s->eng->SetTileResidues(residues);
 }
 
+int mpsft_solver_set_record(mpsft_solver *s, int32_t on) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->SetRecord(on != 0);
+}
+
+int mpsft_solver_info(mpsft_solver *s, int64_t *n, int32_t *precision, int64_t *record_id) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  if (n) *n = s->eng->n();
+  if (precision) *precision = s->eng->precision();
+  if (record_id) *record_id = s->eng->RecordId();
+  return MPSFT_OK;
+}
+
+int mpsft_solver_replay(mpsft_solver *s, const void *x_dev, int precision, void *stream) {
+  if (!s || !x_dev) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->Replay(x_dev, precision, S(stream));
+}
+
+int mpsft_solver_adjoint(mpsft_solver *s, const int64_t *keys_dev, const void *grad_dev,
+                         int64_t m, void *out_dev, int precision, void *stream) {
+  if (!s || !out_dev || m < 0 || (m > 0 && (!keys_dev || !grad_dev))) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->Adjoint(keys_dev, grad_dev, m, out_dev, precision, S(stream));
+}
+
 int mpsft_batch_create(int32_t count, const int64_t *n, const mpsft_solve_options *opts,
                        int precision, mpsft_batch **batch) {
   if (!batch) return MPSFT_ERR_INVALID_ARG;

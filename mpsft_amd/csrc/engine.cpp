@@ -213,6 +213,9 @@ Engine::~Engine() {
   (void)hipFree(d_fold_);
   (void)hipFree(d_A_);
   (void)hipFree(d_big_);
+  (void)hipFree(d_rec_);
+  (void)hipFree(d_lam_);
+  (void)hipFree(d_xbar_);
   (void)hipFree(d_term_bin_);
   (void)hipFree(d_term_val_);
   (void)hipFree(d_keys_);
@@ -501,6 +504,10 @@ Ctx Engine::MakeCtx() const {
   c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
   c.batch_done = nullptr;
   c.batch_host = nullptr;
+  // The recording's rows are rec_B_cap_ apart; a context of another B_cap
+  // (a later Reserve) must not index them.
+  c.rec = ((record_ || rec_valid_) && rec_B_cap_ == B_cap_) ? d_rec_ : nullptr;
+  c.rec_cap = rec_cap_;
   return c;
 }
 
@@ -633,9 +640,11 @@ int Engine::SolveResult(const mpsft_solve_options &opt, int32_t *iters) {
 
 int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
                   int32_t *iters, hipStream_t s, double *kernel_ms) {
+  rec_valid_ = false;
   InitArgs a;
   int rc = PrepareSolve(x, opt, seed, s, &a);
   if (rc != MPSFT_OK) return rc;
+  if (record_ && (rc = EnsureRecord()) != MPSFT_OK) return rc;
   const int32_t max_iter = a.max_iter;
   const int32_t chunk = ChunkIters();
   Ctx c = MakeCtx();
@@ -750,11 +759,135 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
     rc = ReadState(s);
     if (rc != MPSFT_OK) return fail(rc);
   }
-  return SolveResult(opt, iters);
+  rc = SolveResult(opt, iters);
+  FinishRecord(opt, seed, rc, *iters);
+  return rc;
+}
+
+void Engine::FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters) {
+  // Only a loop the device stopped has every iteration's bins logged.
+  const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
+  rec_valid_ = record_ && d_rec_ && stopped &&
+               (rc == MPSFT_OK || rc == MPSFT_ERR_NOT_CONVERGED) && iters <= rec_cap_;
+  if (!rec_valid_) return;
+  ++rec_id_;
+  rec_iters_ = iters;
+  rec_seed_ = seed;
+  rec_opt_ = opt;
+}
+
+// The recording buffer: rec_cap_ iterations of B_cap_ bins.
+int Engine::EnsureRecord() {
+  if (d_rec_ && rec_B_cap_ == B_cap_) return MPSFT_OK;
+  rec_cap_ = 4096;
+  if (Alloc(&d_rec_, size_t(rec_cap_) * B_cap_) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  rec_B_cap_ = B_cap_;
+  ++shape_gen_;  // captured graphs hold the pointer
+  return MPSFT_OK;
+}
+
+int Engine::SetRecord(bool on) {
+  if (status_ != MPSFT_OK) return status_;
+  if (record_ != on) ++shape_gen_;
+  record_ = on;
+  return MPSFT_OK;
+}
+
+// IterParams of a recorded iteration, as k_bin_time derives them on the
+// device in a solve.
+IterParams Engine::HostIterParams(int32_t it, int32_t B) const {
+  const BParams &bp = hbtab_[B];
+  IterParams ip{};
+  ip.tf = h_tfs_[it];
+  for (int32_t t = 0; t < tf_trials_ && t < kMaxTrials; ++t) ip.q[t] = h_qs_[size_t(it) * trials_cap_ + t];
+  ip.tab_off = bp.off;
+  ip.width = bp.width;
+  ip.sigma_f = bp.sigma_f;
+  ip.dfreq = bp.dfreq;
+  ip.bin_thr = bp.bin_thr;
+  ip.sv_thr = bp.sv_thr;
+  ip.B = bp.B;
+  ip.bits = bp.bits;
+  ip.rows = 1 + 2 * bp.bits;
+  ip.f = bp.f;
+  ip.P = bp.P;
+  ip.p2 = (bp.P - 1) / 2;
+  ip.M = bp.M;
+  ip.logM = bp.logM;
+  ip.R = bp.R;
+  ip.it = it;
+  ip.done = 0;
+  return ip;
+}
+
+// The last recorded solve again, on x, with its decisions fixed: the same
+// transforms, bins and identified k1's, so the coefficients are a linear
+// function of x.  One launch sequence per iteration, no graph.
+int Engine::Replay(const void *x, int precision, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!x) return MPSFT_ERR_INVALID_ARG;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  if (!rec_valid_ || !d_rec_ || rec_B_cap_ != B_cap_) return MPSFT_ERR_SHAPE;
+  int rc = EnsureTransforms(rec_seed_, rec_iters_ + 1, rec_opt_.trials, s);
+  if (rc != MPSFT_OK) return rc;
+  Ctx c = MakeCtx();
+  c.mixed = precision == MPSFT_MIXED ? 1 : 0;
+  HIP_OK(LaunchClearModes(c, 0, s));
+  InitArgs a{};
+  a.x = x;
+  a.mode = kModeReplay;
+  a.B = rec_opt_.min_bins;
+  a.trials = rec_opt_.trials;
+  a.k = rec_opt_.k;
+  a.min_bins = rec_opt_.min_bins;
+  a.max_stale = rec_opt_.max_stale_iter;
+  a.max_iter = rec_iters_;
+  a.win_thr = rec_opt_.window_threshold;
+  HIP_OK(LaunchInit(c, a, s));
+  for (int32_t it = 0; it <= rec_iters_; ++it) {
+    rc = LaunchIteration(c, it & 1, s, fft_big_);
+    if (rc != MPSFT_OK) return rc;
+  }
+  return ReadState(s);
+}
+
+int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out, int precision,
+                    hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!out || m < 0 || (m > 0 && (!grad || !keys))) return MPSFT_ERR_INVALID_ARG;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  if (!rec_valid_ || !d_rec_ || rec_B_cap_ != B_cap_) return MPSFT_ERR_SHAPE;
+  int rc = EnsureTransforms(rec_seed_, rec_iters_ + 1, rec_opt_.trials, s);
+  if (rc != MPSFT_OK) return rc;
+  std::vector<int32_t> bins(rec_iters_), found(rec_iters_);
+  int32_t nit = 0;
+  rc = LogGet(bins.data(), found.data(), rec_iters_, &nit);
+  if (rc != MPSFT_OK) return rc;
+  if (nit != rec_iters_) return MPSFT_ERR_SHAPE;
+  for (int32_t it = 0; it < rec_iters_; ++it)  // k_adj_fft is the LDS-resident bin FFT
+    if (bins[it] > kMaxLdsBins) return MPSFT_ERR_UNSUPPORTED;
+  if (!d_lam_ && (Alloc(&d_lam_, size_t(n_)) != MPSFT_OK || Alloc(&d_xbar_, size_t(n_)) != MPSFT_OK))
+    return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemsetAsync(d_lam_, 0, size_t(n_) * sizeof(double2), s));
+  HIP_OK(hipMemsetAsync(d_xbar_, 0, size_t(n_) * sizeof(double2), s));
+  HIP_OK(LaunchAdjSeed(d_lam_, keys, static_cast<const double2 *>(grad), m, n_, s));
+  const Ctx c = MakeCtx();
+  for (int32_t it = rec_iters_ - 1; it >= 0; --it) {
+    const int32_t B = bins[it];
+    const IterParams ip = HostIterParams(it, B);
+    const int32_t count_before = it > 0 ? found[it - 1] : 0;
+    const size_t lds_fold = size_t(ip.f + ip.rows) * ip.R * sizeof(double2);
+    HIP_OK(LaunchAdjIteration(c, ip, rec_opt_.trials, count_before, d_rec_ + size_t(it) * B_cap_,
+                              d_lam_, d_xbar_, IdentifyBlocks(), LdsFft(), lds_fold, s));
+  }
+  HIP_OK(LaunchAdjOut(d_xbar_, out, n_, precision == MPSFT_MIXED ? 1 : 0, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return MPSFT_OK;
 }
 
 int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const DevTf &tf,
                         const int64_t *qs, int32_t *found, hipStream_t s) {
+  rec_valid_ = false;  // the recorded solve's map or tables change
   if (status_ != MPSFT_OK) return status_;
   if (!x || !qs || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
       !(opt.window_threshold > 0) ||
@@ -802,6 +935,7 @@ int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const D
 
 int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const DevTf &tf,
                     int64_t q, double2 *out, hipStream_t s) {
+  rec_valid_ = false;  // the recorded solve's map or tables change
   if (status_ != MPSFT_OK) return status_;
   if (!x || !out || bits < 1 || bits > 30) return MPSFT_ERR_INVALID_ARG;
   std::vector<int32_t> bs{B};
@@ -858,6 +992,7 @@ int Engine::ModesGet(int64_t *keys, double *coefs, int64_t cap, int64_t *count) 
 }
 
 int Engine::ModesSet(const int64_t *keys, const double *coefs, int64_t m) {
+  rec_valid_ = false;  // the recorded solve's map or tables change
   if (m < 0 || m > cap_ || (m > 0 && (!keys || !coefs))) return MPSFT_ERR_INVALID_ARG;
   std::vector<int64_t> hk(keys, keys + m);
   for (auto &k : hk) k = PosMod(k, n_);

@@ -66,7 +66,7 @@ struct DevTf {
   int64_t a, b, c, a_inv;
 };
 
-enum : int32_t { kModeSolve = 0, kModeIterate = 1, kModeBinOnly = 2 };
+enum : int32_t { kModeSolve = 0, kModeIterate = 1, kModeBinOnly = 2, kModeReplay = 3 };
 enum : int32_t { kFlagOverflow = 1 };
 
 // The parameters of one iteration.  Every k_bin_time workgroup derives them
@@ -132,6 +132,8 @@ struct Ctx {
   int32_t trials_cap, rows_cap, B_cap;
   int32_t big_gpr;       // four-step workgroups per row (M_cap / kBigTile), 0 if unused
   int32_t mixed;
+  int64_t *rec;          // recorded solve: k1 of each (iteration, bin), -1 if none; or null
+  int32_t rec_cap;       // iterations rec holds (B_cap entries each)
   int32_t *batch_done;   // batched form: device count of stopped signals, or null
   int32_t *batch_host;   // batched form: host-mapped {iterations, stopped}, or null
 };
@@ -167,6 +169,21 @@ class Engine {
   // k_bin_time tile width: residues per tile (1, 2, 4, 8), or 0 for the
   // default choice.
   int SetTileResidues(int32_t R);
+
+  // Recording: the next solves keep every iteration's identified bins, so
+  // that Replay (the recovery map with those decisions fixed, linear in x)
+  // and Adjoint (its transpose) can run on the last recorded solve.
+  int SetRecord(bool on);
+  // x: device, n samples of `precision` (MPSFT_FP64 or MPSFT_MIXED).
+  int Replay(const void *x, int precision, hipStream_t s);
+  // keys, grad: device, m keys and their complex128 gradients dL/dconj(c);
+  // out: device, n samples of `precision`, gets the recovery map's adjoint
+  // applied to them.
+  int Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out, int precision,
+              hipStream_t s);
+  // Identifies the held recording: a new value per recorded solve, 0 when
+  // none is held.
+  int64_t RecordId() const { return rec_valid_ ? rec_id_ : 0; }
 
  private:
   friend class Batch;
@@ -264,6 +281,22 @@ class Engine {
   int64_t graph_key_ = -1;
   int64_t shape_gen_ = 0;
   int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
+
+  // recorded solve
+  bool record_ = false;
+  int64_t *d_rec_ = nullptr;  // rec_cap_ x B_cap_ k1's
+  int32_t rec_cap_ = 0, rec_B_cap_ = 0;
+  bool rec_valid_ = false;
+  int32_t rec_iters_ = 0;
+  int64_t rec_id_ = 0;
+  uint32_t rec_seed_ = 0;
+  mpsft_solve_options rec_opt_{};
+  double2 *d_lam_ = nullptr, *d_xbar_ = nullptr;
+  int EnsureRecord();
+  // After a solve: keeps its decisions for Replay / Adjoint when the solve
+  // was recorded and its loop stopped on the device.
+  void FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters);
+  IterParams HostIterParams(int32_t it, int32_t B) const;
 };
 
 // Host-side helpers shared with the C-ABI.

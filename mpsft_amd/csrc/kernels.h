@@ -48,6 +48,17 @@ struct BatchResultDev {
   int32_t iters, stop_stale, flags, stopped, last_it, pad;
 };
 hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out, hipStream_t s);
+
+// The adjoint of a recorded solve: seed lam with dL/dc (lam[keys[i]] +=
+// g[i]), then one LaunchAdjIteration per iteration, last iteration first,
+// then LaunchAdjOut.
+hipError_t LaunchAdjSeed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m,
+                         int64_t n, hipStream_t s);
+hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials,
+                              int32_t count_before, const int64_t *rec_row, double2 *lam,
+                              double2 *xbar, int32_t ident_blocks, size_t lds_fft,
+                              size_t lds_fold, hipStream_t s);
+hipError_t LaunchAdjOut(const double2 *xbar, void *out, int64_t n, int mixed, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

@@ -348,7 +348,8 @@ __device__ void NextIteration(const Ctx &c, int p, IterParams *ip, LoopCounters 
   lc->it = prev.it + 1;
   lc->stale = prev.stale + (st->found_iter[1 - p] ? 0 : 1);
   lc->pad = 0;
-  const bool done = (st->flags & kFlagOverflow) || lc->stale >= st->max_stale ||
+  const bool replay = st->mode == kModeReplay;  // a recorded solve's iterations, no stale stop
+  const bool done = (st->flags & kFlagOverflow) || (!replay && lc->stale >= st->max_stale) ||
                     lc->it >= st->max_iter;
   lc->done = done ? 1 : 0;
   ip->done = lc->done;
@@ -358,6 +359,8 @@ __device__ void NextIteration(const Ctx &c, int p, IterParams *ip, LoopCounters 
   if (st->mode == kModeSolve) {
     const int64_t want = (int64_t(st->k) - int64_t(st->count)) * 2 + 1;
     B = int32_t(want > st->min_bins ? want : st->min_bins);
+  } else if (replay) {
+    B = c.log[lc->it];  // the recorded solve's bins
   }
   const BParams bp = c.btab[B];
   FillParams(ip, bp, st->bits_override > 0 ? st->bits_override : bp.bits);
@@ -798,6 +801,8 @@ __device__ __forceinline__ void ComputeStoreR(double2 *a, double2 (&v)[16], int 
 // Either path leaves X_k (chirp and row phase applied) in LDS, at k on the
 // small path and at Pad(k) on the large one.
 
+// CONJ_IN: transform conj(fold) (the adjoint pass feeds conjugated bins).
+template <bool CONJ_IN>
 __device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, const double2 *fold,
                                             double2 rp, double2 *a) {
   constexpr int T = kFftThreads;
@@ -815,7 +820,7 @@ __device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, 
     const int k = tid + r * T;
     fv[r] = cv[r] = make_double2(0.0, 0.0);
     if (k < B) {
-      fv[r] = fold[k];
+      fv[r] = CONJ_IN ? Conj(fold[k]) : fold[k];
       cv[r] = chirp[k];
     }
   }
@@ -870,6 +875,7 @@ __device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, 
   }
 }
 
+template <bool CONJ_IN>
 __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, const double2 *fold,
                                             double2 rp, double2 *a) {
   constexpr int T = kFftThreads;
@@ -899,7 +905,7 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
       if (m < B) {
         const double2 cm = chirp[m];
         lchirp[m] = cm;
-        v[e] = Cmul(fold[m], cm);
+        v[e] = Cmul(CONJ_IN ? Conj(fold[m]) : fold[m], cm);
       }
       hv[e] = filt[m];
     }
@@ -987,9 +993,9 @@ __device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2
   double2 *a = lds;
   const bool small = ip.logM <= kFftSmallLogM;
   if (small)
-    BinFftSmall(c, ip, fold, rp, a);
+    BinFftSmall<false>(c, ip, fold, rp, a);
   else
-    BinFftLarge(c, ip, fold, rp, a);
+    BinFftLarge<false>(c, ip, fold, rp, a);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kPerTerm; ++r) {
@@ -1288,6 +1294,8 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
   }
   __syncthreads();
 
+  const bool replay = st->mode == kModeReplay;
+  int64_t *rec = (c.rec && ip.it < c.rec_cap) ? c.rec + int64_t(ip.it) * c.B_cap : nullptr;
   for (int32_t b = blk * waves + (threadIdx.x >> 6); b < B; b += nblk * waves) {
     const double2 *Ab = c.A + b;
     double2 col[kColSlices];
@@ -1300,6 +1308,15 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
         col[s] = Ab[t * tstride + (p - t * rows) * rstride];
       }
     }
+    int64_t k1;
+    double wf;
+    if (replay) {  // the recorded decision for this bin
+      k1 = rec ? rec[b] : -1;
+      if (k1 < 0) continue;
+      const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
+      wf = SampleInFreq(xi0, width, dfreq);
+    } else {
+    if (rec && lane == 0) rec[b] = -1;
     if (bin_thr > 0) {
       double e = 0;
       for (int t = 0; t < trials; ++t) {
@@ -1340,10 +1357,12 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
     for (int j = 0; j < bits; ++j) xi_sum = (xi_sum << 1) | int32_t((vm >> j) & 1);
     const double xi1 = double(2 * xi_sum + 1) / double(1 << (bits + 1));
     // k1 = std::round(n (b + xi1) / B)  (round half away from zero).
-    const int64_t k1 = int64_t(round(double(n) * (double(b) + xi1) / double(B)));
+    k1 = int64_t(round(double(n) * (double(b) + xi1) / double(B)));
     const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
-    const double wf = SampleInFreq(xi0, width, dfreq);
+    wf = SampleInFreq(xi0, width, dfreq);
     if (fabs(wf) < win_thr) continue;
+    if (rec && lane == 0) rec[b] = k1;
+    }
     // Coefficient estimate (src/iterate.cc:147-166): sum over trials and rows
     // of A * e^{-2 pi i (q + off*B) k1 / n}.
     double2 acc = make_double2(0.0, 0.0);
@@ -1492,6 +1511,202 @@ __global__ void __launch_bounds__(kThreads)
   out[i] = r;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the recovery map with the decisions of a recorded solve held
+// fixed (support, transforms, bins, identified k1's).  The coefficients are
+// then linear in x: per iteration i, A_i = D_i x - T_i c^(i) (BinInTime and
+// BinInFreq) and c^(i+1) = c^(i) + E_i A_i (the estimates of the bins that
+// identified a mode).  Going back over the iterations with lam = dL/dc
+// (dense over k0, the conjugate-Wirtinger convention of torch):
+//   Abar  = E_i^H lam                           k_adj_bins
+//   lam  -= T_i^H Abar                          k_adj_terms
+//   fbar  = conj(rp) DFT^H Abar   (per row)     k_adj_fft
+//   xbar += fold^H fbar                         k_adj_fold (one launch per trial)
+// fold^H scatters through the permutation j = (a u + c) mod n (a bijection
+// for a unit a).  A row's lags span about (2^bits + f) B > n samples, so one
+// sample can be gathered at two (lag, residue) places of an iteration: the
+// scatter adds with device atomics, which collide only there.
+
+// e^{-2 pi i (q + off(row) B) k1 / n}: the estimate's factor for one entry
+// (src/iterate.cc:147-166).
+__device__ __forceinline__ double2 EstFactor(int64_t q, int32_t B, int row, int64_t k1, int64_t n,
+                                             int64_t mask) {
+  const double2 fac = Conj(CisFrac(ModN(q * k1, n, mask), n));
+  if (row == 0) return fac;
+  const int bit = (row - 1) / 2;
+  const double2 fac2 = Conj(CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n));
+  return Cmul(fac, (row & 1) ? fac2 : Conj(fac2));
+}
+
+// lam[keys[i]] += g[i]; keys outside [0, n) are not frequencies of the
+// signal and carry nothing.
+__global__ void __launch_bounds__(kThreads)
+    k_adj_seed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t k = keys[i];
+  if (k < 0 || k >= n) return;
+  atomicAdd(&lam[k].x, g[i].x);
+  atomicAdd(&lam[k].y, g[i].y);
+}
+
+// Abar[t][row][b] = conj(d coef / d A[t][row][b]) lam[k0] for every bin that
+// identified a mode in this iteration, 0 elsewhere.
+__global__ void __launch_bounds__(kThreads)
+    k_adj_bins(Ctx c, IterParams ip, int32_t trials, const double2 *lam, const int64_t *rec_row) {
+  const int32_t B = ip.B, rows = ip.rows;
+  const int32_t npairs = trials * rows;
+  const int64_t n = c.n, mask = c.mask;
+  const int64_t rstride = c.B_cap, tstride = int64_t(c.rows_cap) * c.B_cap;
+  const int lane = threadIdx.x & 63;
+  const int waves = blockDim.x >> 6;
+  const double denom = double(trials * rows);
+  for (int32_t b = blockIdx.x * waves + (threadIdx.x >> 6); b < B; b += gridDim.x * waves) {
+    const int64_t k1 = rec_row[b];
+    double2 g = make_double2(0.0, 0.0), cz = make_double2(0.0, 0.0);
+    double wf = 1.0;
+    if (k1 >= 0) {
+      const int64_t k0 = ModN(ip.tf.a_inv * ModN(k1 - ip.tf.b, n, mask), n, mask);
+      g = lam[k0];
+      cz = Conj(CisFrac(ModN(ip.tf.c * k0, n, mask), n));
+      const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
+      wf = SampleInFreq(xi0, ip.width, ip.dfreq);
+    }
+    for (int pi = lane; pi < npairs; pi += 64) {
+      const int t = pi / rows, row = pi - t * rows;
+      double2 v = make_double2(0.0, 0.0);
+      if (k1 >= 0) {
+        const double2 h = Scale(Cmul(EstFactor(ip.q[t], B, row, k1, n, mask), cz), 1.0 / (wf * denom));
+        v = Cmul(Conj(h), g);
+      }
+      c.A[t * tstride + row * rstride + b] = v;
+    }
+  }
+}
+
+// lam[k0] -= sum_{t,row} conj(term factor) Abar at k0's bin, for every mode
+// in the map before this iteration (keys[0..count_before)).
+__global__ void __launch_bounds__(kThreads)
+    k_adj_terms(Ctx c, IterParams ip, int32_t trials, double2 *lam, int32_t count_before,
+                const int64_t *rec_row) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count_before) return;
+  const int32_t B = ip.B, bits = ip.bits;
+  const int64_t n = c.n, mask = c.mask;
+  const DevTf tf = ip.tf;
+  const int64_t k0 = c.keys[i];
+  const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
+  const int32_t bin = int32_t(k1 * B / n);
+  if (rec_row[bin] < 0) return;  // Abar is 0 there
+  const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
+  const double wf = SampleInFreq(xi, ip.width, ip.dfreq);
+  const int64_t rstride = c.B_cap, tstride = int64_t(c.rows_cap) * c.B_cap;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int t = 0; t < trials; ++t) {
+    const double2 base = Scale(CisFrac(ModN(ip.q[t] * k1 + tf.c * k0, n, mask), n), wf);
+    const double2 *Ab = c.A + t * tstride + bin;
+    acc = Add(acc, Cmul(Conj(base), Ab[0]));
+    for (int32_t bit = 0; bit < bits; ++bit) {
+      const double2 gv = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
+      acc = Add(acc, Cmul(Conj(Cmul(base, gv)), Ab[(2 * bit + 1) * rstride]));
+      acc = Add(acc, Cmul(Conj(Cmul(base, Conj(gv))), Ab[(2 * bit + 2) * rstride]));
+    }
+  }
+  lam[k0] = Sub(lam[k0], acc);
+}
+
+// fbar[t][row] = conj(rp) DFT^H Abar[t][row] = conj(DFT(conj(Abar)) rp).
+__global__ void __launch_bounds__(kFftThreads) k_adj_fft(Ctx c, IterParams ip, int32_t trials) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int32_t trial = blockIdx.x / c.rows_cap;
+  const int32_t row = blockIdx.x - trial * c.rows_cap;
+  if (trial >= trials || row >= ip.rows) return;
+  const int32_t B = ip.B;
+  const int64_t n = c.n, mask = c.mask;
+  const int64_t tau = ModN(ip.q[trial] + RowOffset(row) * B, n, mask);
+  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+  const int64_t off = (int64_t(trial) * c.rows_cap + row) * c.B_cap;
+  double2 *a = lds;
+  const bool small = ip.logM <= kFftSmallLogM;
+  if (small)
+    BinFftSmall<true>(c, ip, c.A + off, rp, a);
+  else
+    BinFftLarge<true>(c, ip, c.A + off, rp, a);
+  __syncthreads();
+  for (int k = threadIdx.x; k < B; k += kFftThreads) c.fold[off + k] = Conj(a[small ? k : Pad(k)]);
+}
+
+// xbar[(a u + c) mod n] += sum over rows of conj(tap) fbar[row][m(rho)], for
+// every staged (lag, residue) of the tile: the fold of k_bin_time read
+// backwards.
+__global__ void __launch_bounds__(kThreads)
+    k_adj_fold(Ctx c, IterParams ip, int32_t trial, double2 *xbar) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
+  const DevTf tf = ip.tf;
+  const int64_t n = c.n, mask = c.mask;
+  const int32_t R = ip.R;
+  const int32_t nblk = (B + R - 1) / R;
+  const int32_t tile = blockIdx.x;
+  if (tile >= nblk) return;
+  const int32_t rho0 = tile * R;
+  const int32_t nrho = min(R, B - rho0);
+  const int logR = R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : 3;
+  const Lags L(bits, f);
+  const int64_t q = ip.q[trial];
+  double2 *T = lds;            // [f][R] taps
+  double2 *FA = T + f * R;     // [rows][R] fbar at the tile's residues
+  const int32_t h = (B - 1) / 2;
+  const double width = ip.width, sigma_f = ip.sigma_f;
+  const double delta = -0.5 / double(B);
+  for (int idx = threadIdx.x; idx < (f << logR); idx += blockDim.x) {
+    const int k = idx >> logR, ri = idx & (R - 1);
+    if (ri < nrho) {
+      const int64_t t = int64_t(k) * B + rho0 + ri - p2;
+      const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
+      const int64_t r = ModN(tf.b * t, n, mask);
+      const double d = delta * double(t);
+      const double turns = double(r) / double(n) + (d - floor(d));
+      T[idx] = Scale(CisTurns(turns), w);
+    }
+  }
+  const double2 *fb = c.fold + int64_t(trial) * c.rows_cap * c.B_cap;
+  for (int idx = threadIdx.x; idx < (rows << logR); idx += blockDim.x) {
+    const int row = idx >> logR, ri = idx & (R - 1);
+    if (ri < nrho) {
+      const int rho = rho0 + ri;
+      const int m = rho >= h ? rho - h : rho - h + B;
+      FA[idx] = fb[int64_t(row) * c.B_cap + m];
+    }
+  }
+  __syncthreads();
+  const int64_t ubase = q - p2 + rho0;
+  for (int idx = threadIdx.x; idx < (L.nlam << logR); idx += blockDim.x) {
+    const int li = idx >> logR, ri = idx & (R - 1);
+    if (ri >= nrho) continue;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int row = 0; row < rows; ++row) {
+      const int k = li - L.RowStart(row);
+      if (k >= 0 && k < f) acc = Add(acc, Cmul(Conj(T[(k << logR) + ri]), FA[(row << logR) + ri]));
+    }
+    const int64_t u = ubase + L.Lambda(li) * B + ri;
+    const int64_t j = ModN(tf.a * u + tf.c, n, mask);
+    atomicAdd(&xbar[j].x, acc.x);
+    atomicAdd(&xbar[j].y, acc.y);
+  }
+}
+
+template <typename Tout>
+__global__ void __launch_bounds__(kThreads)
+    k_adj_out(const double2 *xbar, Tout *out, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = xbar[i];
+    out[i].x = v.x;
+    out[i].y = v.y;
+  }
+}
+
 int32_t Blocks(int64_t work) {
   const int64_t b = (work + kThreads - 1) / kThreads;
   return int32_t(b < 1 ? 1 : (b > 65535 ? 65535 : b));
@@ -1600,6 +1815,40 @@ hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s)
 hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out,
                               hipStream_t s) {
   hipLaunchKernelGGL(k_batch_results, dim3(Blocks(count)), dim3(kThreads), 0, s, ctxs, count, out);
+  return hipGetLastError();
+}
+
+hipError_t LaunchAdjSeed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m,
+                         int64_t n, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_seed, dim3(Blocks(m)), dim3(kThreads), 0, s, lam, keys, g, m, n);
+  return hipGetLastError();
+}
+
+hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials,
+                              int32_t count_before, const int64_t *rec_row, double2 *lam,
+                              double2 *xbar, int32_t ident_blocks, size_t lds_fft,
+                              size_t lds_fold, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_bins, dim3(ident_blocks), dim3(kThreads), 0, s, c, ip, trials, lam,
+                     rec_row);
+  if (count_before > 0)
+    hipLaunchKernelGGL(k_adj_terms, dim3(Blocks(count_before)), dim3(kThreads), 0, s, c, ip,
+                       trials, lam, count_before, rec_row);
+  hipLaunchKernelGGL(k_adj_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds_fft, s,
+                     c, ip, trials);
+  const int32_t tiles = (ip.B + ip.R - 1) / ip.R;
+  for (int32_t t = 0; t < trials; ++t)
+    hipLaunchKernelGGL(k_adj_fold, dim3(tiles), dim3(kThreads), lds_fold, s, c, ip, t, xbar);
+  return hipGetLastError();
+}
+
+hipError_t LaunchAdjOut(const double2 *xbar, void *out, int64_t n, int mixed, hipStream_t s) {
+  if (mixed)
+    hipLaunchKernelGGL(k_adj_out<float2>, dim3(Blocks(n)), dim3(kThreads), 0, s, xbar,
+                       static_cast<float2 *>(out), n);
+  else
+    hipLaunchKernelGGL(k_adj_out<double2>, dim3(Blocks(n)), dim3(kThreads), 0, s, xbar,
+                       static_cast<double2 *>(out), n);
   return hipGetLastError();
 }
 

@@ -79,3 +79,12 @@ def quantization_sigma(x128, x64):
     form sees on top of sigma."""
     d = x64.astype(np.complex128) - x128
     return float(np.sqrt(np.mean(np.abs(d) ** 2)))
+
+
+def loss_targets(truth, seed, rel=0.01):
+    """Targets of the differentiable-recovery loss: each true coefficient
+    times (1 + rel * z), z complex Gaussian with E|z|^2 = 1."""
+    rng = np.random.default_rng(seed)
+    keys = sorted(truth)
+    z = (rng.standard_normal(len(keys)) + 1j * rng.standard_normal(len(keys))) / np.sqrt(2.0)
+    return {key: truth[key] * (1.0 + rel * zz) for key, zz in zip(keys, z)}

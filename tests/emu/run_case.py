@@ -106,7 +106,35 @@ def batch(ns, ks, sigma, seeds, precision=0, autotune=False):
     return out
 
 
+def adjoint(n, k, sigma, seed, trials=1):
+    """Record a solve, replay it, and check the adjoint by the dot-product
+    identity <F dx, g> = <dx, F^H g>."""
+    x = signal(n, k, sigma, seed)
+    s = gpu.Solver(n, gpu.solve_options(k, sigma, trials=trials))
+    s.set_record(True)
+    xd = s.upload(x)
+    iters = s.solve(xd, seed)
+    keys, c0 = s.modes()
+    s.replay(xd)
+    r0 = s.modes_dict()
+    rng = np.random.default_rng(seed + 1)
+    dx = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    s.replay(s.upload(dx))
+    fdx = s.modes_dict()
+    s.replay(s.upload(x + dx))
+    fsum = s.modes_dict()
+    g = rng.standard_normal(keys.size) + 1j * rng.standard_normal(keys.size)
+    adj = s.adjoint(keys, g)
+    fdx_v = np.array([fdx[int(q)] for q in keys])
+    lhs, rhs = np.vdot(g, fdx_v), np.vdot(adj, dx)
+    return dict(iters=iters, n_found=int(keys.size), same_keys=set(r0) == set(map(int, keys)),
+                replay_err=max(abs(r0[int(q)] - v) for q, v in zip(keys, c0)),
+                linear_err=max(abs(fsum[int(q)] - r0[int(q)] - fdx[int(q)]) for q in keys),
+                dot_rel=float(abs(lhs - rhs) / abs(lhs)), dot=abs(lhs))
+
+
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
-    fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch}[case.pop("kind")]
+    fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch,
+          "adjoint": adjoint}[case.pop("kind")]
     print("RESULT " + json.dumps(fn(**case)), flush=True)

@@ -81,6 +81,23 @@ def test_argument_errors_need_no_gpu():
     assert L.mpsft_solver_create(1 << 16, C.byref(opt), 9, 0, C.byref(h)) == 3
     assert L.mpsft_solve(None, None, 0, None, None) == 1
     assert L.mpsft_iterate(None, None, None, None, None, None, None) == 1
+    assert L.mpsft_solver_set_record(None, 1) == 1
+    assert L.mpsft_solver_info(None, None, None, None) == 1
+    assert L.mpsft_solver_replay(None, None, 0, None) == 1
+    assert L.mpsft_solver_adjoint(None, None, None, 0, None, 0, None) == 1
+
+
+def test_torch_operator_registers_and_checks_arguments():
+    """libmpsft_torch.so (the differentiable recovery) loads without a GPU
+    and registers torch.ops.mpsft.recover with its schema."""
+    import torch
+
+    from mpsft_amd import autograd as ag
+    ops = ag.load()
+    assert str(ops.recover.default._schema) == (
+        "mpsft::recover(Tensor x, int solver, int seed, int stream) -> (Tensor, Tensor)")
+    with pytest.raises(RuntimeError, match="null solver"):
+        ops.recover(torch.zeros(8, dtype=torch.complex128), 0, 1, 0)
 
 
 def test_binner_factory_rejects_reference_cpu_types():
