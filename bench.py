@@ -159,8 +159,9 @@ def batch_workload(M, steps, warmup, count=256):
         s_.solve(x, sd)
     M.synchronize()
     t_serial = time.perf_counter() - t0
-    same = sum(1 for i in range(count) if set(b.modes_dict(i)) == set(solvers[i].modes_dict()))
-    hits = sum(sum(1 for key in truths[i] if key in b.modes_dict(i)) for i in range(count))
+    got = [b.modes_dict(i) for i in range(count)]
+    same = sum(1 for i in range(count) if set(got[i]) == set(solvers[i].modes_dict()))
+    hits = sum(sum(1 for key in truths[i] if key in got[i]) for i in range(count))
     return {
         "workload": f"{count} complex64 signals, n=2^U[16,22], S=U[10,500], unit |c|, "
                     "sigma=1e-6, MIXED, one batched solve",

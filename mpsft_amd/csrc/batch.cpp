@@ -122,7 +122,7 @@ int Batch::Build(hipStream_t s) {
     for (int32_t b = 0; b < nf; ++b) map_fft.push_back(make_int2(si, b));
     nblk_ident[i] = e.IdentifyBlocks();
     for (int32_t b = 0; b < nblk_ident[i]; ++b) map_ident.push_back(make_int2(si, b));
-    lds_time = std::max(lds_time, e.LdsBinTime());
+    lds_time = std::max(lds_time, e.LdsBinTime(precision_ == MPSFT_MIXED));
     lds_fft = std::max(lds_fft, e.LdsFft());
     built_gen_[i] = e.shape_gen_;
   }

@@ -342,7 +342,7 @@ int Engine::SetTileResidues(int32_t R) {
 // Sizes the per-iteration buffers for trials and every B in bs.
 int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
   int32_t Bc = 0, rc = 0, tc = 0, Mc = 0, Mbig = 0;
-  size_t lbt = 0, lfft = 0;
+  size_t lbt = 0, lbt32 = 0, lfft = 0;
   bool small = false, big = false;
   for (int32_t B : bs) {
     const BParams &bp = hbtab_[B];
@@ -351,7 +351,10 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     rc = std::max(rc, 1 + 2 * bits);
     tc = std::max(tc, (B + bp.R - 1) / bp.R);
     Mc = std::max(Mc, bp.M);
-    lbt = std::max(lbt, (size_t(NumLags(bits, bp.f) + bp.f) * bp.R + kThreads) * sizeof(double2));
+    const size_t staged = size_t(NumLags(bits, bp.f)) * bp.R;
+    const size_t rest = (size_t(bp.f) * bp.R + kThreads) * sizeof(double2);
+    lbt = std::max(lbt, staged * sizeof(double2) + rest);
+    lbt32 = std::max(lbt32, ((staged + 1) & ~size_t(1)) * sizeof(float2) + rest);
     if (bp.logM <= kMaxLogM) {
       small = true;
       // k_bin_fft: M <= 1024 array + M/2 twiddles; larger M the padded
@@ -396,10 +399,12 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     fft_big_ = big;
     ++shape_gen_;
   }
-  if (tc > tiles_cap_ || Mc > M_cap_ || lbt > lds_bin_time_ || lfft > lds_fft_) {
+  if (tc > tiles_cap_ || Mc > M_cap_ || lbt > lds_bin_time_ || lbt32 > lds_bin_time32_ ||
+      lfft > lds_fft_) {
     tiles_cap_ = std::max(tiles_cap_, tc);
     M_cap_ = std::max(M_cap_, Mc);
     lds_bin_time_ = std::max(lds_bin_time_, lbt);
+    lds_bin_time32_ = std::max(lds_bin_time32_, lbt32);
     lds_fft_ = std::max(lds_fft_, lfft);
     ++shape_gen_;
   }
@@ -511,7 +516,7 @@ Ctx Engine::MakeCtx() const {
   return c;
 }
 
-size_t Engine::LdsBinTime() const { return lds_bin_time_; }
+size_t Engine::LdsBinTime(bool mixed) const { return mixed ? lds_bin_time32_ : lds_bin_time_; }
 size_t Engine::LdsFft() const { return lds_fft_; }
 int32_t Engine::TilesCap() const { return trials_cap_ * tiles_cap_; }
 int32_t Engine::IdentifyBlocks() const {
@@ -529,7 +534,7 @@ int Engine::LaunchFft(const Ctx &c, int parity, hipStream_t s, bool big) {
 }
 
 int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big) {
-  HIP_OK(LaunchBinTime(c, parity, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinTime(c, parity, TilesCap(), LdsBinTime(c.mixed != 0), s));
   const int rc = LaunchFft(c, parity, s, big);
   if (rc != MPSFT_OK) return rc;
   HIP_OK(LaunchIdentify(c, parity, IdentifyBlocks(), s));
@@ -675,7 +680,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       const int par = launched & 1;
       bool ok = hipEventRecord(ev[0], s) == hipSuccess;
       for (int r = 0; r < kTimedRepeats && ok; ++r)
-        ok = LaunchBinTime(c, par, TilesCap(), LdsBinTime(), s) == hipSuccess;
+        ok = LaunchBinTime(c, par, TilesCap(), LdsBinTime(c.mixed != 0), s) == hipSuccess;
       ok = ok && hipEventRecord(ev[1], s) == hipSuccess;
       for (int r = 0; r < kTimedRepeats && ok; ++r) ok = LaunchFft(c, par, s) == MPSFT_OK;
       ok = ok && hipEventRecord(ev[2], s) == hipSuccess &&
@@ -963,7 +968,7 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   a.bits_override = bits;
   a.win_thr = 1;
   HIP_OK(LaunchInit(c, a, s));
-  HIP_OK(LaunchBinTime(c, 0, TilesCap(), LdsBinTime(), s));
+  HIP_OK(LaunchBinTime(c, 0, TilesCap(), LdsBinTime(c.mixed != 0), s));
   rc = LaunchFft(c, 0, s);
   A_override_ = nullptr;
   if (rc != MPSFT_OK) return rc;

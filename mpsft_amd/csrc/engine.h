@@ -200,7 +200,9 @@ class Engine {
   int LaunchFft(const Ctx &c, int parity, hipStream_t s, bool big = true);
   int EnsureGraph(int32_t iters, hipStream_t s);
   int ReadState(hipStream_t s);
-  size_t LdsBinTime() const;
+  // k_bin_time's dynamic LDS: samples staged as stored in x (complex64 for
+  // a mixed launch), taps and the reduction buffer in double2.
+  size_t LdsBinTime(bool mixed) const;
   size_t LdsFft() const;
   int32_t TilesCap() const;
   int32_t IdentifyBlocks() const;
@@ -226,7 +228,7 @@ class Engine {
   bool fft_big_ = false;    // some reserved B runs the four-step kernels
   int32_t big_M_cap_ = 0;   // largest four-step M reserved
   size_t big_len_ = 0;      // elements of d_big_
-  size_t lds_bin_time_ = 0, lds_fft_ = 0;
+  size_t lds_bin_time_ = 0, lds_bin_time32_ = 0, lds_fft_ = 0;
 
   // device memory
   DevState *d_state_ = nullptr;

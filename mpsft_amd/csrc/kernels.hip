@@ -532,9 +532,13 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
   const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
   const Tx *x = static_cast<const Tx *>(st->x);
 
-  double2 *X = lds;               // [nlam][R]
-  double2 *T = lds + L.nlam * R;  // [f][R]
-  double2 *red = T + f * R;       // [kThreads]
+  // [nlam][R] samples as stored in x (complex64 samples take half the LDS
+  // and widen exactly at use), then [f][R] taps and [kThreads] partial sums
+  // in double2 (16-byte aligned).
+  Tx *X = reinterpret_cast<Tx *>(lds);
+  const int xs = sizeof(Tx) == sizeof(double2) ? L.nlam * R : (L.nlam * R + 1) & ~1;
+  double2 *T = reinterpret_cast<double2 *>(X + xs);
+  double2 *red = T + f * R;
 
   // Phase A: gather every distinct sample of the tile once, and phase T:
   // the taps tap(t), t = k*B + rho - p2 (Window::SampleInTime and the phase
@@ -577,7 +581,7 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
 #pragma unroll
     for (int k = 0; k < kGather; ++k) {
       const int idx = base + int(threadIdx.x) + k * kThreads;
-      if (idx < total && (idx & (R - 1)) < nrho) X[idx] = Widen(g[k]);
+      if (idx < total && (idx & (R - 1)) < nrho) X[idx] = g[k];
     }
   }
   __syncthreads();
@@ -599,7 +603,7 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
       ri = pair - row * nrho;
       const int s0 = L.RowStart(row);
       for (int k = part; k < f; k += parts)
-        acc = Add(acc, Cmul(X[(s0 + k) * R + ri], T[k * R + ri]));
+        acc = Add(acc, Cmul(Widen(X[(s0 + k) * R + ri]), T[k * R + ri]));
     }
     if (parts > 1) {
       red[threadIdx.x] = acc;
