@@ -181,14 +181,14 @@ Engine::Engine(int64_t n, int precision, int64_t capacity)
     return;
   }
   int32_t *hp = nullptr;
-  if (hipHostMalloc(reinterpret_cast<void **>(&hp), 4 * sizeof(int32_t),
+  if (hipHostMalloc(reinterpret_cast<void **>(&hp), 8 * sizeof(int32_t),
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer(reinterpret_cast<void **>(&d_progress_), hp, 0) != hipSuccess) {
     status_ = MPSFT_ERR_NOMEM;
     return;
   }
   h_progress_ = hp;
-  hp[0] = hp[1] = hp[2] = hp[3] = 0;
+  for (int i = 0; i < 8; ++i) hp[i] = 0;
   std::memset(h_state_, 0, sizeof(DevState));
   if (hipMemset(d_state_, 0, sizeof(DevState)) != hipSuccess ||
       hipMemset(d_coef_, 0, size_t(n_) * sizeof(double2)) != hipSuccess ||
@@ -237,6 +237,7 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     table_used_ = 0;
     tab_delta_ = delta;
     tab_sigma_ = -1;
+    ++tab_epoch_;
   }
   int32_t Bmax = 0;
   for (int32_t B : bs) {
@@ -251,8 +252,10 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
   std::vector<int32_t> todo;
   int64_t need = 0;
   int32_t Mmax = 0;
+  bool changed = false;
   for (int32_t B : bs) {
     if (built_[B]) continue;
+    changed = true;
     BParams bp;
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
@@ -268,7 +271,11 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     hbtab_[B] = bp;
     built_[B] = 1;
   }
-  if (bits_override != tab_bits_override_) tab_bits_override_ = bits_override;
+  if (bits_override != tab_bits_override_) {
+    tab_bits_override_ = bits_override;
+    ++tab_epoch_;
+  }
+  if (changed) ++tab_epoch_;
   if (table_used_ + need > table_len_) {
     const int64_t len = std::max<int64_t>(table_used_ + need, 2 * table_len_);
     double2 *nt = nullptr;
@@ -289,9 +296,13 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     d_btab_len_ = int32_t(hbtab_.size());
     ++shape_gen_;
     tab_sigma_ = -1;
+    changed = true;
   }
-  HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
-                   hipMemcpyHostToDevice));
+  // The device copy changes only with new entries (a solve with the same
+  // range of bins uploads nothing).
+  if (changed)
+    HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),
+                     hipMemcpyHostToDevice));
   if (!todo.empty()) {
     int32_t *d_list = nullptr;
     if (Alloc(&d_list, todo.size()) != MPSFT_OK) return MPSFT_ERR_NOMEM;
@@ -308,6 +319,7 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
 // (src/demo1.cc:50-51).
 int Engine::SetThresholds(double sigma) {
   if (sigma == tab_sigma_) return MPSFT_OK;
+  ++tab_epoch_;
   for (size_t B = 0; B < hbtab_.size(); ++B) {
     if (!built_[B]) continue;
     hbtab_[B].bin_thr = 3.0 * sigma / std::sqrt(double(B));
@@ -325,6 +337,8 @@ int Engine::SetThresholds(double sigma) {
 int Engine::SetTileResidues(int32_t R) {
   if (status_ != MPSFT_OK) return status_;
   if (R > 8 || (R > 0 && (R & (R - 1)))) return MPSFT_ERR_INVALID_ARG;
+  Drain();
+  ++tab_epoch_;
   forced_R_ = R > 0 ? R : 0;
   for (size_t B = 0; B < hbtab_.size(); ++B) {
     if (!built_[B]) continue;
@@ -550,6 +564,7 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
   if (graph_exec_[0] && graph_key_ == shape_gen_ && graph_iters_ == iters &&
       (want == 1 || graph_exec_[1]))
     return MPSFT_OK;
+  Drain();  // a graph may still run the launches a solve queued past its stop
   for (auto &e : graph_exec_) {
     if (e) (void)hipGraphExecDestroy(e);
     e = nullptr;
@@ -583,7 +598,17 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
   return MPSFT_OK;
 }
 
+// Waits for the launches a solve left queued past its stop (they only read
+// and rewrite the loop state); every call that reads device state back,
+// changes buffers or launches on another stream does so first.
+void Engine::Drain() {
+  if (!tail_pending_) return;
+  (void)hipStreamSynchronize(tail_stream_);
+  tail_pending_ = false;
+}
+
 int Engine::ReadState(hipStream_t s) {
+  Drain();
   HIP_OK(hipMemcpyAsync(h_state_, d_state_, sizeof(DevState), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   return MPSFT_OK;
@@ -603,15 +628,27 @@ int Engine::PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t
   const int64_t want0 = 2 * int64_t(opt.k) + 1;
   const int32_t B0 = int32_t(std::max<int64_t>(want0, opt.min_bins));
   if (B0 > kMaxBins) return MPSFT_ERR_UNSUPPORTED;
-  std::vector<int32_t> bs;
-  for (int32_t B = opt.min_bins; B <= B0; B += 2) bs.push_back(B);
   A_override_ = nullptr;
-  int rc = EnsureTables(bs, opt.window_delta, 0);
-  if (rc != MPSFT_OK) return rc;
-  rc = SetThresholds(opt.sigma);
-  if (rc != MPSFT_OK) return rc;
-  rc = Reserve(opt.trials, bs);
-  if (rc != MPSFT_OK) return rc;
+  int rc = MPSFT_OK;
+  // Tables, thresholds and buffers for these options are in place when
+  // nothing changed them since the last solve with the same options.
+  const bool same = prep_valid_ && prep_gen_ == shape_gen_ && prep_epoch_ == tab_epoch_ &&
+                    std::memcmp(&prep_opt_, &opt, sizeof(opt)) == 0;
+  if (!same) {
+    std::vector<int32_t> bs;
+    for (int32_t B = opt.min_bins; B <= B0; B += 2) bs.push_back(B);
+    prep_valid_ = false;
+    rc = EnsureTables(bs, opt.window_delta, 0);
+    if (rc != MPSFT_OK) return rc;
+    rc = SetThresholds(opt.sigma);
+    if (rc != MPSFT_OK) return rc;
+    rc = Reserve(opt.trials, bs);
+    if (rc != MPSFT_OK) return rc;
+    prep_valid_ = true;
+    prep_opt_ = opt;
+    prep_gen_ = shape_gen_;
+    prep_epoch_ = tab_epoch_;
+  }
   const int32_t max_iter = opt.max_iter > 0 ? opt.max_iter : 100000;
   rc = EnsureTransforms(seed, std::min(max_iter, std::max(256, ChunkIters())), opt.trials, s);
   if (rc != MPSFT_OK) return rc;
@@ -645,6 +682,7 @@ int Engine::SolveResult(const mpsft_solve_options &opt, int32_t *iters) {
 
 int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
                   int32_t *iters, hipStream_t s, double *kernel_ms) {
+  if (tail_stream_ != s || kernel_ms) Drain();
   rec_valid_ = false;
   InitArgs a;
   int rc = PrepareSolve(x, opt, seed, s, &a);
@@ -656,8 +694,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   h_progress_[0] = 0;  // the previous solve has finished; the device writes
   h_progress_[1] = 0;  // these again only after k_init below
   h_progress_[2] = 0;
-  HIP_OK(LaunchClearModes(c, 0, s));
-  HIP_OK(LaunchInit(c, a, s));
+  HIP_OK(LaunchReset(c, a, s));
 
   int32_t launched = 0;
   if (kernel_ms) {
@@ -761,8 +798,20 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
         last_change = std::chrono::steady_clock::now();
       }
     }
-    rc = ReadState(s);
-    if (rc != MPSFT_OK) return fail(rc);
+    if (h_progress_[1]) {
+      // The device reported the stop and its result in pinned memory.  The
+      // launches queued past the stop exit at once without touching the
+      // results; the next call that needs them idle waits (Drain).
+      tail_pending_ = true;
+      tail_stream_ = s;
+      h_state_->L[0].done = 1;
+      h_state_->iters = h_progress_[3];
+      h_state_->stop_stale = h_progress_[4];
+      h_state_->flags = h_progress_[5];
+    } else {
+      rc = ReadState(s);
+      if (rc != MPSFT_OK) return fail(rc);
+    }
   }
   rc = SolveResult(opt, iters);
   FinishRecord(opt, seed, rc, *iters);
@@ -793,6 +842,7 @@ int Engine::EnsureRecord() {
 
 int Engine::SetRecord(bool on) {
   if (status_ != MPSFT_OK) return status_;
+  Drain();
   if (record_ != on) ++shape_gen_;
   record_ = on;
   return MPSFT_OK;
@@ -830,6 +880,7 @@ IterParams Engine::HostIterParams(int32_t it, int32_t B) const {
 // function of x.  One launch sequence per iteration, no graph.
 int Engine::Replay(const void *x, int precision, hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
+  Drain();
   if (!x) return MPSFT_ERR_INVALID_ARG;
   if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
   if (!rec_valid_ || !d_rec_ || rec_B_cap_ != B_cap_) return MPSFT_ERR_SHAPE;
@@ -859,6 +910,7 @@ int Engine::Replay(const void *x, int precision, hipStream_t s) {
 int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out, int precision,
                     hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
+  Drain();
   if (!out || m < 0 || (m > 0 && (!grad || !keys))) return MPSFT_ERR_INVALID_ARG;
   if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
   if (!rec_valid_ || !d_rec_ || rec_B_cap_ != B_cap_) return MPSFT_ERR_SHAPE;
@@ -892,6 +944,7 @@ int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out,
 
 int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const DevTf &tf,
                         const int64_t *qs, int32_t *found, hipStream_t s) {
+  Drain();
   rec_valid_ = false;  // the recorded solve's map or tables change
   if (status_ != MPSFT_OK) return status_;
   if (!x || !qs || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
@@ -940,6 +993,7 @@ int Engine::IterateOnce(const void *x, const mpsft_iterate_options &opt, const D
 
 int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const DevTf &tf,
                     int64_t q, double2 *out, hipStream_t s) {
+  Drain();
   rec_valid_ = false;  // the recorded solve's map or tables change
   if (status_ != MPSFT_OK) return status_;
   if (!x || !out || bits < 1 || bits > 30) return MPSFT_ERR_INVALID_ARG;
@@ -997,6 +1051,7 @@ int Engine::ModesGet(int64_t *keys, double *coefs, int64_t cap, int64_t *count) 
 }
 
 int Engine::ModesSet(const int64_t *keys, const double *coefs, int64_t m) {
+  Drain();
   rec_valid_ = false;  // the recorded solve's map or tables change
   if (m < 0 || m > cap_ || (m > 0 && (!keys || !coefs))) return MPSFT_ERR_INVALID_ARG;
   std::vector<int64_t> hk(keys, keys + m);

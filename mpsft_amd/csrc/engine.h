@@ -126,7 +126,8 @@ struct Ctx {
   double2 *coef;         // n (dense found coefficients)
   uint32_t *present;     // ceil(n/32) bits
   int32_t *log;          // 2 x log_cap
-  int32_t *progress;     // host-mapped {iteration started, stopped, its B, -}, or null
+  int32_t *progress;     // host-mapped {iteration started, stopped, its B, and at the
+                         // stop: iterations, stale count, flags}, or null
   int32_t log_cap;
   int32_t cap;
   int32_t trials_cap, rows_cap, B_cap;
@@ -291,6 +292,16 @@ class Engine {
   bool rec_valid_ = false;
   int32_t rec_iters_ = 0;
   int64_t rec_id_ = 0;
+
+  // launches a solve left queued past its stop, on tail_stream_
+  bool tail_pending_ = false;
+  hipStream_t tail_stream_ = nullptr;
+  void Drain();
+  // PrepareSolve's key: the options, buffer shapes and table contents of
+  // the last full preparation
+  bool prep_valid_ = false;
+  mpsft_solve_options prep_opt_{};
+  int64_t prep_gen_ = -1, tab_epoch_ = 0, prep_epoch_ = -1;
   uint32_t rec_seed_ = 0;
   mpsft_solve_options rec_opt_{};
   double2 *d_lam_ = nullptr, *d_xbar_ = nullptr;

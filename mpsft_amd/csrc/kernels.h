@@ -23,6 +23,8 @@ hipError_t LaunchBuildTables(const int32_t *blist, int32_t nb, const BParams *bt
                              hipStream_t s);
 hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s);
 hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s);
+// LaunchClearModes(c, 0) then LaunchInit in one single-workgroup launch.
+hipError_t LaunchReset(const Ctx &c, const InitArgs &a, hipStream_t s);
 // parity = iteration & 1: the IterParams slot the kernels use.
 hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s);
 hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);

@@ -330,6 +330,14 @@ __global__ void k_init(Ctx c, InitArgs a) {
   InitBody(c, a);
 }
 
+// ModeMap clear and loop-state initialisation in one launch of one
+// workgroup: the clear reads the count that the initialisation resets.
+__global__ void __launch_bounds__(kThreads) k_reset(Ctx c, InitArgs a) {
+  ClearModesBody(c, 0, threadIdx.x, blockDim.x);
+  __syncthreads();
+  if (threadIdx.x == 0) InitBody(c, a);
+}
+
 // The next iteration's parameters, as Demo1::Run derives them
 // (src/demo1.cc:45-58): stale += (previous iteration found nothing); stop
 // at max_stale or max_iter; otherwise B = max((k - |found|)*2 + 1, min_bins)
@@ -402,8 +410,13 @@ __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
     }
   }
   if (c.progress) {
-    if (!lc.done)
+    if (!lc.done) {
       __hip_atomic_store(&c.progress[2], ip.B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {  // the solve's result, read by the host once the stream is idle
+      __hip_atomic_store(&c.progress[3], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[4], lc.stale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[5], st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&c.progress[1], lc.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -1733,6 +1746,11 @@ hipError_t LaunchBuildTables(const int32_t *blist, int32_t nb, const BParams *bt
 
 hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_modes, dim3(1024), dim3(kThreads), 0, s, c, full);
+  return hipGetLastError();
+}
+
+hipError_t LaunchReset(const Ctx &c, const InitArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kThreads), 0, s, c, a);
   return hipGetLastError();
 }
 
