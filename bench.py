@@ -178,15 +178,16 @@ def adjoint_workload(M, steps, warmup, count=32):
     """Differentiable recovery: `count` complex64 signals, n = 2^20, S = 100
     unit modes, sigma = 1e-6; loss = sum |c_hat - c_target|^2 over the
     recovered coefficients with c_target = truth (1 + 0.01 CN(0,1)).  A step
-    is the forward (recorded solve of every signal, torch.ops.mpsft.recover),
-    the loss, and the backward (the adjoint of every signal's recovery)."""
+    is the forward (one batched recorded solve of every signal,
+    torch.ops.mpsft.recover_batch), the loss, and the backward (one batched
+    adjoint of every signal's recovery)."""
     import torch
     from mpsft_amd import autograd as ag
     from mpsft_amd.signals import loss_targets, make_signal
     n, k, sigma = 1 << 20, 100, 1e-6
     dev = torch.device("cuda:0")
     log(f"synthesising {count} complex64 signals n=2^20 S={k}")
-    xs, dense_t, solvers, seeds = [], [], [], []
+    xs, dense_t = [], []
     for i in range(count):
         x, truth = make_signal(n, k, sigma, seed=100 + i, dtype=np.complex64)
         t = loss_targets(truth, seed=200 + i)
@@ -194,22 +195,23 @@ def adjoint_workload(M, steps, warmup, count=32):
         td[torch.tensor(list(t.keys()))] = torch.tensor(list(t.values()), dtype=torch.complex128)
         dense_t.append(td.to(dev))
         xs.append(torch.from_numpy(x).to(dev).requires_grad_(True))
-        solvers.append(M.Solver(n, M.solve_options(k, sigma, window_delta=DELTA), M.MPSFT_MIXED))
-        seeds.append(300 + i)
+    seeds = [300 + i for i in range(count)]
+    batch = M.Batch([n] * count, [M.solve_options(k, sigma, window_delta=DELTA)] * count,
+                    M.MPSFT_MIXED)
     ag.load()
 
     def step():
-        loss = torch.zeros((), dtype=torch.float64, device=dev)
         t0 = time.perf_counter()
-        for x, s_, td, sd in zip(xs, solvers, dense_t, seeds):
-            keys, coefs = ag.recover(x, s_, sd)
-            loss = loss + (coefs - td[keys]).abs().pow(2).sum()
+        keys, coefs = ag.recover_batch(xs, batch, seeds)
+        loss = torch.zeros((), dtype=torch.float64, device=dev)
+        for q, c, td in zip(keys, coefs, dense_t):
+            loss = loss + (c - td[q]).abs().pow(2).sum()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         loss.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        return t1 - t0, t2 - t1, float(loss)
+        return t1 - t0, t2 - t1, float(loss), sum(int(q.numel()) for q in keys)
 
     for _ in range(warmup):
         step()
@@ -217,13 +219,12 @@ def adjoint_workload(M, steps, warmup, count=32):
             x.grad = None
     fw = bw = 0.0
     for _ in range(steps):
-        a, b, loss = step()
+        a, b, loss, found = step()
         fw += a
         bw += b
         for x in xs:
             x.grad = None
     fw, bw = fw / steps, bw / steps
-    found = sum(s_.modes()[0].size for s_ in solvers)
     return {
         "metric": "ms per differentiable-recovery step (forward + backward), "
                   f"{count} x complex64 N=2^20 S=100",
@@ -231,7 +232,8 @@ def adjoint_workload(M, steps, warmup, count=32):
         "steps": steps, "warmup": warmup, "dtype": "c64 input, f64 arithmetic",
         "data": "synthetic",
         "config": {"workload": f"{count} complex64 signals, n=2^20, S=100, unit |c|, sigma=1e-6, "
-                               "loss sum |c_hat - c_target|^2, c_target = truth (1 + 0.01 CN(0,1))"},
+                               "loss sum |c_hat - c_target|^2, c_target = truth (1 + 0.01 CN(0,1)), "
+                               "batched forward and backward"},
         "forward_ms": round(fw * 1e3, 3), "backward_ms": round(bw * 1e3, 3),
         "signals_per_sec": round(count / (fw + bw), 1), "loss": loss, "modes_found": int(found),
     }

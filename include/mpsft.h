@@ -253,6 +253,29 @@ int mpsft_batch_modes_get(mpsft_batch *batch, int32_t i, int64_t *keys, double *
                           int64_t cap, int64_t *count);
 int mpsft_batch_log_get(mpsft_batch *batch, int32_t i, int32_t *bins, int32_t *found,
                         int32_t cap, int32_t *n_iter);
+/* The last batched solve's found-mode counts (host memory, no device
+ * access), and every signal's keys (int64) and coefficients (complex128) in
+ * map order copied into device buffers keys_dev[i], coefs_dev[i] of at least
+ * counts[i] entries (host arrays of device pointers). */
+int mpsft_batch_modes_counts(mpsft_batch *batch, int64_t *counts);
+int mpsft_batch_modes_device(mpsft_batch *batch, int64_t *const *keys_dev,
+                             void *const *coefs_dev, void *stream);
+/* The differentiable form over a batch (see mpsft_solver_set_record):
+ * with recording on, a batched solve keeps every signal's decisions;
+ * replay runs signal i's recorded map on x_dev (as mpsft_solver_replay);
+ * adjoint applies every signal's adjoint at once, one launch per stage for
+ * all signals: out_dev[i] (n[i] samples of `precision`) = F_i^H g_i for the
+ * m[i] keys_dev[i] (int64) and grad_dev[i] (complex128) entries (device
+ * pointers; the arrays of pointers and m are host memory).  info gives
+ * signal i's n, precision and recording identifier. */
+int mpsft_batch_set_record(mpsft_batch *batch, int32_t on);
+int mpsft_batch_info(mpsft_batch *batch, int32_t i, int64_t *n, int32_t *precision,
+                     int64_t *record_id);
+int mpsft_batch_replay(mpsft_batch *batch, int32_t i, const void *x_dev, int precision,
+                       void *stream);
+int mpsft_batch_adjoint(mpsft_batch *batch, const int64_t *const *keys_dev,
+                        const void *const *grad_dev, const int64_t *m, void *const *out_dev,
+                        int precision, void *stream);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,8 @@ EXPORTS = [
     "mpsft_solver_adjoint",
     "mpsft_batch_create", "mpsft_batch_destroy", "mpsft_batch_solve", "mpsft_batch_autotune",
     "mpsft_batch_modes_count", "mpsft_batch_modes_get", "mpsft_batch_log_get",
+    "mpsft_batch_set_record", "mpsft_batch_info", "mpsft_batch_replay", "mpsft_batch_adjoint",
+    "mpsft_batch_modes_counts", "mpsft_batch_modes_device",
 ]
 
 
@@ -140,6 +142,13 @@ def declare(L):
         "mpsft_batch_modes_count": (C.c_int, [vp, i32, pi64]),
         "mpsft_batch_modes_get": (C.c_int, [vp, i32, pi64, pf64, i64, pi64]),
         "mpsft_batch_log_get": (C.c_int, [vp, i32, pi32, pi32, i32, pi32]),
+        "mpsft_batch_set_record": (C.c_int, [vp, i32]),
+        "mpsft_batch_modes_counts": (C.c_int, [vp, pi64]),
+        "mpsft_batch_modes_device": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), vp]),
+        "mpsft_batch_info": (C.c_int, [vp, i32, pi64, pi32, pi64]),
+        "mpsft_batch_replay": (C.c_int, [vp, i32, vp, C.c_int, vp]),
+        "mpsft_batch_adjoint": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), pi64, C.POINTER(vp),
+                                          C.c_int, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -428,6 +428,36 @@ class Batch:
               "mpsft_batch_autotune")
         return r.value
 
+    def set_record(self, on=True):
+        """While on, batched solves keep every signal's decisions."""
+        check(lib().mpsft_batch_set_record(self.h, 1 if on else 0), "batch_set_record")
+
+    def replay(self, i, x_dev, precision=None, stream=None):
+        """Signal i's recorded map on x_dev (see Solver.replay)."""
+        prec = self.precision if precision is None else precision
+        check(lib().mpsft_batch_replay(self.h, i, _ptr(x_dev), prec, stream), "batch_replay")
+
+    def adjoint_dev(self, keys_devs, grad_devs, ms, out_devs, precision=None, stream=None):
+        """Every signal's adjoint in one batched pass (device buffers)."""
+        prec = self.precision if precision is None else precision
+        K = self.count
+        kp = (C.c_void_p * K)(*[_ptr(d).value for d in keys_devs])
+        gp = (C.c_void_p * K)(*[_ptr(d).value for d in grad_devs])
+        op = (C.c_void_p * K)(*[_ptr(d).value for d in out_devs])
+        mm = (C.c_int64 * K)(*[int(v) for v in ms])
+        check(lib().mpsft_batch_adjoint(self.h, kp, gp, mm, op, prec, stream), "batch_adjoint")
+
+    def adjoint(self, keys_list, grad_list, precision=MPSFT_FP64):
+        """Host form: lists of numpy keys / complex gradients in, list of
+        numpy complex arrays (n[i] each) out."""
+        kd = [DeviceArray.from_numpy(np.ascontiguousarray(k, np.int64)) for k in keys_list]
+        gd = [DeviceArray.from_numpy(np.ascontiguousarray(g, np.complex128)) for g in grad_list]
+        esz = 16 if precision == MPSFT_FP64 else 8
+        od = [DeviceArray(esz * n) for n in self.ns]
+        self.adjoint_dev(kd, gd, [len(k) for k in keys_list], od, precision)
+        dt = np.complex128 if precision == MPSFT_FP64 else np.complex64
+        return [o.to_numpy(dt, (n,)) for o, n in zip(od, self.ns)]
+
     def modes_dict(self, i):
         cnt = C.c_int64(0)
         check(lib().mpsft_batch_modes_count(self.h, i, C.byref(cnt)), "batch_modes_count")

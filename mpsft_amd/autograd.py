@@ -40,6 +40,18 @@ def recover(x, solver, seed, stream=None):
     return ops.recover(x, int(solver.h.value), int(seed), int(stream))
 
 
+def recover_batch(xs, batch, seeds, stream=None):
+    """recover over a Batch: xs a list of 1-D device tensors (signal i of
+    n[i] samples in the batch's precision); returns (keys list, coefs list)
+    with every coefs[i] differentiable with respect to xs[i].  One batched
+    solve forward, one batched adjoint backward."""
+    import torch
+    ops = load()
+    if stream is None:
+        stream = torch.cuda.current_stream(xs[0].device).cuda_stream
+    return ops.recover_batch(list(xs), int(batch.h.value), [int(v) for v in seeds], int(stream))
+
+
 def precision_of(dtype):
     import torch
     return MPSFT_MIXED if dtype == torch.complex64 else MPSFT_FP64

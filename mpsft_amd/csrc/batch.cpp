@@ -75,6 +75,9 @@ Batch::~Batch() {
   (void)hipFree(d_nblk_ident_);
   (void)hipFree(d_counters_);
   (void)hipFree(d_results_);
+  (void)hipFree(d_adj_sig_);
+  (void)hipFree(d_modes_out_);
+  (void)hipFree(d_adj_steps_);
   if (h_results_) (void)hipHostFree(h_results_);
   if (h_host_) (void)hipHostFree(const_cast<int32_t *>(h_host_));
 }
@@ -89,8 +92,10 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
   std::vector<Ctx> ctxs(eng_.size());
   for (size_t i = 0; i < eng_.size(); ++i) {
     Engine &e = *eng_[i];
+    e.rec_valid_ = false;
     int rc = e.PrepareSolve(x[i], opts_[i], seeds[i], s, &args_[i]);
     if (rc != MPSFT_OK) return rc;
+    if (e.record_ && (rc = e.EnsureRecord()) != MPSFT_OK) return rc;
     if (e.fft_big_) return MPSFT_ERR_UNSUPPORTED;
     if ((e.precision_ == MPSFT_MIXED) != (precision_ == MPSFT_MIXED)) return MPSFT_ERR_INVALID_ARG;
     ctxs[i] = e.MakeCtx();
@@ -235,6 +240,7 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
 
 int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, int32_t *status,
                  hipStream_t s) {
+  solved_ = false;
   int rc = Prepare(x, seeds, s);
   if (rc != MPSFT_OK) return rc;
   int32_t max_iter = 0;
@@ -248,6 +254,7 @@ int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, in
   HIP_OK(hipMemcpyAsync(h_results_, d_results_, count() * sizeof(BatchResultDev),
                         hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
+  solved_ = true;
   int worst = MPSFT_OK;
   for (int32_t i = 0; i < count(); ++i) {
     const BatchResultDev &r = h_results_[i];
@@ -261,11 +268,121 @@ int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, in
     } else if (r.stop_stale < opts_[i].max_stale_iter) {
       st = MPSFT_ERR_NOT_CONVERGED;
     }
+    eng_[i]->FinishRecord(opts_[i], seeds_[i], st, it, r.stopped != 0);
     if (iters) iters[i] = it;
     if (status) status[i] = st;
     if (st != MPSFT_OK && worst == MPSFT_OK) worst = st;
   }
   return worst;
+}
+
+int Batch::ModesCounts(int64_t *counts) const {
+  if (status_ != MPSFT_OK) return status_;
+  if (!counts) return MPSFT_ERR_INVALID_ARG;
+  if (!solved_) return MPSFT_ERR_SHAPE;
+  for (int32_t i = 0; i < count(); ++i) counts[i] = h_results_[i].count;
+  return MPSFT_OK;
+}
+
+int Batch::ModesToDevice(int64_t *const *keys, void *const *coefs, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!keys || !coefs) return MPSFT_ERR_INVALID_ARG;
+  if (!solved_) return MPSFT_ERR_SHAPE;
+  std::vector<ModesOut> out(count());
+  for (int32_t i = 0; i < count(); ++i) {
+    if (h_results_[i].count > 0 && (!keys[i] || !coefs[i])) return MPSFT_ERR_INVALID_ARG;
+    out[i].keys = keys[i];
+    out[i].coefs = static_cast<double2 *>(coefs[i]);
+    out[i].cap = h_results_[i].count;
+  }
+  if (!d_modes_out_ && AllocDev(&d_modes_out_, count()) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemcpyAsync(d_modes_out_, out.data(), out.size() * sizeof(ModesOut),
+                        hipMemcpyHostToDevice, s));
+  HIP_OK(LaunchBatchModesOut(d_ctxs_, d_modes_out_, count(), s));
+  // out is read by the copy above
+  HIP_OK(hipStreamSynchronize(s));
+  return MPSFT_OK;
+}
+
+int Batch::SetRecord(bool on) {
+  if (status_ != MPSFT_OK) return status_;
+  for (auto &e : eng_) {
+    const int rc = e->SetRecord(on);
+    if (rc != MPSFT_OK) return rc;
+  }
+  return MPSFT_OK;
+}
+
+int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const int64_t *m,
+                   void *const *out, int precision, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!keys || !grad || !m || !out) return MPSFT_ERR_INVALID_ARG;
+  if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
+  const int32_t K = count();
+  if (!graph_) return MPSFT_ERR_SHAPE;  // no batched solve yet
+  std::vector<AdjSig> sig(K);
+  int32_t max_steps = 0;
+  int64_t max_m = 1;
+  size_t lds_fold = 0;
+  for (int32_t i = 0; i < K; ++i) {
+    Engine &e = *eng_[i];
+    if (m[i] < 0 || (m[i] > 0 && (!keys[i] || !grad[i])) || !out[i]) return MPSFT_ERR_INVALID_ARG;
+    if (!e.rec_valid_ || !e.d_rec_ || e.rec_B_cap_ != e.B_cap_ || built_gen_[i] != e.shape_gen_)
+      return MPSFT_ERR_SHAPE;
+    if (!e.d_lam_ && (AllocDev(&e.d_lam_, size_t(e.n_)) != MPSFT_OK ||
+                      AllocDev(&e.d_xbar_, size_t(e.n_)) != MPSFT_OK))
+      return MPSFT_ERR_NOMEM;
+    for (size_t B = 0; B < e.hbtab_.size(); ++B) {
+      if (!e.built_[B]) continue;
+      const BParams &bp = e.hbtab_[B];
+      lds_fold = std::max(lds_fold, size_t(bp.f + 1 + 2 * bp.bits) * bp.R * sizeof(double2));
+    }
+    AdjSig &g = sig[i];
+    g.lam = e.d_lam_;
+    g.xbar = e.d_xbar_;
+    g.rec = e.d_rec_;
+    g.keys = keys[i];
+    g.grad = static_cast<const double2 *>(grad[i]);
+    g.out = out[i];
+    g.n = e.n_;
+    g.m = m[i];
+    g.iters = e.rec_iters_;
+    g.trials = e.rec_opt_.trials;
+    g.tiles_cap = e.tiles_cap_;
+    g.pad = 0;
+    max_steps = std::max(max_steps, e.rec_iters_);
+    max_m = std::max(max_m, m[i]);
+  }
+  if (!d_adj_sig_ && AllocDev(&d_adj_sig_, K) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  const size_t nsteps = size_t(K) * std::max(max_steps, 1);
+  if (nsteps > adj_steps_cap_) {
+    if (AllocDev(&d_adj_steps_, nsteps) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+    adj_steps_cap_ = nsteps;
+  }
+  HIP_OK(hipMemcpyAsync(d_adj_sig_, sig.data(), sig.size() * sizeof(AdjSig), hipMemcpyHostToDevice,
+                        s));
+  AdjBatchLaunch ab{};
+  ab.ctxs = d_ctxs_;
+  ab.sig = d_adj_sig_;
+  ab.steps = d_adj_steps_;
+  ab.count = K;
+  ab.max_steps = max_steps;
+  ab.map_time = bl_.map_time;
+  ab.map_fft = bl_.map_fft;
+  ab.map_ident = bl_.map_ident;
+  ab.nblk_ident = bl_.nblk_ident;
+  ab.nb_time = bl_.nb_time;
+  ab.nb_fft = bl_.nb_fft;
+  ab.nb_ident = bl_.nb_ident;
+  ab.lds_fft = bl_.lds_fft;
+  ab.lds_fold = lds_fold;
+  ab.per_zero = 64;
+  ab.per_seed = int32_t(std::min<int64_t>((max_m + kThreads - 1) / kThreads, 64));
+  ab.mixed = precision == MPSFT_MIXED ? 1 : 0;
+  HIP_OK(LaunchAdjBatch(ab, s));
+  // sig is read by the copy above, the outputs by the caller: wait for both.
+  HIP_OK(hipStreamSynchronize(s));
+  return MPSFT_OK;
 }
 
 int Batch::Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen,

@@ -31,6 +31,20 @@ class Batch {
   // these signals and keeps the fastest; *chosen (may be null) gets it.
   int Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen, hipStream_t s);
 
+  // The differentiable form over the batch: with recording on, a batched
+  // solve keeps every signal's decisions (Engine::SetRecord); Adjoint then
+  // applies every signal's adjoint at once: out[i] (device, n[i] samples of
+  // `precision`) = F_i^H g_i for the m[i] keys / complex128 gradients
+  // keys[i], grad[i] (device).
+  int SetRecord(bool on);
+  // Found-mode counts of the last batched solve (host, no device access),
+  // and every signal's keys / coefficients (complex128) copied into device
+  // buffers keys[i], coefs[i] of at least counts[i] entries, on stream s.
+  int ModesCounts(int64_t *counts) const;
+  int ModesToDevice(int64_t *const *keys, void *const *coefs, hipStream_t s);
+  int Adjoint(const int64_t *const *keys, const void *const *grad, const int64_t *m,
+              void *const *out, int precision, hipStream_t s);
+
  private:
   int Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s);
   int Build(hipStream_t s);
@@ -55,6 +69,11 @@ class Batch {
   int32_t *d_host_ = nullptr;
   BatchLaunch bl_{};
   hipGraphExec_t graph_ = nullptr;
+  AdjSig *d_adj_sig_ = nullptr;
+  ModesOut *d_modes_out_ = nullptr;
+  bool solved_ = false;  // h_results_ hold a finished batched solve
+  AdjStep *d_adj_steps_ = nullptr;
+  size_t adj_steps_cap_ = 0;
 };
 
 }  // namespace mpsft

@@ -328,6 +328,45 @@ int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uin
   return batch->b->Autotune(x_dev, seeds, residues, S(stream));
 }
 
+int mpsft_batch_modes_counts(mpsft_batch *batch, int64_t *counts) {
+  if (!batch || !counts) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->ModesCounts(counts);
+}
+
+int mpsft_batch_modes_device(mpsft_batch *batch, int64_t *const *keys_dev,
+                             void *const *coefs_dev, void *stream) {
+  if (!batch || !keys_dev || !coefs_dev) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->ModesToDevice(keys_dev, coefs_dev, S(stream));
+}
+
+int mpsft_batch_set_record(mpsft_batch *batch, int32_t on) {
+  if (!batch) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->SetRecord(on != 0);
+}
+
+int mpsft_batch_info(mpsft_batch *batch, int32_t i, int64_t *n, int32_t *precision,
+                     int64_t *record_id) {
+  if (!batch || i < 0 || i >= batch->b->count()) return MPSFT_ERR_INVALID_ARG;
+  Engine &e = batch->b->engine(i);
+  if (n) *n = e.n();
+  if (precision) *precision = e.precision();
+  if (record_id) *record_id = e.RecordId();
+  return MPSFT_OK;
+}
+
+int mpsft_batch_replay(mpsft_batch *batch, int32_t i, const void *x_dev, int precision,
+                       void *stream) {
+  if (!batch || !x_dev || i < 0 || i >= batch->b->count()) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->engine(i).Replay(x_dev, precision, S(stream));
+}
+
+int mpsft_batch_adjoint(mpsft_batch *batch, const int64_t *const *keys_dev,
+                        const void *const *grad_dev, const int64_t *m, void *const *out_dev,
+                        int precision, void *stream) {
+  if (!batch || !keys_dev || !grad_dev || !m || !out_dev) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->Adjoint(keys_dev, grad_dev, m, out_dev, precision, S(stream));
+}
+
 int mpsft_batch_modes_count(mpsft_batch *batch, int32_t i, int64_t *count) {
   if (!batch || !count || i < 0 || i >= batch->b->count()) return MPSFT_ERR_INVALID_ARG;
   return batch->b->engine(i).ModesCount(count);

@@ -814,13 +814,13 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
     }
   }
   rc = SolveResult(opt, iters);
-  FinishRecord(opt, seed, rc, *iters);
+  FinishRecord(opt, seed, rc, *iters, (h_state_->L[0].done | h_state_->L[1].done) != 0);
   return rc;
 }
 
-void Engine::FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters) {
-  // Only a loop the device stopped has every iteration's bins logged.
-  const bool stopped = (h_state_->L[0].done | h_state_->L[1].done) != 0;
+// stopped: the device stopped the loop, so every iteration's bins are logged.
+void Engine::FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters,
+                          bool stopped) {
   rec_valid_ = record_ && d_rec_ && stopped &&
                (rc == MPSFT_OK || rc == MPSFT_ERR_NOT_CONVERGED) && iters <= rec_cap_;
   if (!rec_valid_) return;

@@ -308,7 +308,8 @@ class Engine {
   int EnsureRecord();
   // After a solve: keeps its decisions for Replay / Adjoint when the solve
   // was recorded and its loop stopped on the device.
-  void FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters);
+  void FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters,
+                    bool stopped);
   IterParams HostIterParams(int32_t it, int32_t B) const;
 };
 

@@ -47,9 +47,16 @@ hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count
                             int32_t *counters, hipStream_t s);
 hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s);
 struct BatchResultDev {
-  int32_t iters, stop_stale, flags, stopped, last_it, pad;
+  int32_t iters, stop_stale, flags, stopped, last_it, count;
 };
 hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out, hipStream_t s);
+struct ModesOut {  // where one signal's found keys and coefficients go (device)
+  int64_t *keys;
+  double2 *coefs;
+  int64_t cap;     // entries the buffers hold
+};
+// Every signal's found modes, in map order, into caller buffers.
+hipError_t LaunchBatchModesOut(const Ctx *ctxs, const ModesOut *out, int32_t count, hipStream_t s);
 
 // The adjoint of a recorded solve: seed lam with dL/dc (lam[keys[i]] +=
 // g[i]), then one LaunchAdjIteration per iteration, last iteration first,
@@ -61,6 +68,35 @@ hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials
                               double2 *xbar, int32_t ident_blocks, size_t lds_fft,
                               size_t lds_fold, hipStream_t s);
 hipError_t LaunchAdjOut(const double2 *xbar, void *out, int64_t n, int mixed, hipStream_t s);
+
+// The batched adjoint: every signal of a batch at once, one launch per
+// stage and reverse step.
+struct AdjSig {            // one signal (device memory, per call)
+  double2 *lam, *xbar;     // n each
+  const int64_t *rec;      // recorded k1 rows, Ctx.B_cap apart
+  const int64_t *keys;     // m seed keys and their gradients
+  const double2 *grad;
+  void *out;               // n samples of the output precision
+  int64_t n, m;
+  int32_t iters, trials, tiles_cap, pad;
+};
+struct AdjStep {           // one signal at one reverse step (built on the device)
+  IterParams ip;
+  int32_t active, trials, count_before, it;
+};
+struct AdjBatchLaunch {
+  const Ctx *ctxs;
+  const AdjSig *sig;
+  AdjStep *steps;          // max_steps x count
+  int32_t count, max_steps;
+  const int2 *map_time, *map_fft, *map_ident;  // the forward's block maps
+  const int32_t *nblk_ident;
+  int32_t nb_time, nb_fft, nb_ident;
+  size_t lds_fft, lds_fold;
+  int32_t per_zero, per_seed;  // blocks per signal of the zero / seed / out launches
+  int mixed;                   // output precision
+};
+hipError_t LaunchAdjBatch(const AdjBatchLaunch &b, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

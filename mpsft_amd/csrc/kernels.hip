@@ -1524,7 +1524,7 @@ __global__ void __launch_bounds__(kThreads)
   r.flags = st->flags;
   r.stopped = (st->L[0].done | st->L[1].done) ? 1 : 0;
   r.last_it = max(st->L[0].it, st->L[1].it);
-  r.pad = 0;
+  r.count = min(st->count, ctxs[i].cap);
   out[i] = r;
 }
 
@@ -1535,14 +1535,17 @@ __global__ void __launch_bounds__(kThreads)
 // BinInFreq) and c^(i+1) = c^(i) + E_i A_i (the estimates of the bins that
 // identified a mode).  Going back over the iterations with lam = dL/dc
 // (dense over k0, the conjugate-Wirtinger convention of torch):
-//   Abar  = E_i^H lam                           k_adj_bins
-//   lam  -= T_i^H Abar                          k_adj_terms
-//   fbar  = conj(rp) DFT^H Abar   (per row)     k_adj_fft
-//   xbar += fold^H fbar                         k_adj_fold (one launch per trial)
+//   Abar  = E_i^H lam                           AdjBinsBody
+//   lam  -= T_i^H Abar                          AdjTermsBody
+//   fbar  = conj(rp) DFT^H Abar   (per row)     AdjFftBody
+//   xbar += fold^H fbar                         AdjFoldBody
 // fold^H scatters through the permutation j = (a u + c) mod n (a bijection
 // for a unit a).  A row's lags span about (2^bits + f) B > n samples, so one
 // sample can be gathered at two (lag, residue) places of an iteration: the
-// scatter adds with device atomics, which collide only there.
+// scatter adds with device atomics, which collide only there.  The terms
+// update reads Abar and writes lam, the DFT reads Abar and writes fbar, so
+// after the bins stage they may run in either order (the batched form runs
+// the terms with the fold).
 
 // e^{-2 pi i (q + off(row) B) k1 / n}: the estimate's factor for one entry
 // (src/iterate.cc:147-166).
@@ -1556,21 +1559,22 @@ __device__ __forceinline__ double2 EstFactor(int64_t q, int32_t B, int row, int6
 }
 
 // lam[keys[i]] += g[i]; keys outside [0, n) are not frequencies of the
-// signal and carry nothing.
-__global__ void __launch_bounds__(kThreads)
-    k_adj_seed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m, int64_t n) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const int64_t k = keys[i];
-  if (k < 0 || k >= n) return;
-  atomicAdd(&lam[k].x, g[i].x);
-  atomicAdd(&lam[k].y, g[i].y);
+// signal and carry nothing.  Threads tid < m of a grid of gsize.
+__device__ __forceinline__ void AdjSeedBody(double2 *lam, const int64_t *keys, const double2 *g,
+                                            int64_t m, int64_t n, int64_t tid, int64_t gsize) {
+  for (int64_t i = tid; i < m; i += gsize) {
+    const int64_t k = keys[i];
+    if (k < 0 || k >= n) continue;
+    atomicAdd(&lam[k].x, g[i].x);
+    atomicAdd(&lam[k].y, g[i].y);
+  }
 }
 
 // Abar[t][row][b] = conj(d coef / d A[t][row][b]) lam[k0] for every bin that
-// identified a mode in this iteration, 0 elsewhere.
-__global__ void __launch_bounds__(kThreads)
-    k_adj_bins(Ctx c, IterParams ip, int32_t trials, const double2 *lam, const int64_t *rec_row) {
+// identified a mode in this iteration, 0 elsewhere.  One wavefront per bin.
+__device__ __forceinline__ void AdjBinsBody(const Ctx &c, const IterParams &ip, int32_t trials,
+                                            const double2 *lam, const int64_t *rec_row, int blk,
+                                            int nblk) {
   const int32_t B = ip.B, rows = ip.rows;
   const int32_t npairs = trials * rows;
   const int64_t n = c.n, mask = c.mask;
@@ -1578,16 +1582,18 @@ __global__ void __launch_bounds__(kThreads)
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
   const double denom = double(trials * rows);
-  for (int32_t b = blockIdx.x * waves + (threadIdx.x >> 6); b < B; b += gridDim.x * waves) {
+  const DevTf tf = ip.tf;
+  const double width = ip.width, dfreq = ip.dfreq;
+  for (int32_t b = blk * waves + (threadIdx.x >> 6); b < B; b += nblk * waves) {
     const int64_t k1 = rec_row[b];
     double2 g = make_double2(0.0, 0.0), cz = make_double2(0.0, 0.0);
     double wf = 1.0;
     if (k1 >= 0) {
-      const int64_t k0 = ModN(ip.tf.a_inv * ModN(k1 - ip.tf.b, n, mask), n, mask);
+      const int64_t k0 = ModN(tf.a_inv * ModN(k1 - tf.b, n, mask), n, mask);
       g = lam[k0];
-      cz = Conj(CisFrac(ModN(ip.tf.c * k0, n, mask), n));
+      cz = Conj(CisFrac(ModN(tf.c * k0, n, mask), n));
       const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
-      wf = SampleInFreq(xi0, ip.width, ip.dfreq);
+      wf = SampleInFreq(xi0, width, dfreq);
     }
     for (int pi = lane; pi < npairs; pi += 64) {
       const int t = pi / rows, row = pi - t * rows;
@@ -1602,41 +1608,44 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // lam[k0] -= sum_{t,row} conj(term factor) Abar at k0's bin, for every mode
-// in the map before this iteration (keys[0..count_before)).
-__global__ void __launch_bounds__(kThreads)
-    k_adj_terms(Ctx c, IterParams ip, int32_t trials, double2 *lam, int32_t count_before,
-                const int64_t *rec_row) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count_before) return;
+// in the map before this iteration (keys[0..count_before)).  Threads tid of
+// a grid of gsize.
+__device__ __forceinline__ void AdjTermsBody(const Ctx &c, const IterParams &ip, int32_t trials,
+                                             double2 *lam, int32_t count_before,
+                                             const int64_t *rec_row, int64_t tid, int64_t gsize) {
   const int32_t B = ip.B, bits = ip.bits;
   const int64_t n = c.n, mask = c.mask;
   const DevTf tf = ip.tf;
-  const int64_t k0 = c.keys[i];
-  const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
-  const int32_t bin = int32_t(k1 * B / n);
-  if (rec_row[bin] < 0) return;  // Abar is 0 there
-  const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
-  const double wf = SampleInFreq(xi, ip.width, ip.dfreq);
   const int64_t rstride = c.B_cap, tstride = int64_t(c.rows_cap) * c.B_cap;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int t = 0; t < trials; ++t) {
-    const double2 base = Scale(CisFrac(ModN(ip.q[t] * k1 + tf.c * k0, n, mask), n), wf);
-    const double2 *Ab = c.A + t * tstride + bin;
-    acc = Add(acc, Cmul(Conj(base), Ab[0]));
-    for (int32_t bit = 0; bit < bits; ++bit) {
-      const double2 gv = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
-      acc = Add(acc, Cmul(Conj(Cmul(base, gv)), Ab[(2 * bit + 1) * rstride]));
-      acc = Add(acc, Cmul(Conj(Cmul(base, Conj(gv))), Ab[(2 * bit + 2) * rstride]));
+  for (int64_t i = tid; i < count_before; i += gsize) {
+    const int64_t k0 = c.keys[i];
+    const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
+    const int32_t bin = int32_t(k1 * B / n);
+    if (rec_row[bin] < 0) continue;  // Abar is 0 there
+    const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
+    const double wf = SampleInFreq(xi, ip.width, ip.dfreq);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int t = 0; t < trials; ++t) {
+      const double2 base = Scale(CisFrac(ModN(ip.q[t] * k1 + tf.c * k0, n, mask), n), wf);
+      const double2 *Ab = c.A + t * tstride + bin;
+      acc = Add(acc, Cmul(Conj(base), Ab[0]));
+      for (int32_t bit = 0; bit < bits; ++bit) {
+        const double2 gv = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
+        acc = Add(acc, Cmul(Conj(Cmul(base, gv)), Ab[(2 * bit + 1) * rstride]));
+        acc = Add(acc, Cmul(Conj(Cmul(base, Conj(gv))), Ab[(2 * bit + 2) * rstride]));
+      }
     }
+    lam[k0] = Sub(lam[k0], acc);
   }
-  lam[k0] = Sub(lam[k0], acc);
 }
 
-// fbar[t][row] = conj(rp) DFT^H Abar[t][row] = conj(DFT(conj(Abar)) rp).
-__global__ void __launch_bounds__(kFftThreads) k_adj_fft(Ctx c, IterParams ip, int32_t trials) {
-  HIP_DYNAMIC_SHARED(double2, lds)
-  const int32_t trial = blockIdx.x / c.rows_cap;
-  const int32_t row = blockIdx.x - trial * c.rows_cap;
+// fbar[t][row] = conj(rp) DFT^H Abar[t][row] = conj(DFT(conj(Abar)) rp): the
+// forward bin FFT on conj(Abar), output conjugated (the Bluestein matrix is
+// symmetric).  Block blk = trial * rows_cap + row.
+__device__ __forceinline__ void AdjFftBody(const Ctx &c, const IterParams &ip, int32_t trials,
+                                           int blk, double2 *lds) {
+  const int32_t trial = blk / c.rows_cap;
+  const int32_t row = blk - trial * c.rows_cap;
   if (trial >= trials || row >= ip.rows) return;
   const int32_t B = ip.B;
   const int64_t n = c.n, mask = c.mask;
@@ -1654,18 +1663,14 @@ __global__ void __launch_bounds__(kFftThreads) k_adj_fft(Ctx c, IterParams ip, i
 }
 
 // xbar[(a u + c) mod n] += sum over rows of conj(tap) fbar[row][m(rho)], for
-// every staged (lag, residue) of the tile: the fold of k_bin_time read
-// backwards.
-__global__ void __launch_bounds__(kThreads)
-    k_adj_fold(Ctx c, IterParams ip, int32_t trial, double2 *xbar) {
-  HIP_DYNAMIC_SHARED(double2, lds)
+// every staged (lag, residue) of tile `tile` of trial `trial`: the fold of
+// k_bin_time read backwards.  LDS: (f + rows) R double2.
+__device__ __forceinline__ void AdjFoldBody(const Ctx &c, const IterParams &ip, int32_t trial,
+                                            int32_t tile, double2 *xbar, double2 *lds) {
   const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
   const DevTf tf = ip.tf;
   const int64_t n = c.n, mask = c.mask;
   const int32_t R = ip.R;
-  const int32_t nblk = (B + R - 1) / R;
-  const int32_t tile = blockIdx.x;
-  if (tile >= nblk) return;
   const int32_t rho0 = tile * R;
   const int32_t nrho = min(R, B - rho0);
   const int logR = R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : 3;
@@ -1714,14 +1719,151 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 template <typename Tout>
-__global__ void __launch_bounds__(kThreads)
-    k_adj_out(const double2 *xbar, Tout *out, int64_t n) {
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+__device__ __forceinline__ void AdjOutBody(const double2 *xbar, Tout *out, int64_t n, int64_t tid,
+                                           int64_t gsize) {
+  for (int64_t i = tid; i < n; i += gsize) {
     const double2 v = xbar[i];
     out[i].x = v.x;
     out[i].y = v.y;
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_seed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m, int64_t n) {
+  AdjSeedBody(lam, keys, g, m, n, int64_t(blockIdx.x) * blockDim.x + threadIdx.x,
+              int64_t(gridDim.x) * blockDim.x);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_bins(Ctx c, IterParams ip, int32_t trials, const double2 *lam, const int64_t *rec_row) {
+  AdjBinsBody(c, ip, trials, lam, rec_row, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_terms(Ctx c, IterParams ip, int32_t trials, double2 *lam, int32_t count_before,
+                const int64_t *rec_row) {
+  AdjTermsBody(c, ip, trials, lam, count_before, rec_row,
+               int64_t(blockIdx.x) * blockDim.x + threadIdx.x, int64_t(gridDim.x) * blockDim.x);
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_adj_fft(Ctx c, IterParams ip, int32_t trials) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  AdjFftBody(c, ip, trials, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_fold(Ctx c, IterParams ip, int32_t trial, double2 *xbar) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  if (int32_t(blockIdx.x) >= (ip.B + ip.R - 1) / ip.R) return;
+  AdjFoldBody(c, ip, trial, blockIdx.x, xbar, lds);
+}
+
+template <typename Tout>
+__global__ void __launch_bounds__(kThreads)
+    k_adj_out(const double2 *xbar, Tout *out, int64_t n) {
+  AdjOutBody(xbar, out, n, int64_t(blockIdx.x) * blockDim.x + threadIdx.x,
+             int64_t(gridDim.x) * blockDim.x);
+}
+
+// ---- The batched adjoint: one launch per stage over every signal of a
+// batch, reverse step j taking each signal's iteration iters_i - 1 - j.
+// sig[i] holds signal i's buffers; steps (count entries per step) its
+// iteration parameters, built on the device from the recorded log.
+
+// Per signal: IterParams of every reverse step, as NextIteration derives
+// them in a replay (bins from the log, the transform and q's drawn).
+__global__ void __launch_bounds__(kThreads)
+    k_adj_steps_batch(const Ctx *ctxs, const AdjSig *sig, int32_t count, int32_t max_steps,
+                      AdjStep *steps) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(count) * max_steps) return;
+  const int32_t j = int32_t(i / count), s = int32_t(i - int64_t(j) * count);
+  const Ctx &c = ctxs[s];
+  AdjStep &st = steps[i];
+  const int32_t it = sig[s].iters - 1 - j;
+  st.active = it >= 0 ? 1 : 0;
+  st.trials = sig[s].trials;
+  st.count_before = 0;
+  st.it = it;
+  if (it < 0) return;
+  const int32_t B = c.log[it];
+  FillParams(&st.ip, c.btab[B], c.btab[B].bits);
+  st.ip.bin_thr = 0;
+  st.ip.sv_thr = 0;
+  st.ip.tf = c.tfs[it];
+  for (int t = 0; t < kMaxTrials; ++t)
+    st.ip.q[t] = t < st.trials ? c.qs[int64_t(it) * c.trials_cap + t] : 0;
+  st.ip.it = it;
+  st.ip.done = 0;
+  st.count_before = it > 0 ? c.log[c.log_cap + it - 1] : 0;
+}
+
+// Zeroes lam and xbar of every signal (blocks per signal: `per`).
+__global__ void __launch_bounds__(kThreads)
+    k_adj_zero_batch(const AdjSig *sig, int32_t per) {
+  const AdjSig &g = sig[blockIdx.x / per];
+  const int64_t tid = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x;
+  const int64_t gsize = int64_t(per) * blockDim.x;
+  for (int64_t k = tid; k < g.n; k += gsize) {
+    g.lam[k] = make_double2(0.0, 0.0);
+    g.xbar[k] = make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_seed_batch(const AdjSig *sig, int32_t per) {
+  const AdjSig &g = sig[blockIdx.x / per];
+  AdjSeedBody(g.lam, g.keys, g.grad, g.m, g.n, int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x,
+              int64_t(per) * blockDim.x);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_adj_bins_batch(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map,
+                     const int32_t *nblk) {
+  const int2 m = map[blockIdx.x];
+  const AdjStep &st = steps[m.x];
+  if (!st.active) return;
+  const Ctx &c = ctxs[m.x];
+  AdjBinsBody(c, st.ip, st.trials, sig[m.x].lam, sig[m.x].rec + int64_t(st.it) * c.B_cap, m.y,
+              nblk[m.x]);
+}
+
+__global__ void __launch_bounds__(kFftThreads)
+    k_adj_fft_batch(const Ctx *ctxs, const AdjStep *steps, const int2 *map) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int2 m = map[blockIdx.x];
+  const AdjStep &st = steps[m.x];
+  if (!st.active) return;
+  AdjFftBody(ctxs[m.x], st.ip, st.trials, m.y, lds);
+}
+
+// The map of k_bin_time: blocks below kPrepBlocks run the terms update,
+// the others the fold of tile (block - kPrepBlocks) mod tiles_cap of trial
+// (block - kPrepBlocks) / tiles_cap.
+__global__ void __launch_bounds__(kThreads)
+    k_adj_fold_batch(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int2 m = map[blockIdx.x];
+  const AdjStep &st = steps[m.x];
+  if (!st.active) return;
+  const Ctx &c = ctxs[m.x];
+  const AdjSig &g = sig[m.x];
+  if (m.y < kPrepBlocks) {
+    AdjTermsBody(c, st.ip, st.trials, g.lam, st.count_before, g.rec + int64_t(st.it) * c.B_cap,
+                 int64_t(m.y) * blockDim.x + threadIdx.x, int64_t(kPrepBlocks) * blockDim.x);
+    return;
+  }
+  const int32_t b = m.y - kPrepBlocks;
+  const int32_t trial = b / g.tiles_cap, tile = b - trial * g.tiles_cap;
+  if (trial >= st.trials || tile >= (st.ip.B + st.ip.R - 1) / st.ip.R) return;
+  AdjFoldBody(c, st.ip, trial, tile, g.xbar, lds);
+}
+
+template <typename Tout>
+__global__ void __launch_bounds__(kThreads) k_adj_out_batch(const AdjSig *sig, int32_t per) {
+  const AdjSig &g = sig[blockIdx.x / per];
+  AdjOutBody(g.xbar, static_cast<Tout *>(g.out), g.n,
+             int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x, int64_t(per) * blockDim.x);
 }
 
 int32_t Blocks(int64_t work) {
@@ -1834,6 +1976,30 @@ hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s)
   return hipGetLastError();
 }
 
+// Blocks per signal of k_batch_modes_out.
+constexpr int32_t kModesOutBlocks = 8;
+
+__global__ void __launch_bounds__(kThreads)
+    k_batch_modes_out(const Ctx *ctxs, const ModesOut *out) {
+  const int32_t si = blockIdx.x / kModesOutBlocks;
+  const Ctx &c = ctxs[si];
+  const ModesOut o = out[si];
+  const int32_t cnt = int32_t(min<int64_t>(min(c.st->count, c.cap), o.cap));
+  for (int32_t i = (blockIdx.x % kModesOutBlocks) * blockDim.x + threadIdx.x; i < cnt;
+       i += kModesOutBlocks * blockDim.x) {
+    const int64_t k = c.keys[i];
+    o.keys[i] = k;
+    o.coefs[i] = c.coef[k];
+  }
+}
+
+hipError_t LaunchBatchModesOut(const Ctx *ctxs, const ModesOut *out, int32_t count,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_modes_out, dim3(count * kModesOutBlocks), dim3(kThreads), 0, s, ctxs,
+                     out);
+  return hipGetLastError();
+}
+
 hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *out,
                               hipStream_t s) {
   hipLaunchKernelGGL(k_batch_results, dim3(Blocks(count)), dim3(kThreads), 0, s, ctxs, count, out);
@@ -1861,6 +2027,32 @@ hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials
   const int32_t tiles = (ip.B + ip.R - 1) / ip.R;
   for (int32_t t = 0; t < trials; ++t)
     hipLaunchKernelGGL(k_adj_fold, dim3(tiles), dim3(kThreads), lds_fold, s, c, ip, t, xbar);
+  return hipGetLastError();
+}
+
+hipError_t LaunchAdjBatch(const AdjBatchLaunch &b, hipStream_t s) {
+  const int64_t nsteps = int64_t(b.count) * b.max_steps;
+  hipLaunchKernelGGL(k_adj_steps_batch, dim3(Blocks(nsteps)), dim3(kThreads), 0, s, b.ctxs, b.sig,
+                     b.count, b.max_steps, b.steps);
+  hipLaunchKernelGGL(k_adj_zero_batch, dim3(b.count * b.per_zero), dim3(kThreads), 0, s, b.sig,
+                     b.per_zero);
+  hipLaunchKernelGGL(k_adj_seed_batch, dim3(b.count * b.per_seed), dim3(kThreads), 0, s, b.sig,
+                     b.per_seed);
+  for (int32_t j = 0; j < b.max_steps; ++j) {
+    const AdjStep *st = b.steps + int64_t(j) * b.count;
+    hipLaunchKernelGGL(k_adj_bins_batch, dim3(b.nb_ident), dim3(kThreads), 0, s, b.ctxs, b.sig, st,
+                       b.map_ident, b.nblk_ident);
+    hipLaunchKernelGGL(k_adj_fft_batch, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
+                       st, b.map_fft);
+    hipLaunchKernelGGL(k_adj_fold_batch, dim3(b.nb_time), dim3(kThreads), b.lds_fold, s, b.ctxs,
+                       b.sig, st, b.map_time);
+  }
+  if (b.mixed)
+    hipLaunchKernelGGL(k_adj_out_batch<float2>, dim3(b.count * b.per_zero), dim3(kThreads), 0, s,
+                       b.sig, b.per_zero);
+  else
+    hipLaunchKernelGGL(k_adj_out_batch<double2>, dim3(b.count * b.per_zero), dim3(kThreads), 0, s,
+                       b.sig, b.per_zero);
   return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 // operator, over the C-ABI of libmpsft_hip.so:
 //
 //   torch.ops.mpsft.recover(x, solver, seed, stream) -> (keys, coefs)
+//   torch.ops.mpsft.recover_batch(xs, batch, seeds, stream) -> (keys[], coefs[])
 //
 // Forward: Demo1::Run (src/demo1.cc:36-59) on the device signal x, with the
 // solve's decisions recorded (its transforms, q's, bins per iteration and
@@ -103,6 +104,113 @@ class RecoverFn : public torch::autograd::Function<RecoverFn> {
   }
 };
 
+// The same over a batch (an mpsft_batch* as an integer): one batched
+// recorded solve of every signal, and in the backward one batched adjoint.
+class RecoverBatchFn : public torch::autograd::Function<RecoverBatchFn> {
+ public:
+  static variable_list forward(AutogradContext *ctx, at::TensorList xs, int64_t batch,
+                               std::vector<int64_t> seeds, int64_t stream) {
+    auto *b = reinterpret_cast<mpsft_batch *>(batch);
+    TORCH_CHECK(b != nullptr, "mpsft.recover_batch: null batch");
+    const int64_t K = int64_t(xs.size());
+    TORCH_CHECK(int64_t(seeds.size()) == K, "mpsft.recover_batch: one seed per signal");
+    std::vector<const void *> ptrs(K);
+    std::vector<uint32_t> sd(K);
+    std::vector<int64_t> ns(K);
+    int32_t prec = 0;
+    for (int64_t i = 0; i < K; ++i) {
+      int64_t n = 0;
+      Check(mpsft_batch_info(b, int32_t(i), &n, &prec, nullptr), "mpsft_batch_info");
+      const auto want = prec == MPSFT_MIXED ? at::kComplexFloat : at::kComplexDouble;
+      const at::Tensor &x = xs[i];
+      TORCH_CHECK(x.is_cuda() && x.dim() == 1 && x.size(0) == n && x.scalar_type() == want &&
+                      x.is_contiguous(),
+                  "mpsft.recover_batch: signal ", i, " must be a contiguous device vector of ", n,
+                  " ", prec == MPSFT_MIXED ? "complex64" : "complex128", " samples");
+      ptrs[i] = x.data_ptr();
+      sd[i] = uint32_t(seeds[i]);
+      ns[i] = n;
+    }
+    void *st = reinterpret_cast<void *>(stream);
+    Check(mpsft_batch_set_record(b, 1), "mpsft_batch_set_record");
+    std::vector<int32_t> status(K);
+    const int rc = mpsft_batch_solve(b, ptrs.data(), sd.data(), nullptr, status.data(), st);
+    TORCH_CHECK(rc == MPSFT_OK || rc == MPSFT_ERR_NOT_CONVERGED || rc == MPSFT_ERR_CAPACITY,
+                "mpsft: mpsft_batch_solve failed: ", mpsft_status_string(rc));
+    variable_list out(2 * K);
+    std::vector<int64_t> recs(K), counts(K);
+    std::vector<int64_t *> kp(K);
+    std::vector<void *> cp(K);
+    Check(mpsft_batch_modes_counts(b, counts.data()), "mpsft_batch_modes_counts");
+    for (int64_t i = 0; i < K; ++i) {
+      TORCH_CHECK(status[i] == MPSFT_OK || status[i] == MPSFT_ERR_NOT_CONVERGED,
+                  "mpsft.recover_batch: signal ", i, ": ", mpsft_status_string(status[i]));
+      Check(mpsft_batch_info(b, int32_t(i), nullptr, nullptr, &recs[i]), "mpsft_batch_info");
+      TORCH_CHECK(recs[i] != 0, "mpsft.recover_batch: signal ", i,
+                  " kept no recording (its loop did not stop)");
+      out[i] = at::empty({counts[i]}, xs[i].options().dtype(at::kLong));
+      out[K + i] = at::empty({counts[i]}, xs[i].options().dtype(at::kComplexDouble));
+      kp[i] = out[i].data_ptr<int64_t>();
+      cp[i] = out[K + i].data_ptr();
+    }
+    Check(mpsft_batch_modes_device(b, kp.data(), cp.data(), st), "mpsft_batch_modes_device");
+    ctx->saved_data["batch"] = batch;
+    ctx->saved_data["stream"] = stream;
+    ctx->saved_data["records"] = recs;
+    ctx->saved_data["n"] = ns;
+    ctx->saved_data["precision"] = int64_t(prec);
+    variable_list keys(out.begin(), out.begin() + K);
+    ctx->save_for_backward(keys);
+    ctx->mark_non_differentiable(keys);
+    return out;
+  }
+
+  static variable_list backward(AutogradContext *ctx, variable_list grads) {
+    auto *b = reinterpret_cast<mpsft_batch *>(ctx->saved_data["batch"].toInt());
+    const std::vector<int64_t> recs = ctx->saved_data["records"].toIntVector();
+    const std::vector<int64_t> ns = ctx->saved_data["n"].toIntVector();
+    const int prec = int(ctx->saved_data["precision"].toInt());
+    void *st = reinterpret_cast<void *>(ctx->saved_data["stream"].toInt());
+    const variable_list keys = ctx->get_saved_variables();
+    const int64_t K = int64_t(keys.size());
+    const auto xtype = prec == MPSFT_MIXED ? at::kComplexFloat : at::kComplexDouble;
+    std::vector<at::Tensor> gs(K), outs(K);
+    std::vector<const int64_t *> kp(K);
+    std::vector<const void *> gp(K);
+    std::vector<void *> op(K);
+    std::vector<int64_t> ms(K);
+    for (int64_t i = 0; i < K; ++i) {
+      int64_t rec = 0;
+      Check(mpsft_batch_info(b, int32_t(i), nullptr, nullptr, &rec), "mpsft_batch_info");
+      TORCH_CHECK(rec == recs[i],
+                  "mpsft.recover_batch backward: the batch ran another solve after this forward");
+      const at::Tensor &g = grads[K + i];
+      gs[i] = g.defined() ? g.to(at::kComplexDouble).contiguous()
+                          : at::zeros({keys[i].numel()}, keys[i].options().dtype(at::kComplexDouble));
+      outs[i] = at::empty({ns[i]}, keys[i].options().dtype(xtype));
+      kp[i] = keys[i].data_ptr<int64_t>();
+      gp[i] = gs[i].data_ptr();
+      op[i] = outs[i].data_ptr();
+      ms[i] = keys[i].numel();
+    }
+    Check(mpsft_batch_adjoint(b, kp.data(), gp.data(), ms.data(), op.data(), prec, st),
+          "mpsft_batch_adjoint");
+    variable_list res(outs.begin(), outs.end());
+    res.emplace_back();  // batch
+    res.emplace_back();  // seeds
+    res.emplace_back();  // stream
+    return res;
+  }
+};
+
+std::tuple<std::vector<at::Tensor>, std::vector<at::Tensor>> RecoverBatch(
+    at::TensorList xs, int64_t batch, std::vector<int64_t> seeds, int64_t stream) {
+  variable_list out = RecoverBatchFn::apply(xs, batch, seeds, stream);
+  const size_t K = xs.size();
+  return {std::vector<at::Tensor>(out.begin(), out.begin() + K),
+          std::vector<at::Tensor>(out.begin() + K, out.end())};
+}
+
 std::tuple<at::Tensor, at::Tensor> Recover(const at::Tensor &x, int64_t solver, int64_t seed,
                                            int64_t stream) {
   variable_list out = RecoverFn::apply(x, solver, seed, stream);
@@ -113,4 +221,6 @@ std::tuple<at::Tensor, at::Tensor> Recover(const at::Tensor &x, int64_t solver, 
 
 TORCH_LIBRARY(mpsft, m) {
   m.def("recover(Tensor x, int solver, int seed, int stream) -> (Tensor, Tensor)", &Recover);
+  m.def("recover_batch(Tensor[] xs, int batch, int[] seeds, int stream) -> (Tensor[], Tensor[])",
+        &RecoverBatch);
 }

@@ -133,8 +133,42 @@ def adjoint(n, k, sigma, seed, trials=1):
                 dot_rel=float(abs(lhs - rhs) / abs(lhs)), dot=abs(lhs))
 
 
+def batch_adjoint(ns, ks, sigma, seeds, precision=0):
+    """Batched record + adjoint: per signal the dot-product identity with the
+    replayed map, and agreement with the single-signal adjoint."""
+    xs = [signal(n, k, sigma, sd) for n, k, sd in zip(ns, ks, seeds)]
+    xg = [x.astype(np.complex64) if precision else x for x in xs]
+    opts = [gpu.solve_options(k, sigma) for k in ks]
+    b = gpu.Batch(ns, opts, precision)
+    b.set_record(True)
+    xd = b.upload(xg)
+    iters, stats = b.solve(xd, seeds)
+    rng = np.random.default_rng(5)
+    keys_l, g_l, fd_l, d_l = [], [], [], []
+    for i, n in enumerate(ns):
+        keys = np.array(sorted(b.modes_dict(i)), np.int64)
+        d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        b.replay(i, gpu.DeviceArray.from_numpy(d), precision=gpu.MPSFT_FP64)
+        fd = b.modes_dict(i)
+        keys_l.append(keys)
+        g_l.append(rng.standard_normal(keys.size) + 1j * rng.standard_normal(keys.size))
+        fd_l.append(np.array([fd[int(q)] for q in keys]))
+        d_l.append(d)
+    adj = b.adjoint(keys_l, g_l)
+    dot, single = [], []
+    for i, n in enumerate(ns):
+        lhs, rhs = np.vdot(g_l[i], fd_l[i]), np.vdot(adj[i], d_l[i])
+        dot.append(float(abs(lhs - rhs) / abs(lhs)))
+        s = gpu.Solver(n, opts[i], precision)
+        s.set_record(True)
+        s.solve(s.upload(xg[i]), seeds[i])
+        a1 = s.adjoint(keys_l[i], g_l[i])
+        single.append(float(np.abs(a1 - adj[i]).max() / np.abs(a1).max()))
+    return dict(iters=iters, status=stats, dot_rel=dot, vs_single=single)
+
+
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
     fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch,
-          "adjoint": adjoint}[case.pop("kind")]
+          "adjoint": adjoint, "batch_adjoint": batch_adjoint}[case.pop("kind")]
     print("RESULT " + json.dumps(fn(**case)), flush=True)

@@ -91,6 +91,16 @@ def test_emu_replay_and_adjoint(emu_env, n, k, trials):
     assert r["linear_err"] < 1e-13 and r["dot_rel"] < 1e-12
 
 
+def test_emu_batched_adjoint(emu_env):
+    """The batched adjoint (one launch per stage over 3 signals of different
+    n and S): the dot-product identity per signal, and the same output as
+    each signal's single-signal adjoint."""
+    r = run_case(emu_env, kind="batch_adjoint", ns=[4096, 8192, 16384], ks=[5, 12, 20],
+                 sigma=1e-6, seeds=[3, 4, 5])
+    assert r["status"] == [0, 0, 0]
+    assert max(r["dot_rel"]) < 1e-12 and max(r["vs_single"]) < 1e-12
+
+
 @pytest.mark.skipif(os.environ.get("MPSFT_SLOW_TESTS") != "1",
                     reason="about 15 minutes; set MPSFT_SLOW_TESTS=1")
 def test_emu_solve_past_initial_transform_draw(emu_env, tmp_path):

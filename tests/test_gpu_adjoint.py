@@ -98,66 +98,94 @@ def _loss(got, target):
     return sum(abs(v - target.get(q, 0.0)) ** 2 for q, v in got.items())
 
 
+def _targets_dense(t, keys, dev):
+    import torch
+    return torch.tensor([complex(t.get(int(q), 0.0)) for q in keys.cpu().numpy()],
+                        dtype=torch.complex128, device=dev)
+
+
 @pytest.mark.timeout(600)
 def test_autograd_config5_against_finite_differences(gpu):
     """32 complex64 signals, n = 2^20, S = 100, sigma = 1e-6; loss
-    sum |c_hat - c_target|^2 with c_target = truth (1 + 0.01 CN(0,1))."""
+    sum |c_hat - c_target|^2 with c_target = truth (1 + 0.01 CN(0,1)),
+    through the batched operator (one batched solve, one batched adjoint)."""
     import torch
 
     from mpsft_amd import autograd as ag
 
     count, n, k, sigma = 32, 1 << 20, 100, 1e-6
     dev = torch.device("cuda:0")
-    solvers, xs, targets, seeds = [], [], [], []
+    xs, targets = [], []
     for i in range(count):
         x, truth = make_signal(n, k, sigma, seed=100 + i, dtype=np.complex64)
         targets.append(loss_targets(truth, seed=200 + i))
         xs.append(torch.from_numpy(x).to(dev).requires_grad_(True))
-        solvers.append(gpu.Solver(n, gpu.solve_options(k, sigma), gpu.MPSFT_MIXED))
-        seeds.append(300 + i)
-    loss = 0.0
-    for x, s, t, sd in zip(xs, solvers, targets, seeds):
-        keys, coefs = ag.recover(x, s, sd)
-        tk = torch.tensor([complex(t.get(int(q), 0.0)) for q in keys.cpu().numpy()],
-                          dtype=torch.complex128, device=dev)
-        loss = loss + (coefs - tk).abs().pow(2).sum()
+    seeds = [300 + i for i in range(count)]
+    batch = gpu.Batch([n] * count, [gpu.solve_options(k, sigma)] * count, gpu.MPSFT_MIXED)
+    keys, coefs = ag.recover_batch(xs, batch, seeds)
+    loss = sum((c - _targets_dense(t, q, dev)).abs().pow(2).sum()
+               for q, c, t in zip(keys, coefs, targets))
     loss.backward()
-    grads = [x.grad.detach().cpu().numpy().astype(np.complex128) for x in xs]
-    assert all(np.isfinite(g).all() for g in grads)
+    grads = [x.grad.detach().to(torch.complex128) for x in xs]
+    assert all(bool(torch.isfinite(torch.view_as_real(g)).all()) for g in grads)
 
-    # The same loss through the FP64 forward on the same signal values.
-    grads64 = []
-    for x, t, sd in zip(xs, targets, seeds):
+    # The same loss through the FP64 forward (single-signal operator) on the
+    # same signal values.
+    scale, diff = 0.0, 0.0
+    for i, (x, t, sd) in enumerate(zip(xs, targets, seeds)):
         x64 = x.detach().to(torch.complex128).requires_grad_(True)
         s64 = gpu.Solver(n, gpu.solve_options(k, sigma), gpu.MPSFT_FP64)
-        keys, coefs = ag.recover(x64, s64, sd)
-        tk = torch.tensor([complex(t.get(int(q), 0.0)) for q in keys.cpu().numpy()],
-                          dtype=torch.complex128, device=dev)
-        (coefs - tk).abs().pow(2).sum().backward()
-        grads64.append(x64.grad.cpu().numpy())
-    scale = max(np.abs(g).max() for g in grads64)
-    assert max(np.abs(a - b).max() for a, b in zip(grads, grads64)) <= 1e-6 * scale
+        kk, cc = ag.recover(x64, s64, sd)
+        assert torch.equal(kk.sort().values, keys[i].sort().values)
+        (cc - _targets_dense(t, kk, dev)).abs().pow(2).sum().backward()
+        scale = max(scale, float(x64.grad.abs().max()))
+        diff = max(diff, float((x64.grad - grads[i]).abs().max()))
+    assert diff <= 1e-6 * scale, (diff, scale)
 
-    # FP64 central differences of the loss with the decisions held.
+    # FP64 central differences of the loss with the decisions held (FP64
+    # replays of each signal's recorded map).
     h = 1e-6
     base = [x.detach().to(torch.complex128) for x in xs]
-    gdev = [torch.from_numpy(g).to(dev) for g in grads]
     gen = torch.Generator(device=dev)
     gen.manual_seed(17)
     fds, ans = [], []
     for _ in range(64):
         fd = 0.0
         an = 0.0
-        for s, x0, g, t in zip(solvers, base, gdev, targets):
+        for i, (x0, g, t) in enumerate(zip(base, grads, targets)):
             d = torch.randn(n, generator=gen, device=dev, dtype=torch.complex128)
             vals = []
             for sgn in (1.0, -1.0):
                 xp = (x0 + (sgn * h) * d).contiguous()
-                s.replay(xp, precision=gpu.MPSFT_FP64)
-                vals.append(_loss(s.modes_dict(), t))
+                batch.replay(i, xp, precision=gpu.MPSFT_FP64)
+                vals.append(_loss(batch.modes_dict(i), t))
             fd += (vals[0] - vals[1]) / (2 * h)
             an += float(torch.vdot(g, d).real)
         fds.append(fd)
         ans.append(an)
     fds, ans = np.array(fds), np.array(ans)
     assert np.abs(fds - ans).max() <= 1e-5 * np.abs(fds).max(), (fds[:4], ans[:4])
+
+
+def test_batched_adjoint_matches_single(gpu):
+    """mpsft_batch_adjoint over 6 signals of different n and S equals each
+    signal's single-signal adjoint of the same recorded solve."""
+    ns = [1 << 14, 1 << 16, 1 << 18, 1 << 15, 1 << 17, 1 << 16]
+    ks = [10, 40, 150, 25, 90, 60]
+    seeds = [5, 6, 7, 8, 9, 10]
+    xs = [make_signal(n, k, 1e-6, seed=40 + i)[0] for i, (n, k) in enumerate(zip(ns, ks))]
+    opts = [gpu.solve_options(k, 1e-6) for k in ks]
+    b = gpu.Batch(ns, opts)
+    b.set_record(True)
+    _, stats = b.solve(b.upload(xs), seeds)
+    assert stats == [0] * len(ns)
+    rng = np.random.default_rng(3)
+    keys_l = [np.array(sorted(b.modes_dict(i)), np.int64) for i in range(len(ns))]
+    g_l = [rng.standard_normal(q.size) + 1j * rng.standard_normal(q.size) for q in keys_l]
+    adj = b.adjoint(keys_l, g_l)
+    for i, (n, x) in enumerate(zip(ns, xs)):
+        s = gpu.Solver(n, opts[i])
+        s.set_record(True)
+        s.solve(s.upload(x), seeds[i])
+        a1 = s.adjoint(keys_l[i], g_l[i])
+        assert np.abs(a1 - adj[i]).max() <= 1e-12 * np.abs(a1).max(), i
