@@ -193,8 +193,11 @@ def adjoint_workload(M, steps, warmup, count=32):
         t = loss_targets(truth, seed=200 + i)
         td = torch.zeros(n, dtype=torch.complex128)
         td[torch.tensor(list(t.keys()))] = torch.tensor(list(t.values()), dtype=torch.complex128)
-        dense_t.append(td.to(dev))
+        dense_t.append(td)
         xs.append(torch.from_numpy(x).to(dev).requires_grad_(True))
+    # every signal's targets in one vector: signal i's frequency k at i n + k
+    dense_t = torch.cat(dense_t).to(dev)
+    offs = [i * n for i in range(count)]
     seeds = [300 + i for i in range(count)]
     batch = M.Batch([n] * count, [M.solve_options(k, sigma, window_delta=DELTA)] * count,
                     M.MPSFT_MIXED)
@@ -203,9 +206,8 @@ def adjoint_workload(M, steps, warmup, count=32):
     def step():
         t0 = time.perf_counter()
         keys, coefs = ag.recover_batch(xs, batch, seeds)
-        loss = torch.zeros((), dtype=torch.float64, device=dev)
-        for q, c, td in zip(keys, coefs, dense_t):
-            loss = loss + (c - td[q]).abs().pow(2).sum()
+        q = torch.cat([kk + o for kk, o in zip(keys, offs)])
+        loss = (torch.cat(coefs) - dense_t[q]).abs().pow(2).sum()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         loss.backward()

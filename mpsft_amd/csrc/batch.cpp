@@ -76,6 +76,7 @@ Batch::~Batch() {
   (void)hipFree(d_counters_);
   (void)hipFree(d_results_);
   (void)hipFree(d_adj_sig_);
+  (void)hipFree(d_adj_map_fft_);
   (void)hipFree(d_modes_out_);
   (void)hipFree(d_adj_steps_);
   if (h_results_) (void)hipHostFree(h_results_);
@@ -131,6 +132,16 @@ int Batch::Build(hipStream_t s) {
     lds_fft = std::max(lds_fft, e.LdsFft());
     built_gen_[i] = e.shape_gen_;
   }
+  // the adjoint's DFT-and-terms launch: kPrepBlocks terms blocks, then the
+  // DFT blocks, per signal
+  std::vector<int2> map_adj;
+  for (size_t i = 0; i < eng_.size(); ++i)
+    for (int32_t b = 0; b < kPrepBlocks + eng_[i]->trials_cap_ * eng_[i]->rows_cap_; ++b)
+      map_adj.push_back(make_int2(int32_t(i), b));
+  if (AllocDev(&d_adj_map_fft_, map_adj.size()) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemcpy(d_adj_map_fft_, map_adj.data(), map_adj.size() * sizeof(int2),
+                   hipMemcpyHostToDevice));
+  adj_nb_fft_ = int32_t(map_adj.size());
   const size_t total = map_time.size() + map_fft.size() + map_ident.size();
   if (AllocDev(&d_map_, total) != MPSFT_OK || AllocDev(&d_nblk_ident_, eng_.size()) != MPSFT_OK)
     return MPSFT_ERR_NOMEM;
@@ -322,24 +333,19 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   if (!graph_) return MPSFT_ERR_SHAPE;  // no batched solve yet
   std::vector<AdjSig> sig(K);
   int32_t max_steps = 0;
-  int64_t max_m = 1;
-  size_t lds_fold = 0;
+  int64_t max_m = 1, max_n = 1;
   for (int32_t i = 0; i < K; ++i) {
     Engine &e = *eng_[i];
     if (m[i] < 0 || (m[i] > 0 && (!keys[i] || !grad[i])) || !out[i]) return MPSFT_ERR_INVALID_ARG;
     if (!e.rec_valid_ || !e.d_rec_ || e.rec_B_cap_ != e.B_cap_ || built_gen_[i] != e.shape_gen_)
       return MPSFT_ERR_SHAPE;
-    if (!e.d_lam_ && (AllocDev(&e.d_lam_, size_t(e.n_)) != MPSFT_OK ||
-                      AllocDev(&e.d_xbar_, size_t(e.n_)) != MPSFT_OK))
-      return MPSFT_ERR_NOMEM;
-    for (size_t B = 0; B < e.hbtab_.size(); ++B) {
-      if (!e.built_[B]) continue;
-      const BParams &bp = e.hbtab_[B];
-      lds_fold = std::max(lds_fold, size_t(bp.f + 1 + 2 * bp.bits) * bp.R * sizeof(double2));
-    }
+    const int rc = e.EnsureAdjBuffers(e.rec_iters_);
+    if (rc != MPSFT_OK) return rc;
     AdjSig &g = sig[i];
     g.lam = e.d_lam_;
-    g.xbar = e.d_xbar_;
+    g.fbar = e.d_fbar_;
+    g.taps = e.d_taps_;
+    g.taps_stride = e.taps_stride_;
     g.rec = e.d_rec_;
     g.keys = keys[i];
     g.grad = static_cast<const double2 *>(grad[i]);
@@ -348,10 +354,9 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
     g.m = m[i];
     g.iters = e.rec_iters_;
     g.trials = e.rec_opt_.trials;
-    g.tiles_cap = e.tiles_cap_;
-    g.pad = 0;
     max_steps = std::max(max_steps, e.rec_iters_);
     max_m = std::max(max_m, m[i]);
+    max_n = std::max(max_n, e.n_);
   }
   if (!d_adj_sig_ && AllocDev(&d_adj_sig_, K) != MPSFT_OK) return MPSFT_ERR_NOMEM;
   const size_t nsteps = size_t(K) * std::max(max_steps, 1);
@@ -361,25 +366,22 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   }
   HIP_OK(hipMemcpyAsync(d_adj_sig_, sig.data(), sig.size() * sizeof(AdjSig), hipMemcpyHostToDevice,
                         s));
-  AdjBatchLaunch ab{};
-  ab.ctxs = d_ctxs_;
-  ab.sig = d_adj_sig_;
-  ab.steps = d_adj_steps_;
-  ab.count = K;
-  ab.max_steps = max_steps;
-  ab.map_time = bl_.map_time;
-  ab.map_fft = bl_.map_fft;
-  ab.map_ident = bl_.map_ident;
-  ab.nblk_ident = bl_.nblk_ident;
-  ab.nb_time = bl_.nb_time;
-  ab.nb_fft = bl_.nb_fft;
-  ab.nb_ident = bl_.nb_ident;
-  ab.lds_fft = bl_.lds_fft;
-  ab.lds_fold = lds_fold;
-  ab.per_zero = 64;
-  ab.per_seed = int32_t(std::min<int64_t>((max_m + kThreads - 1) / kThreads, 64));
-  ab.mixed = precision == MPSFT_MIXED ? 1 : 0;
-  HIP_OK(LaunchAdjBatch(ab, s));
+  AdjLaunch a{};
+  a.ctxs = d_ctxs_;
+  a.sig = d_adj_sig_;
+  a.steps = d_adj_steps_;
+  a.count = K;
+  a.max_steps = max_steps;
+  a.map_bins = bl_.map_ident;
+  a.nblk_bins = bl_.nblk_ident;
+  a.nb_bins = bl_.nb_ident;
+  a.map_fft = d_adj_map_fft_;
+  a.nb_fft = adj_nb_fft_;
+  a.lds_fft = bl_.lds_fft;
+  a.per_small = int32_t(std::min<int64_t>((max_m + kThreads - 1) / kThreads, 16));
+  a.per_gather = int32_t(std::min<int64_t>(4096, (max_n + 1023) / 1024));
+  a.mixed = precision == MPSFT_MIXED ? 1 : 0;
+  HIP_OK(LaunchAdj(a, s));
   // sig is read by the copy above, the outputs by the caller: wait for both.
   HIP_OK(hipStreamSynchronize(s));
   return MPSFT_OK;

@@ -70,6 +70,8 @@ class Batch {
   BatchLaunch bl_{};
   hipGraphExec_t graph_ = nullptr;
   AdjSig *d_adj_sig_ = nullptr;
+  int2 *d_adj_map_fft_ = nullptr;
+  int32_t adj_nb_fft_ = 0;
   ModesOut *d_modes_out_ = nullptr;
   bool solved_ = false;  // h_results_ hold a finished batched solve
   AdjStep *d_adj_steps_ = nullptr;

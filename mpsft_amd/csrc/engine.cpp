@@ -215,7 +215,11 @@ Engine::~Engine() {
   (void)hipFree(d_big_);
   (void)hipFree(d_rec_);
   (void)hipFree(d_lam_);
-  (void)hipFree(d_xbar_);
+  (void)hipFree(d_fbar_);
+  (void)hipFree(d_taps_);
+  (void)hipFree(d_ctx1_);
+  (void)hipFree(d_sig1_);
+  (void)hipFree(d_steps1_);
   (void)hipFree(d_term_bin_);
   (void)hipFree(d_term_val_);
   (void)hipFree(d_keys_);
@@ -848,33 +852,6 @@ int Engine::SetRecord(bool on) {
   return MPSFT_OK;
 }
 
-// IterParams of a recorded iteration, as k_bin_time derives them on the
-// device in a solve.
-IterParams Engine::HostIterParams(int32_t it, int32_t B) const {
-  const BParams &bp = hbtab_[B];
-  IterParams ip{};
-  ip.tf = h_tfs_[it];
-  for (int32_t t = 0; t < tf_trials_ && t < kMaxTrials; ++t) ip.q[t] = h_qs_[size_t(it) * trials_cap_ + t];
-  ip.tab_off = bp.off;
-  ip.width = bp.width;
-  ip.sigma_f = bp.sigma_f;
-  ip.dfreq = bp.dfreq;
-  ip.bin_thr = bp.bin_thr;
-  ip.sv_thr = bp.sv_thr;
-  ip.B = bp.B;
-  ip.bits = bp.bits;
-  ip.rows = 1 + 2 * bp.bits;
-  ip.f = bp.f;
-  ip.P = bp.P;
-  ip.p2 = (bp.P - 1) / 2;
-  ip.M = bp.M;
-  ip.logM = bp.logM;
-  ip.R = bp.R;
-  ip.it = it;
-  ip.done = 0;
-  return ip;
-}
-
 // The last recorded solve again, on x, with its decisions fixed: the same
 // transforms, bins and identified k1's, so the coefficients are a linear
 // function of x.  One launch sequence per iteration, no graph.
@@ -907,6 +884,31 @@ int Engine::Replay(const void *x, int precision, hipStream_t s) {
   return ReadState(s);
 }
 
+int Engine::EnsureAdjBuffers(int32_t iters) {
+  if (!d_lam_ && Alloc(&d_lam_, size_t(n_)) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  const size_t want = size_t(std::max(iters, 1)) * trials_cap_ * rows_cap_ * B_cap_;
+  if (want > fbar_len_) {
+    if (Alloc(&d_fbar_, want) != MPSFT_OK) {
+      fbar_len_ = 0;
+      return MPSFT_ERR_NOMEM;
+    }
+    fbar_len_ = want;
+  }
+  int64_t stride = 1;
+  for (size_t B = 0; B < hbtab_.size(); ++B)
+    if (built_[B]) stride = std::max<int64_t>(stride, int64_t(hbtab_[B].f) * int64_t(B));
+  taps_stride_ = stride;
+  const size_t twant = size_t(std::max(iters, 1)) * size_t(stride);
+  if (twant > taps_len_) {
+    if (Alloc(&d_taps_, twant) != MPSFT_OK) {
+      taps_len_ = 0;
+      return MPSFT_ERR_NOMEM;
+    }
+    taps_len_ = twant;
+  }
+  return MPSFT_OK;
+}
+
 int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out, int precision,
                     hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
@@ -914,30 +916,46 @@ int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out,
   if (!out || m < 0 || (m > 0 && (!grad || !keys))) return MPSFT_ERR_INVALID_ARG;
   if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
   if (!rec_valid_ || !d_rec_ || rec_B_cap_ != B_cap_) return MPSFT_ERR_SHAPE;
+  if (fft_big_) return MPSFT_ERR_UNSUPPORTED;  // the adjoint DFT is k_bin_fft's (B <= kMaxLdsBins)
   int rc = EnsureTransforms(rec_seed_, rec_iters_ + 1, rec_opt_.trials, s);
   if (rc != MPSFT_OK) return rc;
-  std::vector<int32_t> bins(rec_iters_), found(rec_iters_);
-  int32_t nit = 0;
-  rc = LogGet(bins.data(), found.data(), rec_iters_, &nit);
-  if (rc != MPSFT_OK) return rc;
-  if (nit != rec_iters_) return MPSFT_ERR_SHAPE;
-  for (int32_t it = 0; it < rec_iters_; ++it)  // k_adj_fft is the LDS-resident bin FFT
-    if (bins[it] > kMaxLdsBins) return MPSFT_ERR_UNSUPPORTED;
-  if (!d_lam_ && (Alloc(&d_lam_, size_t(n_)) != MPSFT_OK || Alloc(&d_xbar_, size_t(n_)) != MPSFT_OK))
+  if ((rc = EnsureAdjBuffers(rec_iters_)) != MPSFT_OK) return rc;
+  if (!d_ctx1_ && (Alloc(&d_ctx1_, 1) != MPSFT_OK || Alloc(&d_sig1_, 1) != MPSFT_OK))
     return MPSFT_ERR_NOMEM;
-  HIP_OK(hipMemsetAsync(d_lam_, 0, size_t(n_) * sizeof(double2), s));
-  HIP_OK(hipMemsetAsync(d_xbar_, 0, size_t(n_) * sizeof(double2), s));
-  HIP_OK(LaunchAdjSeed(d_lam_, keys, static_cast<const double2 *>(grad), m, n_, s));
-  const Ctx c = MakeCtx();
-  for (int32_t it = rec_iters_ - 1; it >= 0; --it) {
-    const int32_t B = bins[it];
-    const IterParams ip = HostIterParams(it, B);
-    const int32_t count_before = it > 0 ? found[it - 1] : 0;
-    const size_t lds_fold = size_t(ip.f + ip.rows) * ip.R * sizeof(double2);
-    HIP_OK(LaunchAdjIteration(c, ip, rec_opt_.trials, count_before, d_rec_ + size_t(it) * B_cap_,
-                              d_lam_, d_xbar_, IdentifyBlocks(), LdsFft(), lds_fold, s));
+  if (rec_iters_ > steps1_cap_) {
+    if (Alloc(&d_steps1_, size_t(rec_iters_)) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+    steps1_cap_ = rec_iters_;
   }
-  HIP_OK(LaunchAdjOut(d_xbar_, out, n_, precision == MPSFT_MIXED ? 1 : 0, s));
+  const Ctx c = MakeCtx();
+  AdjSig g{};
+  g.lam = d_lam_;
+  g.fbar = d_fbar_;
+  g.taps = d_taps_;
+  g.taps_stride = taps_stride_;
+  g.rec = d_rec_;
+  g.keys = keys;
+  g.grad = static_cast<const double2 *>(grad);
+  g.out = out;
+  g.n = n_;
+  g.m = m;
+  g.iters = rec_iters_;
+  g.trials = rec_opt_.trials;
+  HIP_OK(hipMemcpyAsync(d_ctx1_, &c, sizeof(Ctx), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_sig1_, &g, sizeof(AdjSig), hipMemcpyHostToDevice, s));
+  AdjLaunch a{};
+  a.ctxs = d_ctx1_;
+  a.sig = d_sig1_;
+  a.steps = d_steps1_;
+  a.count = 1;
+  a.max_steps = rec_iters_;
+  a.nb_bins = IdentifyBlocks();
+  a.nb_fft = kPrepBlocks + trials_cap_ * rows_cap_;
+  a.lds_fft = LdsFft();
+  a.per_small = 16;
+  a.per_gather = int32_t(std::min<int64_t>(4096, (n_ + 1023) / 1024));
+  a.mixed = precision == MPSFT_MIXED ? 1 : 0;
+  HIP_OK(LaunchAdj(a, s));
+  // c and g are read by the copies above; the output is the caller's next read
   HIP_OK(hipStreamSynchronize(s));
   return MPSFT_OK;
 }

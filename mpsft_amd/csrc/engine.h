@@ -139,6 +139,9 @@ struct Ctx {
   int32_t *batch_host;   // batched form: host-mapped {iterations, stopped}, or null
 };
 
+struct AdjSig;
+struct AdjStep;
+
 class Engine {
  public:
   Engine(int64_t n, int precision, int64_t capacity);
@@ -304,13 +307,21 @@ class Engine {
   int64_t prep_gen_ = -1, tab_epoch_ = 0, prep_epoch_ = -1;
   uint32_t rec_seed_ = 0;
   mpsft_solve_options rec_opt_{};
-  double2 *d_lam_ = nullptr, *d_xbar_ = nullptr;
+  double2 *d_lam_ = nullptr, *d_fbar_ = nullptr, *d_taps_ = nullptr;
+  size_t fbar_len_ = 0, taps_len_ = 0;
+  int64_t taps_stride_ = 0;  // max f B over the built tables
+  // one-signal launch state of Adjoint (device): Ctx, AdjSig, steps
+  Ctx *d_ctx1_ = nullptr;
+  AdjSig *d_sig1_ = nullptr;
+  AdjStep *d_steps1_ = nullptr;
+  int32_t steps1_cap_ = 0;
+  // the adjoint buffers of this signal: lam, and fbar for `iters` iterations
+  int EnsureAdjBuffers(int32_t iters);
   int EnsureRecord();
   // After a solve: keeps its decisions for Replay / Adjoint when the solve
   // was recorded and its loop stopped on the device.
   void FinishRecord(const mpsft_solve_options &opt, uint32_t seed, int rc, int32_t iters,
                     bool stopped);
-  IterParams HostIterParams(int32_t it, int32_t B) const;
 };
 
 // Host-side helpers shared with the C-ABI.

@@ -58,45 +58,40 @@ struct ModesOut {  // where one signal's found keys and coefficients go (device)
 // Every signal's found modes, in map order, into caller buffers.
 hipError_t LaunchBatchModesOut(const Ctx *ctxs, const ModesOut *out, int32_t count, hipStream_t s);
 
-// The adjoint of a recorded solve: seed lam with dL/dc (lam[keys[i]] +=
-// g[i]), then one LaunchAdjIteration per iteration, last iteration first,
-// then LaunchAdjOut.
-hipError_t LaunchAdjSeed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m,
-                         int64_t n, hipStream_t s);
-hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials,
-                              int32_t count_before, const int64_t *rec_row, double2 *lam,
-                              double2 *xbar, int32_t ident_blocks, size_t lds_fft,
-                              size_t lds_fold, hipStream_t s);
-hipError_t LaunchAdjOut(const double2 *xbar, void *out, int64_t n, int mixed, hipStream_t s);
-
-// The batched adjoint: every signal of a batch at once, one launch per
-// stage and reverse step.
+// The adjoint of recorded solves, one signal or every signal of a batch at
+// once: per reverse step one launch for the bins and one for the bin DFTs
+// and terms, then one gather launch writing every output sample.
 struct AdjSig {            // one signal (device memory, per call)
-  double2 *lam, *xbar;     // n each
+  double2 *lam;            // n: dL/dc over k0 (read only at found keys)
+  double2 *fbar;           // iters x trials_cap x rows_cap x B_cap: fold gradients
+  double2 *taps;           // iters x taps_stride: each iteration's conjugated taps (f B)
+  int64_t taps_stride;
   const int64_t *rec;      // recorded k1 rows, Ctx.B_cap apart
   const int64_t *keys;     // m seed keys and their gradients
   const double2 *grad;
   void *out;               // n samples of the output precision
   int64_t n, m;
-  int32_t iters, trials, tiles_cap, pad;
+  int32_t iters, trials;
 };
 struct AdjStep {           // one signal at one reverse step (built on the device)
   IterParams ip;
   int32_t active, trials, count_before, it;
 };
-struct AdjBatchLaunch {
+struct AdjLaunch {
   const Ctx *ctxs;
   const AdjSig *sig;
   AdjStep *steps;          // max_steps x count
   int32_t count, max_steps;
-  const int2 *map_time, *map_fft, *map_ident;  // the forward's block maps
-  const int32_t *nblk_ident;
-  int32_t nb_time, nb_fft, nb_ident;
-  size_t lds_fft, lds_fold;
-  int32_t per_zero, per_seed;  // blocks per signal of the zero / seed / out launches
-  int mixed;                   // output precision
+  // block maps {signal, local block}; null for one signal (block = index)
+  const int2 *map_bins, *map_fft;
+  const int32_t *nblk_bins;  // bins blocks per signal (with map_bins)
+  int32_t nb_bins, nb_fft;   // blocks per launch; map_fft: kPrepBlocks terms
+                             // blocks, then trials_cap x rows_cap DFT blocks
+  size_t lds_fft;
+  int32_t per_small, per_gather;  // blocks per signal: clear / seed, gather
+  int mixed;                      // output precision
 };
-hipError_t LaunchAdjBatch(const AdjBatchLaunch &b, hipStream_t s);
+hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

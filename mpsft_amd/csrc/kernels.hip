@@ -1535,17 +1535,18 @@ __global__ void __launch_bounds__(kThreads)
 // BinInFreq) and c^(i+1) = c^(i) + E_i A_i (the estimates of the bins that
 // identified a mode).  Going back over the iterations with lam = dL/dc
 // (dense over k0, the conjugate-Wirtinger convention of torch):
-//   Abar  = E_i^H lam                           AdjBinsBody
-//   lam  -= T_i^H Abar                          AdjTermsBody
-//   fbar  = conj(rp) DFT^H Abar   (per row)     AdjFftBody
-//   xbar += fold^H fbar                         AdjFoldBody
-// fold^H scatters through the permutation j = (a u + c) mod n (a bijection
-// for a unit a).  A row's lags span about (2^bits + f) B > n samples, so one
-// sample can be gathered at two (lag, residue) places of an iteration: the
-// scatter adds with device atomics, which collide only there.  The terms
-// update reads Abar and writes lam, the DFT reads Abar and writes fbar, so
-// after the bins stage they may run in either order (the batched form runs
-// the terms with the fold).
+//   Abar    = E_i^H lam                          AdjBinsBody
+//   lam    -= T_i^H Abar                         AdjTermsBody
+//   fbar_i  = conj(rp) DFT^H Abar (per row)      AdjFftBody, kept per iteration
+// (with iteration i's conjugated taps, AdjTapsBody) and at the end, for every
+// sample j at once,
+//   xbar[j] = sum_i fold_i^H fbar_i              AdjGatherBody.
+// The last step is the fold read backwards as a gather: sample j was
+// staged by iteration i at u = a_inv (j - c) mod n, i.e. at every lag
+// lambda and residue rho with lambda B + rho = u - q + p2 (mod n) whose
+// lambda lies in some row's window, so each output is written once, with
+// no atomics and no accumulation buffer.  Only lam entries at found keys
+// are ever read, so only those are cleared.
 
 // e^{-2 pi i (q + off(row) B) k1 / n}: the estimate's factor for one entry
 // (src/iterate.cc:147-166).
@@ -1556,18 +1557,6 @@ __device__ __forceinline__ double2 EstFactor(int64_t q, int32_t B, int row, int6
   const int bit = (row - 1) / 2;
   const double2 fac2 = Conj(CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n));
   return Cmul(fac, (row & 1) ? fac2 : Conj(fac2));
-}
-
-// lam[keys[i]] += g[i]; keys outside [0, n) are not frequencies of the
-// signal and carry nothing.  Threads tid < m of a grid of gsize.
-__device__ __forceinline__ void AdjSeedBody(double2 *lam, const int64_t *keys, const double2 *g,
-                                            int64_t m, int64_t n, int64_t tid, int64_t gsize) {
-  for (int64_t i = tid; i < m; i += gsize) {
-    const int64_t k = keys[i];
-    if (k < 0 || k >= n) continue;
-    atomicAdd(&lam[k].x, g[i].x);
-    atomicAdd(&lam[k].y, g[i].y);
-  }
 }
 
 // Abar[t][row][b] = conj(d coef / d A[t][row][b]) lam[k0] for every bin that
@@ -1641,9 +1630,10 @@ __device__ __forceinline__ void AdjTermsBody(const Ctx &c, const IterParams &ip,
 
 // fbar[t][row] = conj(rp) DFT^H Abar[t][row] = conj(DFT(conj(Abar)) rp): the
 // forward bin FFT on conj(Abar), output conjugated (the Bluestein matrix is
-// symmetric).  Block blk = trial * rows_cap + row.
+// symmetric).  Block blk = trial * rows_cap + row; fbar holds
+// trials_cap x rows_cap x B_cap entries.
 __device__ __forceinline__ void AdjFftBody(const Ctx &c, const IterParams &ip, int32_t trials,
-                                           int blk, double2 *lds) {
+                                           double2 *fbar, int blk, double2 *lds) {
   const int32_t trial = blk / c.rows_cap;
   const int32_t row = blk - trial * c.rows_cap;
   if (trial >= trials || row >= ip.rows) return;
@@ -1659,122 +1649,215 @@ __device__ __forceinline__ void AdjFftBody(const Ctx &c, const IterParams &ip, i
   else
     BinFftLarge<true>(c, ip, c.A + off, rp, a);
   __syncthreads();
-  for (int k = threadIdx.x; k < B; k += kFftThreads) c.fold[off + k] = Conj(a[small ? k : Pad(k)]);
+  for (int k = threadIdx.x; k < B; k += kFftThreads) fbar[off + k] = Conj(a[small ? k : Pad(k)]);
 }
 
-// xbar[(a u + c) mod n] += sum over rows of conj(tap) fbar[row][m(rho)], for
-// every staged (lag, residue) of tile `tile` of trial `trial`: the fold of
-// k_bin_time read backwards.  LDS: (f + rows) R double2.
-__device__ __forceinline__ void AdjFoldBody(const Ctx &c, const IterParams &ip, int32_t trial,
-                                            int32_t tile, double2 *xbar, double2 *lds) {
-  const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
-  const DevTf tf = ip.tf;
-  const int64_t n = c.n, mask = c.mask;
-  const int32_t R = ip.R;
-  const int32_t rho0 = tile * R;
-  const int32_t nrho = min(R, B - rho0);
-  const int logR = R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : 3;
-  const Lags L(bits, f);
-  const int64_t q = ip.q[trial];
-  double2 *T = lds;            // [f][R] taps
-  double2 *FA = T + f * R;     // [rows][R] fbar at the tile's residues
-  const int32_t h = (B - 1) / 2;
-  const double width = ip.width, sigma_f = ip.sigma_f;
+// taps[k B + rho] = conj(tap(k B + rho - p2)), k < f, rho < B: the taps of
+// k_bin_time, conjugated.  Threads tid of a grid of gsize.
+__device__ __forceinline__ void AdjTapsBody(const IterParams &ip, int64_t n, int64_t mask,
+                                            double2 *taps, int64_t tid, int64_t gsize) {
+  const int32_t B = ip.B;
   const double delta = -0.5 / double(B);
-  for (int idx = threadIdx.x; idx < (f << logR); idx += blockDim.x) {
-    const int k = idx >> logR, ri = idx & (R - 1);
-    if (ri < nrho) {
-      const int64_t t = int64_t(k) * B + rho0 + ri - p2;
-      const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
-      const int64_t r = ModN(tf.b * t, n, mask);
-      const double d = delta * double(t);
-      const double turns = double(r) / double(n) + (d - floor(d));
-      T[idx] = Scale(CisTurns(turns), w);
-    }
+  for (int64_t i = tid; i < int64_t(ip.f) * B; i += gsize) {
+    const int64_t t = i - ip.p2;  // i = k B + rho
+    const double w = (t == 0) ? ip.width : SampleInTime(t, ip.width, ip.sigma_f);
+    const int64_t r = ModN(ip.tf.b * t, n, mask);
+    const double d = delta * double(t);
+    const double turns = double(r) / double(n) + (d - floor(d));
+    taps[i] = Conj(Scale(CisTurns(turns), w));
   }
-  const double2 *fb = c.fold + int64_t(trial) * c.rows_cap * c.B_cap;
-  for (int idx = threadIdx.x; idx < (rows << logR); idx += blockDim.x) {
-    const int row = idx >> logR, ri = idx & (R - 1);
-    if (ri < nrho) {
-      const int rho = rho0 + ri;
-      const int m = rho >= h ? rho - h : rho - h + B;
-      FA[idx] = fb[int64_t(row) * c.B_cap + m];
+}
+
+// Sum over the rows whose window [off(row), off(row) + f) holds lag lam of
+// taps[k B + rho] fbar[row][m(rho)], k = lam - off(row).  Row 0 has offset
+// 0, row 2b+1 offset 2^b, row 2b+2 offset -2^b (b < bits).  tp = taps + rho,
+// fm = fbar + m(rho).
+__device__ __forceinline__ double2 AdjLagTerms(int32_t f, int32_t bits, int64_t B, int64_t lam,
+                                               const double2 *tp, const double2 *fm,
+                                               int64_t rstride) {
+  double2 acc = make_double2(0.0, 0.0);
+  if (lam >= 0) {
+    if (lam < f) acc = Add(acc, Cmul(tp[lam * B], fm[0]));
+    if (lam >= 1) {  // offsets 2^b in (lam - f, lam]
+      for (int b = 63 - __clzll(lam); b >= 0 && (int64_t(1) << b) > lam - f; --b)
+        if (b < bits)
+          acc = Add(acc, Cmul(tp[(lam - (int64_t(1) << b)) * B], fm[(2 * b + 1) * rstride]));
     }
+    for (int b = 0; b < bits && (int64_t(1) << b) < f - lam; ++b)  // offsets -2^b
+      acc = Add(acc, Cmul(tp[(lam + (int64_t(1) << b)) * B], fm[(2 * b + 2) * rstride]));
+  } else {  // offsets -2^b with 2^b in [-lam, f - lam)
+    int b = 63 - __clzll(-lam);
+    if ((int64_t(1) << b) < -lam) ++b;
+    for (; b < bits && (int64_t(1) << b) < f - lam; ++b)
+      acc = Add(acc, Cmul(tp[(lam + (int64_t(1) << b)) * B], fm[(2 * b + 2) * rstride]));
   }
-  __syncthreads();
-  const int64_t ubase = q - p2 + rho0;
-  for (int idx = threadIdx.x; idx < (L.nlam << logR); idx += blockDim.x) {
-    const int li = idx >> logR, ri = idx & (R - 1);
-    if (ri >= nrho) continue;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int row = 0; row < rows; ++row) {
-      const int k = li - L.RowStart(row);
-      if (k >= 0 && k < f) acc = Add(acc, Cmul(Conj(T[(k << logR) + ri]), FA[(row << logR) + ri]));
+  return acc;
+}
+
+// out[j] = sum over the recorded iterations of fold^H fbar at sample j, for
+// kGatherJ samples per thread (iteration parameters loaded once for them).
+constexpr int kGatherJ = 4;
+
+// AdjLagTerms in 32-bit integers (every lag and offset of an n <= 2^30
+// signal fits).
+__device__ __forceinline__ double2 AdjLagTerms32(int32_t f, int32_t bits, int32_t B, int32_t lam,
+                                                 const double2 *tp, const double2 *fm,
+                                                 int32_t rstride) {
+  double2 acc = make_double2(0.0, 0.0);
+  if (lam >= 0) {
+    if (lam < f) acc = Add(acc, Cmul(tp[lam * B], fm[0]));
+    if (lam >= 1) {  // offsets 2^b in (lam - f, lam]
+      for (int b = 31 - __clz(lam); b >= 0 && (1 << b) > lam - f; --b)
+        if (b < bits) acc = Add(acc, Cmul(tp[(lam - (1 << b)) * B], fm[(2 * b + 1) * rstride]));
     }
-    const int64_t u = ubase + L.Lambda(li) * B + ri;
-    const int64_t j = ModN(tf.a * u + tf.c, n, mask);
-    atomicAdd(&xbar[j].x, acc.x);
-    atomicAdd(&xbar[j].y, acc.y);
+    for (int b = 0; b < bits && (1 << b) < f - lam; ++b)  // offsets -2^b
+      acc = Add(acc, Cmul(tp[(lam + (1 << b)) * B], fm[(2 * b + 2) * rstride]));
+  } else {  // offsets -2^b with 2^b in [-lam, f - lam)
+    int b = 31 - __clz(-lam);
+    if ((1 << b) < -lam) ++b;
+    for (; b < bits && (1 << b) < f - lam; ++b)
+      acc = Add(acc, Cmul(tp[(lam + (1 << b)) * B], fm[(2 * b + 2) * rstride]));
+  }
+  return acc;
+}
+
+// AdjGatherBody for n = 2^k <= 2^30: the products mod n are the low bits
+// of 32-bit products, and every lag range fits int32.
+template <typename Tout>
+__device__ __forceinline__ void AdjGatherBody32(const Ctx &c, const AdjSig &g, const AdjStep *steps,
+                                                int32_t count, int32_t si, Tout *out, int64_t tid,
+                                                int64_t gsize) {
+  const int32_t n = int32_t(c.n);
+  const uint32_t m32 = uint32_t(c.n - 1);
+  const int32_t rstride = c.B_cap;
+  const int64_t tstride = int64_t(c.rows_cap) * c.B_cap;
+  const int64_t istride = int64_t(c.trials_cap) * tstride;
+  for (int64_t j0 = tid; j0 < n; j0 += gsize * kGatherJ) {
+    double2 acc[kGatherJ];
+#pragma unroll
+    for (int q = 0; q < kGatherJ; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int32_t s = 0; s < g.iters; ++s) {
+      const AdjStep &st = steps[int64_t(s) * count + si];
+      const IterParams &ip = st.ip;
+      const int32_t B = ip.B, f = ip.f, bits = ip.bits;
+      const int32_t h = (B - 1) / 2;
+      const double invB = 1.0 / double(B);
+      const int32_t half = 1 << (bits - 1);
+      const int32_t lo = -half * B, hi = (half + f) * B;  // lag range times B
+      const double2 *taps = g.taps + int64_t(st.it) * g.taps_stride;
+      const uint32_t a_inv = uint32_t(ip.tf.a_inv), cc = uint32_t(ip.tf.c);
+      for (int32_t t = 0; t < st.trials; ++t) {
+        const uint32_t base = uint32_t(ip.q[t] - ip.p2) & m32;
+        const double2 *fb = g.fbar + int64_t(st.it) * istride + t * tstride;
+#pragma unroll
+        for (int q = 0; q < kGatherJ; ++q) {
+          const int64_t j = j0 + q * gsize;
+          if (j >= n) continue;
+          const uint32_t u = (a_inv * ((uint32_t(j) - cc) & m32)) & m32;
+          const int32_t v = int32_t((u - base) & m32);
+          for (int32_t w = v >= hi ? v - n : (v - n < lo ? v : v - n); w < hi; w += n) {
+            // lam B + rho = w, w = v mod n, lo <= w < hi
+            int32_t lam = int32_t(floor(double(w) * invB));
+            int32_t rho = w - lam * B;
+            if (rho < 0) {
+              --lam;
+              rho += B;
+            } else if (rho >= B) {
+              ++lam;
+              rho -= B;
+            }
+            const int32_t m = rho >= h ? rho - h : rho - h + B;
+            acc[q] = Add(acc[q], AdjLagTerms32(f, bits, B, lam, taps + rho, fb + m, rstride));
+            if (w > hi - n) break;  // w + n >= hi
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGatherJ; ++q) {
+      const int64_t j = j0 + q * gsize;
+      if (j < n) {
+        out[j].x = acc[q].x;
+        out[j].y = acc[q].y;
+      }
+    }
   }
 }
 
 template <typename Tout>
-__device__ __forceinline__ void AdjOutBody(const double2 *xbar, Tout *out, int64_t n, int64_t tid,
-                                           int64_t gsize) {
-  for (int64_t i = tid; i < n; i += gsize) {
-    const double2 v = xbar[i];
-    out[i].x = v.x;
-    out[i].y = v.y;
+__device__ __forceinline__ void AdjGatherBody(const Ctx &c, const AdjSig &g, const AdjStep *steps,
+                                              int32_t count, int32_t si, Tout *out, int64_t tid,
+                                              int64_t gsize) {
+  const int64_t n = c.n, mask = c.mask;
+  const int64_t rstride = c.B_cap;
+  const int64_t tstride = int64_t(c.rows_cap) * c.B_cap;
+  const int64_t istride = int64_t(c.trials_cap) * tstride;
+  for (int64_t j0 = tid; j0 < n; j0 += gsize * kGatherJ) {
+    double2 acc[kGatherJ];
+#pragma unroll
+    for (int q = 0; q < kGatherJ; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int32_t s = 0; s < g.iters; ++s) {
+      const AdjStep &st = steps[int64_t(s) * count + si];
+      const IterParams &ip = st.ip;
+      const int64_t B = ip.B;
+      const int32_t f = ip.f, bits = ip.bits;
+      const int32_t h = (ip.B - 1) / 2;
+      const double invB = 1.0 / double(B);
+      const int64_t half = int64_t(1) << (bits - 1);
+      const int64_t lo = -half * B, hi = (half + f) * B;  // lag range times B
+      const double2 *taps = g.taps + int64_t(st.it) * g.taps_stride;
+      const int64_t a_inv = ip.tf.a_inv, cc = ip.tf.c;
+      for (int32_t t = 0; t < st.trials; ++t) {
+        const int64_t base = ModN(ip.q[t] - ip.p2, n, mask);
+        const double2 *fb = g.fbar + int64_t(st.it) * istride + t * tstride;
+#pragma unroll
+        for (int q = 0; q < kGatherJ; ++q) {
+          const int64_t j = j0 + q * gsize;
+          if (j >= n) continue;
+          const int64_t u = ModN(a_inv * ModN(j - cc, n, mask), n, mask);
+          const int64_t v = ModN(u - base, n, mask);
+          for (int64_t w = v >= hi ? v - n : v - n + (v - n < lo ? n : 0); w < hi; w += n) {
+            // lam B + rho = w, w = v mod n, lo <= w < hi
+            int64_t lam = int64_t(floor(double(w) * invB));
+            int64_t rho = w - lam * B;
+            if (rho < 0) {
+              --lam;
+              rho += B;
+            } else if (rho >= B) {
+              ++lam;
+              rho -= B;
+            }
+            const int64_t m = rho >= h ? rho - h : rho - h + B;
+            acc[q] = Add(acc[q], AdjLagTerms(f, bits, B, lam, taps + rho, fb + m, rstride));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGatherJ; ++q) {
+      const int64_t j = j0 + q * gsize;
+      if (j < n) {
+        out[j].x = acc[q].x;
+        out[j].y = acc[q].y;
+      }
+    }
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-    k_adj_seed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m, int64_t n) {
-  AdjSeedBody(lam, keys, g, m, n, int64_t(blockIdx.x) * blockDim.x + threadIdx.x,
-              int64_t(gridDim.x) * blockDim.x);
+// ---- Kernels.  Every signal of a batch (or the one signal of a solver)
+// at once: map[blockIdx.x] = {signal, local block}, or with map null signal
+// 0 and the block index itself.  sig[i] holds signal i's buffers, steps
+// (count entries per reverse step) its iteration parameters.
+
+__device__ __forceinline__ int2 AdjBlock(const int2 *map) {
+  return map ? map[blockIdx.x] : make_int2(0, int(blockIdx.x));
 }
 
-__global__ void __launch_bounds__(kThreads)
-    k_adj_bins(Ctx c, IterParams ip, int32_t trials, const double2 *lam, const int64_t *rec_row) {
-  AdjBinsBody(c, ip, trials, lam, rec_row, blockIdx.x, gridDim.x);
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_adj_terms(Ctx c, IterParams ip, int32_t trials, double2 *lam, int32_t count_before,
-                const int64_t *rec_row) {
-  AdjTermsBody(c, ip, trials, lam, count_before, rec_row,
-               int64_t(blockIdx.x) * blockDim.x + threadIdx.x, int64_t(gridDim.x) * blockDim.x);
-}
-
-__global__ void __launch_bounds__(kFftThreads) k_adj_fft(Ctx c, IterParams ip, int32_t trials) {
-  HIP_DYNAMIC_SHARED(double2, lds)
-  AdjFftBody(c, ip, trials, blockIdx.x, lds);
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_adj_fold(Ctx c, IterParams ip, int32_t trial, double2 *xbar) {
-  HIP_DYNAMIC_SHARED(double2, lds)
-  if (int32_t(blockIdx.x) >= (ip.B + ip.R - 1) / ip.R) return;
-  AdjFoldBody(c, ip, trial, blockIdx.x, xbar, lds);
-}
-
-template <typename Tout>
-__global__ void __launch_bounds__(kThreads)
-    k_adj_out(const double2 *xbar, Tout *out, int64_t n) {
-  AdjOutBody(xbar, out, n, int64_t(blockIdx.x) * blockDim.x + threadIdx.x,
-             int64_t(gridDim.x) * blockDim.x);
-}
-
-// ---- The batched adjoint: one launch per stage over every signal of a
-// batch, reverse step j taking each signal's iteration iters_i - 1 - j.
-// sig[i] holds signal i's buffers; steps (count entries per step) its
-// iteration parameters, built on the device from the recorded log.
-
-// Per signal: IterParams of every reverse step, as NextIteration derives
+// Per signal: the IterParams of every reverse step, as NextIteration derives
 // them in a replay (bins from the log, the transform and q's drawn).
 __global__ void __launch_bounds__(kThreads)
-    k_adj_steps_batch(const Ctx *ctxs, const AdjSig *sig, int32_t count, int32_t max_steps,
-                      AdjStep *steps) {
+    k_adj_steps(const Ctx *ctxs, const AdjSig *sig, int32_t count, int32_t max_steps,
+                AdjStep *steps) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= int64_t(count) * max_steps) return;
   const int32_t j = int32_t(i / count), s = int32_t(i - int64_t(j) * count);
@@ -1798,52 +1881,50 @@ __global__ void __launch_bounds__(kThreads)
   st.count_before = it > 0 ? c.log[c.log_cap + it - 1] : 0;
 }
 
-// Zeroes lam and xbar of every signal (blocks per signal: `per`).
+// lam = 0 at every found key (the only entries read), `per` blocks a signal.
 __global__ void __launch_bounds__(kThreads)
-    k_adj_zero_batch(const AdjSig *sig, int32_t per) {
+    k_adj_clear(const Ctx *ctxs, const AdjSig *sig, int32_t per) {
+  const int32_t si = blockIdx.x / per;
+  const Ctx &c = ctxs[si];
+  const int32_t cnt = min(c.st->count, c.cap);
+  double2 *lam = sig[si].lam;
+  for (int32_t i = (blockIdx.x % per) * blockDim.x + threadIdx.x; i < cnt; i += per * blockDim.x)
+    lam[c.keys[i]] = make_double2(0.0, 0.0);
+}
+
+// lam[keys[i]] += g[i]; keys outside [0, n) are not frequencies of the
+// signal and carry nothing.
+__global__ void __launch_bounds__(kThreads) k_adj_seed(const AdjSig *sig, int32_t per) {
   const AdjSig &g = sig[blockIdx.x / per];
-  const int64_t tid = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x;
-  const int64_t gsize = int64_t(per) * blockDim.x;
-  for (int64_t k = tid; k < g.n; k += gsize) {
-    g.lam[k] = make_double2(0.0, 0.0);
-    g.xbar[k] = make_double2(0.0, 0.0);
+  for (int64_t i = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x; i < g.m;
+       i += int64_t(per) * blockDim.x) {
+    const int64_t k = g.keys[i];
+    if (k < 0 || k >= g.n) continue;
+    atomicAdd(&g.lam[k].x, g.grad[i].x);
+    atomicAdd(&g.lam[k].y, g.grad[i].y);
   }
 }
 
 __global__ void __launch_bounds__(kThreads)
-    k_adj_seed_batch(const AdjSig *sig, int32_t per) {
-  const AdjSig &g = sig[blockIdx.x / per];
-  AdjSeedBody(g.lam, g.keys, g.grad, g.m, g.n, int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x,
-              int64_t(per) * blockDim.x);
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_adj_bins_batch(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map,
-                     const int32_t *nblk) {
-  const int2 m = map[blockIdx.x];
+    k_adj_bins(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map,
+               const int32_t *nblk) {
+  const int2 m = AdjBlock(map);
   const AdjStep &st = steps[m.x];
   if (!st.active) return;
   const Ctx &c = ctxs[m.x];
-  AdjBinsBody(c, st.ip, st.trials, sig[m.x].lam, sig[m.x].rec + int64_t(st.it) * c.B_cap, m.y,
-              nblk[m.x]);
+  const AdjSig &g = sig[m.x];
+  const int nb = map ? nblk[m.x] : int(gridDim.x);
+  AdjBinsBody(c, st.ip, st.trials, g.lam, g.rec + int64_t(st.it) * c.B_cap, m.y, nb);
+  AdjTapsBody(st.ip, c.n, c.mask, g.taps + int64_t(st.it) * g.taps_stride,
+              int64_t(m.y) * blockDim.x + threadIdx.x, int64_t(nb) * blockDim.x);
 }
 
+// Blocks below kPrepBlocks of each signal update lam (the terms), the
+// others transform one (trial, row) each.
 __global__ void __launch_bounds__(kFftThreads)
-    k_adj_fft_batch(const Ctx *ctxs, const AdjStep *steps, const int2 *map) {
+    k_adj_fft_terms(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map) {
   HIP_DYNAMIC_SHARED(double2, lds)
-  const int2 m = map[blockIdx.x];
-  const AdjStep &st = steps[m.x];
-  if (!st.active) return;
-  AdjFftBody(ctxs[m.x], st.ip, st.trials, m.y, lds);
-}
-
-// The map of k_bin_time: blocks below kPrepBlocks run the terms update,
-// the others the fold of tile (block - kPrepBlocks) mod tiles_cap of trial
-// (block - kPrepBlocks) / tiles_cap.
-__global__ void __launch_bounds__(kThreads)
-    k_adj_fold_batch(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map) {
-  HIP_DYNAMIC_SHARED(double2, lds)
-  const int2 m = map[blockIdx.x];
+  const int2 m = AdjBlock(map);
   const AdjStep &st = steps[m.x];
   if (!st.active) return;
   const Ctx &c = ctxs[m.x];
@@ -1853,17 +1934,24 @@ __global__ void __launch_bounds__(kThreads)
                  int64_t(m.y) * blockDim.x + threadIdx.x, int64_t(kPrepBlocks) * blockDim.x);
     return;
   }
-  const int32_t b = m.y - kPrepBlocks;
-  const int32_t trial = b / g.tiles_cap, tile = b - trial * g.tiles_cap;
-  if (trial >= st.trials || tile >= (st.ip.B + st.ip.R - 1) / st.ip.R) return;
-  AdjFoldBody(c, st.ip, trial, tile, g.xbar, lds);
+  const int64_t istride = int64_t(c.trials_cap) * c.rows_cap * c.B_cap;
+  AdjFftBody(c, st.ip, st.trials, g.fbar + int64_t(st.it) * istride, m.y - kPrepBlocks, lds);
 }
 
 template <typename Tout>
-__global__ void __launch_bounds__(kThreads) k_adj_out_batch(const AdjSig *sig, int32_t per) {
-  const AdjSig &g = sig[blockIdx.x / per];
-  AdjOutBody(g.xbar, static_cast<Tout *>(g.out), g.n,
-             int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x, int64_t(per) * blockDim.x);
+__global__ void __launch_bounds__(kThreads)
+    k_adj_gather(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, int32_t count,
+                 int32_t per) {
+  const int32_t si = blockIdx.x / per;
+  const AdjSig &g = sig[si];
+  const Ctx &c = ctxs[si];
+  const int64_t tid = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x;
+  if (c.mask >= 0 && c.n <= (int64_t(1) << 30))
+    AdjGatherBody32(c, g, steps, count, si, static_cast<Tout *>(g.out), tid,
+                    int64_t(per) * blockDim.x);
+  else
+    AdjGatherBody(c, g, steps, count, si, static_cast<Tout *>(g.out), tid,
+                  int64_t(per) * blockDim.x);
 }
 
 int32_t Blocks(int64_t work) {
@@ -2006,63 +2094,27 @@ hipError_t LaunchBatchResults(const Ctx *ctxs, int32_t count, BatchResultDev *ou
   return hipGetLastError();
 }
 
-hipError_t LaunchAdjSeed(double2 *lam, const int64_t *keys, const double2 *g, int64_t m,
-                         int64_t n, hipStream_t s) {
-  if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_adj_seed, dim3(Blocks(m)), dim3(kThreads), 0, s, lam, keys, g, m, n);
-  return hipGetLastError();
-}
-
-hipError_t LaunchAdjIteration(const Ctx &c, const IterParams &ip, int32_t trials,
-                              int32_t count_before, const int64_t *rec_row, double2 *lam,
-                              double2 *xbar, int32_t ident_blocks, size_t lds_fft,
-                              size_t lds_fold, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_bins, dim3(ident_blocks), dim3(kThreads), 0, s, c, ip, trials, lam,
-                     rec_row);
-  if (count_before > 0)
-    hipLaunchKernelGGL(k_adj_terms, dim3(Blocks(count_before)), dim3(kThreads), 0, s, c, ip,
-                       trials, lam, count_before, rec_row);
-  hipLaunchKernelGGL(k_adj_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds_fft, s,
-                     c, ip, trials);
-  const int32_t tiles = (ip.B + ip.R - 1) / ip.R;
-  for (int32_t t = 0; t < trials; ++t)
-    hipLaunchKernelGGL(k_adj_fold, dim3(tiles), dim3(kThreads), lds_fold, s, c, ip, t, xbar);
-  return hipGetLastError();
-}
-
-hipError_t LaunchAdjBatch(const AdjBatchLaunch &b, hipStream_t s) {
+hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s) {
   const int64_t nsteps = int64_t(b.count) * b.max_steps;
-  hipLaunchKernelGGL(k_adj_steps_batch, dim3(Blocks(nsteps)), dim3(kThreads), 0, s, b.ctxs, b.sig,
+  hipLaunchKernelGGL(k_adj_steps, dim3(Blocks(nsteps)), dim3(kThreads), 0, s, b.ctxs, b.sig,
                      b.count, b.max_steps, b.steps);
-  hipLaunchKernelGGL(k_adj_zero_batch, dim3(b.count * b.per_zero), dim3(kThreads), 0, s, b.sig,
-                     b.per_zero);
-  hipLaunchKernelGGL(k_adj_seed_batch, dim3(b.count * b.per_seed), dim3(kThreads), 0, s, b.sig,
-                     b.per_seed);
+  hipLaunchKernelGGL(k_adj_clear, dim3(b.count * b.per_small), dim3(kThreads), 0, s, b.ctxs, b.sig,
+                     b.per_small);
+  hipLaunchKernelGGL(k_adj_seed, dim3(b.count * b.per_small), dim3(kThreads), 0, s, b.sig,
+                     b.per_small);
   for (int32_t j = 0; j < b.max_steps; ++j) {
     const AdjStep *st = b.steps + int64_t(j) * b.count;
-    hipLaunchKernelGGL(k_adj_bins_batch, dim3(b.nb_ident), dim3(kThreads), 0, s, b.ctxs, b.sig, st,
-                       b.map_ident, b.nblk_ident);
-    hipLaunchKernelGGL(k_adj_fft_batch, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
-                       st, b.map_fft);
-    hipLaunchKernelGGL(k_adj_fold_batch, dim3(b.nb_time), dim3(kThreads), b.lds_fold, s, b.ctxs,
-                       b.sig, st, b.map_time);
+    hipLaunchKernelGGL(k_adj_bins, dim3(b.nb_bins), dim3(kThreads), 0, s, b.ctxs, b.sig, st,
+                       b.map_bins, b.nblk_bins);
+    hipLaunchKernelGGL(k_adj_fft_terms, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
+                       b.sig, st, b.map_fft);
   }
   if (b.mixed)
-    hipLaunchKernelGGL(k_adj_out_batch<float2>, dim3(b.count * b.per_zero), dim3(kThreads), 0, s,
-                       b.sig, b.per_zero);
+    hipLaunchKernelGGL(k_adj_gather<float2>, dim3(b.count * b.per_gather), dim3(kThreads), 0, s,
+                       b.ctxs, b.sig, b.steps, b.count, b.per_gather);
   else
-    hipLaunchKernelGGL(k_adj_out_batch<double2>, dim3(b.count * b.per_zero), dim3(kThreads), 0, s,
-                       b.sig, b.per_zero);
-  return hipGetLastError();
-}
-
-hipError_t LaunchAdjOut(const double2 *xbar, void *out, int64_t n, int mixed, hipStream_t s) {
-  if (mixed)
-    hipLaunchKernelGGL(k_adj_out<float2>, dim3(Blocks(n)), dim3(kThreads), 0, s, xbar,
-                       static_cast<float2 *>(out), n);
-  else
-    hipLaunchKernelGGL(k_adj_out<double2>, dim3(Blocks(n)), dim3(kThreads), 0, s, xbar,
-                       static_cast<double2 *>(out), n);
+    hipLaunchKernelGGL(k_adj_gather<double2>, dim3(b.count * b.per_gather), dim3(kThreads), 0, s,
+                       b.ctxs, b.sig, b.steps, b.count, b.per_gather);
   return hipGetLastError();
 }
 

@@ -76,6 +76,8 @@ inline double __shfl_xor(double v, int mask, int = 64) {
 }
 inline uint64_t __ballot(int pred) { return emu::Ballot(pred); }
 inline int __any(int pred) { return emu::Ballot(pred) != 0; }
+inline int __clz(int v) { return v ? __builtin_clz((unsigned)v) : 32; }
+inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
 inline unsigned __brev(unsigned v) {
   unsigned r = 0;
   for (int i = 0; i < 32; ++i) r |= ((v >> i) & 1u) << (31 - i);

@@ -80,12 +80,14 @@ def test_emu_batch_matches_single_and_oracle(emu_env):
     assert all(r["same_as_single"]) and all(r["same_as_oracle"])
 
 
-@pytest.mark.parametrize("n,k,trials", [(1 << 14, 20, 1), (1 << 14, 20, 3), (1 << 16, 300, 1)])
+@pytest.mark.parametrize("n,k,trials", [(1 << 14, 20, 1), (1 << 14, 20, 3), (1 << 16, 300, 1),
+                                         (10007, 10, 1)])
 def test_emu_replay_and_adjoint(emu_env, n, k, trials):
     """A recorded solve replayed on its own input gives the solve's modes;
     the replayed map is linear; the adjoint kernels satisfy
     <F d, g> = <d, F^H g>.  k = 300 takes B = 601 through the register
-    Stockham bin FFT."""
+    Stockham bin FFT; the prime n takes the 64-bit index path of the
+    gather."""
     r = run_case(emu_env, kind="adjoint", n=n, k=k, sigma=1e-6, seed=7, trials=trials)
     assert r["n_found"] >= k and r["same_keys"] and r["replay_err"] == 0.0
     assert r["linear_err"] < 1e-13 and r["dot_rel"] < 1e-12
