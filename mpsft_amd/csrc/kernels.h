@@ -75,7 +75,8 @@ struct AdjSig {            // one signal (device memory, per call)
 };
 struct AdjStep {           // one signal at one reverse step (built on the device)
   IterParams ip;
-  int32_t active, trials, count_before, it;
+  int32_t active;          // the signal has this iteration and it identified something
+  int32_t trials, count_before, it;
 };
 struct AdjLaunch {
   const Ctx *ctxs;

@@ -1699,25 +1699,41 @@ __device__ __forceinline__ double2 AdjLagTerms(int32_t f, int32_t bits, int64_t 
 constexpr int kGatherJ = 4;
 
 // AdjLagTerms in 32-bit integers (every lag and offset of an n <= 2^30
-// signal fits).
-__device__ __forceinline__ double2 AdjLagTerms32(int32_t f, int32_t bits, int32_t B, int32_t lam,
-                                                 const double2 *tp, const double2 *fm,
+// signal fits), with no loop whose length depends on the lag.  With K the
+// smallest power with 2^K >= f (at most bits), segment 0 =
+// [-2^(K-1), 2^(K-1) + f) holds the windows of row 0 and of the rows with
+// offsets +-2^b, b < K, and meets those of offsets +-2^K: a lag there is
+// checked against row 0 and offsets +-2^b for b <= K (a loop the whole
+// wave runs alike).  A lag outside segment 0 lies in at most one window:
+// offset 2^b with b = floor(log2 lam) for lam > 0, offset -2^b with
+// b = ceil(log2 -lam) for lam < 0.
+__device__ __forceinline__ double2 AdjLagTerms32(int32_t f, int32_t bits, int32_t K, int32_t B,
+                                                 int32_t lam, const double2 *tp, const double2 *fm,
                                                  int32_t rstride) {
   double2 acc = make_double2(0.0, 0.0);
-  if (lam >= 0) {
-    if (lam < f) acc = Add(acc, Cmul(tp[lam * B], fm[0]));
-    if (lam >= 1) {  // offsets 2^b in (lam - f, lam]
-      for (int b = 31 - __clz(lam); b >= 0 && (1 << b) > lam - f; --b)
-        if (b < bits) acc = Add(acc, Cmul(tp[(lam - (1 << b)) * B], fm[(2 * b + 1) * rstride]));
+  const int32_t s0 = K ? (1 << (K - 1)) : 0;
+  if (lam >= -s0 && lam < s0 + f) {
+    if (lam >= 0 && lam < f) acc = Cmul(tp[lam * B], fm[0]);
+    for (int b = 0; b <= K && b < bits; ++b) {
+      const int32_t kp = lam - (1 << b), kn = lam + (1 << b);
+      if (kp >= 0 && kp < f) acc = Add(acc, Cmul(tp[kp * B], fm[(2 * b + 1) * rstride]));
+      if (kn >= 0 && kn < f) acc = Add(acc, Cmul(tp[kn * B], fm[(2 * b + 2) * rstride]));
     }
-    for (int b = 0; b < bits && (1 << b) < f - lam; ++b)  // offsets -2^b
-      acc = Add(acc, Cmul(tp[(lam + (1 << b)) * B], fm[(2 * b + 2) * rstride]));
-  } else {  // offsets -2^b with 2^b in [-lam, f - lam)
-    int b = 31 - __clz(-lam);
-    if ((1 << b) < -lam) ++b;
-    for (; b < bits && (1 << b) < f - lam; ++b)
-      acc = Add(acc, Cmul(tp[(lam + (1 << b)) * B], fm[(2 * b + 2) * rstride]));
+    return acc;
   }
+  int32_t b, k, row;
+  if (lam > 0) {
+    b = 31 - __clz(lam);
+    k = lam - (1 << b);
+    row = 2 * b + 1;
+  } else {
+    const int32_t nl = -lam;
+    b = 31 - __clz(nl);
+    if ((1 << b) < nl) ++b;
+    k = lam + (b < 31 ? (1 << b) : 0);
+    row = 2 * b + 2;
+  }
+  if (b < bits && k >= 0 && k < f) acc = Cmul(tp[k * B], fm[row * rstride]);
   return acc;
 }
 
@@ -1738,12 +1754,15 @@ __device__ __forceinline__ void AdjGatherBody32(const Ctx &c, const AdjSig &g, c
     for (int q = 0; q < kGatherJ; ++q) acc[q] = make_double2(0.0, 0.0);
     for (int32_t s = 0; s < g.iters; ++s) {
       const AdjStep &st = steps[int64_t(s) * count + si];
+      if (!st.active) continue;
       const IterParams &ip = st.ip;
       const int32_t B = ip.B, f = ip.f, bits = ip.bits;
       const int32_t h = (B - 1) / 2;
       const double invB = 1.0 / double(B);
       const int32_t half = 1 << (bits - 1);
       const int32_t lo = -half * B, hi = (half + f) * B;  // lag range times B
+      int32_t K = 0;  // as Lags: the smallest power with 2^K >= f, at most bits
+      while (K < bits && (1 << K) < f) ++K;
       const double2 *taps = g.taps + int64_t(st.it) * g.taps_stride;
       const uint32_t a_inv = uint32_t(ip.tf.a_inv), cc = uint32_t(ip.tf.c);
       for (int32_t t = 0; t < st.trials; ++t) {
@@ -1767,7 +1786,7 @@ __device__ __forceinline__ void AdjGatherBody32(const Ctx &c, const AdjSig &g, c
               rho -= B;
             }
             const int32_t m = rho >= h ? rho - h : rho - h + B;
-            acc[q] = Add(acc[q], AdjLagTerms32(f, bits, B, lam, taps + rho, fb + m, rstride));
+            acc[q] = Add(acc[q], AdjLagTerms32(f, bits, K, B, lam, taps + rho, fb + m, rstride));
             if (w > hi - n) break;  // w + n >= hi
           }
         }
@@ -1798,6 +1817,7 @@ __device__ __forceinline__ void AdjGatherBody(const Ctx &c, const AdjSig &g, con
     for (int q = 0; q < kGatherJ; ++q) acc[q] = make_double2(0.0, 0.0);
     for (int32_t s = 0; s < g.iters; ++s) {
       const AdjStep &st = steps[int64_t(s) * count + si];
+      if (!st.active) continue;
       const IterParams &ip = st.ip;
       const int64_t B = ip.B;
       const int32_t f = ip.f, bits = ip.bits;
@@ -1879,6 +1899,12 @@ __global__ void __launch_bounds__(kThreads)
   st.ip.it = it;
   st.ip.done = 0;
   st.count_before = it > 0 ? c.log[c.log_cap + it - 1] : 0;
+  // An iteration whose bins identified nothing has Abar = 0: it changes
+  // neither lam nor xbar, and every stage skips it.
+  const int64_t *rec_row = sig[s].rec + int64_t(it) * c.B_cap;
+  int32_t live = 0;
+  for (int32_t b = 0; b < B && !live; ++b) live = rec_row[b] >= 0;
+  st.active = live;
 }
 
 // lam = 0 at every found key (the only entries read), `per` blocks a signal.
