@@ -458,6 +458,20 @@ class Batch:
         dt = np.complex128 if precision == MPSFT_FP64 else np.complex64
         return [o.to_numpy(dt, (n,)) for o, n in zip(od, self.ns)]
 
+    def modes_all(self):
+        """Every signal's (keys, coefs) of the last batched solve, copied on
+        the device in one launch (mpsft_batch_modes_counts / _device)."""
+        K = self.count
+        counts = (C.c_int64 * K)()
+        check(lib().mpsft_batch_modes_counts(self.h, counts), "batch_modes_counts")
+        kd = [DeviceArray(8 * max(int(m), 1)) for m in counts]
+        cd = [DeviceArray(16 * max(int(m), 1)) for m in counts]
+        kp = (C.c_void_p * K)(*[d.ptr.value for d in kd])
+        cp = (C.c_void_p * K)(*[d.ptr.value for d in cd])
+        check(lib().mpsft_batch_modes_device(self.h, kp, cp, None), "batch_modes_device")
+        return [(k.to_numpy(np.int64, (int(m),)), c.to_numpy(np.complex128, (int(m),)))
+                for k, c, m in zip(kd, cd, counts)]
+
     def modes_dict(self, i):
         cnt = C.c_int64(0)
         check(lib().mpsft_batch_modes_count(self.h, i, C.byref(cnt)), "batch_modes_count")

@@ -91,6 +91,9 @@ def batch(ns, ks, sigma, seeds, precision=0, autotune=False):
     tuned = b.autotune(xd, seeds) if autotune else None
     iters, stats = b.solve(xd, seeds)
     out = {"iters": iters, "status": stats, "tuned": tuned, "same_as_single": [], "same_as_oracle": []}
+    allm = b.modes_all()
+    out["modes_all_match"] = all(
+        {int(k): complex(c) for k, c in zip(*allm[i])} == b.modes_dict(i) for i in range(len(ns)))
     for i, (n, k, sd) in enumerate(zip(ns, ks, seeds)):
         got = b.modes_dict(i)
         s = gpu.Solver(n, opts[i], precision)

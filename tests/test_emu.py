@@ -78,6 +78,7 @@ def test_emu_batch_matches_single_and_oracle(emu_env):
                  sigma=1e-6, seeds=[3, 4, 5, 6])
     assert r["status"] == [0, 0, 0, 0]
     assert all(r["same_as_single"]) and all(r["same_as_oracle"])
+    assert r["modes_all_match"]  # the device-side read-back of every signal's modes
 
 
 @pytest.mark.parametrize("n,k,trials", [(1 << 14, 20, 1), (1 << 14, 20, 3), (1 << 16, 300, 1),

@@ -353,6 +353,9 @@ def test_batch_matches_single_and_oracle(gpu, orc, precision):
     iters, stats = b.solve(xd, seeds)
     assert stats == [0] * 12
     before = [b.modes_dict(i) for i in range(12)]
+    # every signal's modes copied on the device in one launch, same as read one by one
+    for i, (k_, c_) in enumerate(b.modes_all()):
+        assert {int(q): complex(v) for q, v in zip(k_, c_)} == before[i]
     tol = FP64_TOL if precision == 0 else MIXED_TOL
     for i in range(12):
         got = b.modes_dict(i)
