@@ -331,14 +331,24 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   if (precision != MPSFT_FP64 && precision != MPSFT_MIXED) return MPSFT_ERR_UNSUPPORTED;
   const int32_t K = count();
   if (!graph_) return MPSFT_ERR_SHAPE;  // no batched solve yet
+  // A replay since the solve may have grown an engine's arrays: the block
+  // maps and the Ctx array must describe the engines as they are now.
+  int rc0 = Build(s);
+  if (rc0 != MPSFT_OK) return rc0;
+  std::vector<Ctx> ctxs(K);
+  for (int32_t i = 0; i < K; ++i) {
+    ctxs[i] = eng_[i]->MakeCtx();
+    ctxs[i].batch_done = d_counters_ + 1;
+    ctxs[i].batch_host = d_host_;
+  }
+  HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
   std::vector<AdjSig> sig(K);
   int32_t max_steps = 0;
   int64_t max_m = 1, max_n = 1;
   for (int32_t i = 0; i < K; ++i) {
     Engine &e = *eng_[i];
     if (m[i] < 0 || (m[i] > 0 && (!keys[i] || !grad[i])) || !out[i]) return MPSFT_ERR_INVALID_ARG;
-    if (!e.rec_valid_ || !e.d_rec_ || e.rec_B_cap_ != e.B_cap_ || built_gen_[i] != e.shape_gen_)
-      return MPSFT_ERR_SHAPE;
+    if (!e.rec_valid_ || !e.d_rec_ || e.rec_B_cap_ != e.B_cap_) return MPSFT_ERR_SHAPE;
     const int rc = e.EnsureAdjBuffers(e.rec_iters_);
     if (rc != MPSFT_OK) return rc;
     AdjSig &g = sig[i];
@@ -382,7 +392,8 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   a.per_gather = int32_t(std::min<int64_t>(4096, (max_n + 1023) / 1024));
   a.mixed = precision == MPSFT_MIXED ? 1 : 0;
   HIP_OK(LaunchAdj(a, s));
-  // sig is read by the copy above, the outputs by the caller: wait for both.
+  // ctxs and sig are read by the copies above, the outputs by the caller:
+  // wait for all.
   HIP_OK(hipStreamSynchronize(s));
   return MPSFT_OK;
 }
