@@ -44,38 +44,77 @@ def run_case(env, timeout=600, **case):
     return json.loads(line[-1][7:])
 
 
-def test_emu_plan_bin_in_time(emu_env):
-    assert run_case(emu_env, kind="plan", n=4096, bins=101, bits=5, seed=1)["rel_err"] < 1e-12
+# Every case of this module, run as child processes side by side (each is
+# single-threaded) the first time a test asks for one.
+CASES = {
+    "plan": dict(kind="plan", n=4096, bins=101, bits=5, seed=1),
+    "plan_four_step": dict(kind="plan", n=1 << 16, bins=4097, bits=4, seed=2),
+    "iterate_four_step": dict(kind="iterate", n=1 << 16, k=2100, sigma=1e-6, seed=3, bins=4201),
+    "solve_0": dict(kind="solve", n=1 << 14, k=20, sigma=1e-6, seed=7, precision=0, repeat=2),
+    "solve_1": dict(kind="solve", n=1 << 14, k=20, sigma=3e-6, seed=7, precision=1, repeat=2),
+    "batch": dict(kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8], sigma=1e-6,
+                  seeds=[3, 4, 5, 6]),
+    "adjoint_16384_20_1": dict(kind="adjoint", n=1 << 14, k=20, sigma=1e-6, seed=7, trials=1),
+    "adjoint_16384_20_3": dict(kind="adjoint", n=1 << 14, k=20, sigma=1e-6, seed=7, trials=3),
+    "adjoint_65536_300_1": dict(kind="adjoint", n=1 << 16, k=300, sigma=1e-6, seed=7, trials=1),
+    "adjoint_10007_10_1": dict(kind="adjoint", n=10007, k=10, sigma=1e-6, seed=7, trials=1),
+    "batch_adjoint": dict(kind="batch_adjoint", ns=[4096, 8192, 16384], ks=[5, 12, 20],
+                          sigma=1e-6, seeds=[3, 4, 5]),
+}
 
 
-def test_emu_plan_four_step(emu_env):
+@pytest.fixture(scope="module")
+def emu_runs(emu_env):
+    import concurrent.futures as cf
+
+    def one(case):
+        try:
+            return run_case(emu_env, **case)
+        except AssertionError as e:  # reported by the test that reads the case
+            return e
+    workers = max(1, min(len(CASES), (os.cpu_count() or 2) - 1))
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = {name: ex.submit(one, case) for name, case in CASES.items()}
+        return {name: f.result() for name, f in futs.items()}
+
+
+def result(runs, name):
+    r = runs[name]
+    if isinstance(r, Exception):
+        raise r
+    return r
+
+
+def test_emu_plan_bin_in_time(emu_runs):
+    assert result(emu_runs, "plan")["rel_err"] < 1e-12
+
+
+def test_emu_plan_four_step(emu_runs):
     """B = 4097: M = 16384, the four-step bin FFT (k_big_cols/rows/out)."""
-    assert run_case(emu_env, kind="plan", n=1 << 16, bins=4097, bits=4, seed=2)["rel_err"] < 1e-12
+    assert result(emu_runs, "plan_four_step")["rel_err"] < 1e-12
 
 
-def test_emu_iterate_four_step_with_found_modes(emu_env):
+def test_emu_iterate_four_step_with_found_modes(emu_runs):
     """Two Iterate steps at B = 4201: the second subtracts the first's
     modes inside k_big_out."""
-    r = run_case(emu_env, kind="iterate", n=1 << 16, k=2100, sigma=1e-6, seed=3, bins=4201)
+    r = result(emu_runs, "iterate_four_step")
     assert r["same_support"] and r["n_found"] > 1000 and r["max_abs_err"] < 1e-10
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_emu_solve_graph(emu_env, precision):
+def test_emu_solve_graph(emu_runs, precision):
     """Demo1::Run through the captured two-iteration graph, twice on one
     solver."""
-    r = run_case(emu_env, kind="solve", n=1 << 14, k=20, sigma=1e-6 if precision == 0 else 3e-6,
-                 seed=7, precision=precision, repeat=2)
+    r = result(emu_runs, f"solve_{precision}")
     assert r["iters0"] == r["iters1"] == r["ref_iters"]
     assert r["bins"] == r["ref_bins"] and r["same_support"]
     assert r["max_rel_err"] < 1e-10
 
 
-def test_emu_batch_matches_single_and_oracle(emu_env):
+def test_emu_batch_matches_single_and_oracle(emu_runs):
     """The batched form on 4 signals of different n and S: each signal's
     result equals its own single-signal solve and the oracle."""
-    r = run_case(emu_env, kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8],
-                 sigma=1e-6, seeds=[3, 4, 5, 6])
+    r = result(emu_runs, "batch")
     assert r["status"] == [0, 0, 0, 0]
     assert all(r["same_as_single"]) and all(r["same_as_oracle"])
     assert r["modes_all_match"]  # the device-side read-back of every signal's modes
@@ -83,23 +122,22 @@ def test_emu_batch_matches_single_and_oracle(emu_env):
 
 @pytest.mark.parametrize("n,k,trials", [(1 << 14, 20, 1), (1 << 14, 20, 3), (1 << 16, 300, 1),
                                          (10007, 10, 1)])
-def test_emu_replay_and_adjoint(emu_env, n, k, trials):
+def test_emu_replay_and_adjoint(emu_runs, n, k, trials):
     """A recorded solve replayed on its own input gives the solve's modes;
     the replayed map is linear; the adjoint kernels satisfy
     <F d, g> = <d, F^H g>.  k = 300 takes B = 601 through the register
     Stockham bin FFT; the prime n takes the 64-bit index path of the
     gather."""
-    r = run_case(emu_env, kind="adjoint", n=n, k=k, sigma=1e-6, seed=7, trials=trials)
+    r = result(emu_runs, f"adjoint_{n}_{k}_{trials}")
     assert r["n_found"] >= k and r["same_keys"] and r["replay_err"] == 0.0
     assert r["linear_err"] < 1e-13 and r["dot_rel"] < 1e-12
 
 
-def test_emu_batched_adjoint(emu_env):
+def test_emu_batched_adjoint(emu_runs):
     """The batched adjoint (one launch per stage over 3 signals of different
     n and S): the dot-product identity per signal, and the same output as
     each signal's single-signal adjoint."""
-    r = run_case(emu_env, kind="batch_adjoint", ns=[4096, 8192, 16384], ks=[5, 12, 20],
-                 sigma=1e-6, seeds=[3, 4, 5])
+    r = result(emu_runs, "batch_adjoint")
     assert r["status"] == [0, 0, 0]
     assert max(r["dot_rel"]) < 1e-12 and max(r["vs_single"]) < 1e-12
 
