@@ -205,6 +205,7 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
   const int32_t K = count();
   const int32_t lookahead = 2;
   int32_t launched = 0;
+  const auto stall = StallWait();
   auto last_change = std::chrono::steady_clock::now();
   int32_t seen = 0;
   for (uint64_t spin = 0;; ++spin) {
@@ -240,7 +241,7 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
       seen = started;
       last_change = std::chrono::steady_clock::now();
     } else if ((spin & 4095) == 0 &&
-               std::chrono::steady_clock::now() - last_change > std::chrono::milliseconds(200)) {
+               std::chrono::steady_clock::now() - last_change > stall) {
       // No report for a while: wait for the queued work, then go on.
       if (hipStreamSynchronize(s) != hipSuccess) return fail(MPSFT_ERR_HIP);
       last_change = std::chrono::steady_clock::now();

@@ -87,7 +87,31 @@ int32_t ChunkIters() {
   return 2;
 }
 
+// Lookahead once every launch is a late (two-kernel) iteration: B no longer
+// changes, so launching further ahead commits no graph choice early, and
+// each iteration is short enough that two queued ones do not cover the
+// host's launch latency.
+int32_t LateLookahead() {
+  const char *e = std::getenv("MPSFT_LATE_LOOKAHEAD");
+  if (e) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4096) return v;
+  }
+  return 3;
+}
+
 }  // namespace
+
+// How long a host launch loop spins without a progress report before it
+// waits on the stream instead (MPSFT_STALL_MS, default 200 ms).
+std::chrono::milliseconds StallWait() {
+  const char *e = std::getenv("MPSFT_STALL_MS");
+  if (e) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 3600000) return std::chrono::milliseconds(v);
+  }
+  return std::chrono::milliseconds(200);
+}
 
 // NumBits (src/iterate.cc:30-42): 1 + floor(log2(n / bins)), at least 1.
 int32_t NumBits(int64_t n, int32_t bins) {
@@ -566,7 +590,7 @@ int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big) {
 int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
   const int want = (fft_big_ && fft_small_) ? 2 : 1;
   if (graph_exec_[0] && graph_key_ == shape_gen_ && graph_iters_ == iters &&
-      (want == 1 || graph_exec_[1]))
+      (want == 1 || graph_exec_[1]) && graph_late_B_ == late_B_)
     return MPSFT_OK;
   Drain();  // a graph may still run the launches a solve queued past its stop
   for (auto &e : graph_exec_) {
@@ -596,6 +620,36 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
       return MPSFT_ERR_HIP;
     }
   }
+  if (late_B_ > 0) {
+    // The late graphs: launches sized for B = late_B_ only.
+    const BParams &bp = hbtab_[late_B_];
+    const int32_t waves = kThreads / 64;
+    const int32_t nb_ident = std::max<int32_t>(1, (late_B_ + waves - 1) / waves);
+    const int32_t tiles = trials_cap_ * ((late_B_ + bp.R - 1) / bp.R);
+    for (int k = 2; k < 4; ++k) {
+      hipStream_t cs;
+      HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipGraph_t g;
+      if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return MPSFT_ERR_HIP;
+      }
+      hipError_t el = hipSuccess;
+      for (int32_t i = 0; i < iters && el == hipSuccess; ++i)
+        el = LaunchLateIteration(c, i & 1, nb_ident, tiles, LdsBinTime(c.mixed != 0), LdsFft(),
+                                 (k == 3 || i > 0) ? 1 : 0, cs);
+      const hipError_t ec = hipStreamEndCapture(cs, &g);
+      (void)hipStreamDestroy(cs);
+      if (el != hipSuccess || ec != hipSuccess) return MPSFT_ERR_HIP;
+      const hipError_t ei = hipGraphInstantiate(&graph_exec_[k], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        graph_exec_[k] = nullptr;
+        return MPSFT_ERR_HIP;
+      }
+    }
+  }
+  graph_late_B_ = late_B_;
   graph_iters_ = iters;
   graph_key_ = shape_gen_;
   (void)s;
@@ -756,9 +810,17 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       (void)hipStreamSynchronize(s);
       return code;
     };
+    // Late graphs when the last B of the solve takes the LDS bin FFT.
+    late_B_ = (opt.min_bins <= kMaxLdsBins && hbtab_[opt.min_bins].logM <= kMaxLogM)
+                  ? opt.min_bins : 0;
     rc = EnsureGraph(2, s);
     if (rc != MPSFT_OK) return fail(rc);
     const int32_t lookahead = chunk < 2 ? 2 : chunk;
+    const int32_t late_lookahead = std::max(lookahead, LateLookahead());
+    // 0: three-launch iterations; 1: the next graph opens the late form
+    // (the iterations launched so far have their identify); 2: late form.
+    int late = 0;
+    const auto stall = StallWait();
     auto last_change = std::chrono::steady_clock::now();
     int32_t seen = 0, synced = 0;
     bool done = false;
@@ -768,7 +830,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       // Launch up to iteration max_iter: its first kernel sees the cap and
       // stops the loop (logging the last iteration).
       if (h_progress_[1] || launched > max_iter) break;
-      if (launched - progressed < lookahead) {
+      if (launched - progressed < (late == 2 ? late_lookahead : lookahead)) {
         const int32_t need = std::min(max_iter, launched + 3);
         if (need > tf_count_) {
           // Draw ahead in doubling batches.  Growing the arrays moves them,
@@ -780,10 +842,18 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
           if (rc == MPSFT_OK) rc = EnsureGraph(2, s);
           if (rc != MPSFT_OK) return fail(rc);
         }
-        // progress[2] is the B of the latest iteration the device started
+        // progress[2] is the B of the latest iteration the device started;
+        // B never grows, so once it is min_bins every later iteration has it.
         const int32_t bnow = h_progress_[2];
+        if (late == 0 && late_B_ > 0 && bnow == late_B_) late = 1;
         hipGraphExec_t ge = (graph_exec_[1] && bnow > 0 && bnow <= kMaxLdsBins) ? graph_exec_[1]
                                                                                  : graph_exec_[0];
+        if (late == 1) {
+          ge = graph_exec_[2];
+          late = 2;
+        } else if (late == 2) {
+          ge = graph_exec_[3];
+        }
         if (hipGraphLaunch(ge, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
         launched += 2;
         continue;
@@ -792,7 +862,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
         seen = progressed;
         last_change = std::chrono::steady_clock::now();
       } else if ((spin & 4095) == 0 &&
-                 std::chrono::steady_clock::now() - last_change > std::chrono::milliseconds(200)) {
+                 std::chrono::steady_clock::now() - last_change > stall) {
         // No progress report for a while (a long iteration): wait on the
         // stream instead of spinning, then continue from the device state.
         rc = ReadState(s);

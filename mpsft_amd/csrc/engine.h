@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <random>
 #include <vector>
@@ -105,6 +106,8 @@ struct DevState {
   int32_t stop_stale;     // stale count when the loop stopped
   LoopCounters L[2];
   IterParams P[2];
+  int32_t cnt_base[2];  // per parity: |found| when that iteration started
+  int32_t pad_[2];
 };
 
 // Everything a kernel needs; passed by value.
@@ -282,9 +285,12 @@ class Engine {
   std::vector<int64_t> h_qs_;
 
   // graph of graph_iters_ iterations for the current shapes
-  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};  // [1]: without the four-step FFT
+  // [1]: without the four-step FFT; [2], [3]: the late iterations at
+  // B = late_B_ (two launches each), [2] opening with no identify
+  hipGraphExec_t graph_exec_[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t graph_iters_ = 0;
   int64_t graph_key_ = -1;
+  int32_t late_B_ = 0, graph_late_B_ = 0;  // min_bins of the solve (0: no late graphs)
   int64_t shape_gen_ = 0;
   int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
 
@@ -329,5 +335,6 @@ int32_t NumBits(int64_t n, int32_t bins);
 bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp);
 int64_t InvMod(int64_t a, int64_t n);
 DevTf ReduceTf(int64_t n, int64_t a, int64_t b, int64_t c, int64_t a_inv);
+std::chrono::milliseconds StallWait();
 
 }  // namespace mpsft

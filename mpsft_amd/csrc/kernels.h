@@ -25,6 +25,13 @@ hipError_t LaunchClearModes(const Ctx &c, int32_t full, hipStream_t s);
 hipError_t LaunchInit(const Ctx &c, const InitArgs &a, hipStream_t s);
 // LaunchClearModes(c, 0) then LaunchInit in one single-workgroup launch.
 hipError_t LaunchReset(const Ctx &c, const InitArgs &a, hipStream_t s);
+// One late iteration of a solve (every iteration from here on has
+// B = min_bins <= kMaxLdsBins): k_late_fold (the previous iteration's
+// identify unless with_identify is 0, beside this one's BinInFreq factors
+// and gather-fold over `tiles` tiles) then k_late_fft.
+hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32_t tiles,
+                               size_t lds_time, size_t lds_fft, int with_identify,
+                               hipStream_t s);
 // parity = iteration & 1: the IterParams slot the kernels use.
 hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s);
 hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);

@@ -316,6 +316,7 @@ __device__ __forceinline__ void InitBody(const Ctx &c, const InitArgs &a) {
   st->L[1].stale = 0;
   st->L[1].done = 0;
   st->L[0] = st->L[1];  // no stale "stopped" from an earlier call
+  st->cnt_base[0] = st->cnt_base[1] = 0;
   st->P[1].B = a.B;
   st->P[1].done = 0;
   if (c.progress) {
@@ -396,6 +397,7 @@ __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
       st->P[p].q[t] = c.qs[int64_t(lc.it) * c.trials_cap + t];
   st->L[p] = lc;
   st->found_iter[p] = 0;
+  st->cnt_base[p] = st->count;
   if (prev.done) return;
   if (st->mode == kModeSolve && prev.it >= 0 && prev.it < c.log_cap) {
     c.log[prev.it] = st->P[1 - p].B;
@@ -491,6 +493,31 @@ struct Lags {
 // found mode for this iteration.
 // blk: this workgroup's block index within the signal's launch.
 template <typename Tx>
+__device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int32_t trials,
+                                         int32_t tile, double2 *lds);
+
+// The BinInFreq factor of mode k0 (k1 = its image under the transform) in
+// row 0 for offset q: wf(xi) e^{2 pi i (q k1 + c k0)/n}, and its bin.  Every
+// form of an iteration builds a term as coef (base) or coef (base g) in
+// this order, so all of them subtract bit-identical terms.
+__device__ __forceinline__ double2 FreqBase(const Ctx &c, const IterParams &ip, int64_t q,
+                                            int64_t k0, int64_t k1, int32_t *bin) {
+  const int64_t n = c.n, mask = c.mask;
+  const int32_t B = ip.B;
+  const int32_t b = int32_t(k1 * B / n);
+  const double xi = (double(b) + 0.5) / double(B) - double(k1) / double(n);
+  const double wf = SampleInFreq(xi, ip.width, ip.dfreq);
+  *bin = b;
+  return Scale(CisFrac(ModN(q * k1 + ip.tf.c * k0, n, mask), n), wf);
+}
+
+// g = e^{2 pi i (B 2^bit k1)/n}: rows 2bit+1 and 2bit+2 take base g and
+// base conj(g).
+__device__ __forceinline__ double2 BitTwiddle(const Ctx &c, int32_t B, int bit, int64_t k1) {
+  return CisFrac(ModN((int64_t(B) << bit) * k1, c.n, c.mask), c.n);
+}
+
+template <typename Tx>
 __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double2 *lds) {
   const DevState *st = c.st;
   IterParams ip;
@@ -499,17 +526,16 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
   if (blk == 0 && threadIdx.x == 0) PublishIteration(c, p, ip, lc);
   if (ip.done) return;
   const int32_t trials = st->trials;
-  const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
+  const int32_t B = ip.B, bits = ip.bits;
   const DevTf tf = ip.tf;
   const int64_t n = c.n, mask = c.mask;
 
   if (blk < kPrepBlocks) {
     // BinInFreqV1 (src/binner.cc:565-591) per found mode and trial: bin,
-    // base = wf * coef * e^{2 pi i (q k1 + c k0)/n}, and per bit
-    // g = e^{2 pi i (B 2^bit k1)/n} for rows 2bit+1 (base g) and 2bit+2
-    // (base conj(g)).
+    // base = wf e^{2 pi i (q k1 + c k0)/n}, and per bit
+    // g = e^{2 pi i (B 2^bit k1)/n}; the terms are coef base (row 0),
+    // coef (base g) (row 2bit+1) and coef (base conj(g)) (row 2bit+2).
     const int32_t cnt = min(st->count, c.cap);
-    const double width = ip.width, dfreq = ip.dfreq;
     for (int64_t i = int64_t(blk) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
          i += int64_t(kPrepBlocks) * blockDim.x) {
       const int32_t trial = int32_t(i / cnt);
@@ -518,25 +544,34 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
       const double2 cf = c.coef[k0];
       const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
       const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
-      const int32_t bin = int32_t(k1 * B / n);
-      const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
-      const double wf = SampleInFreq(xi, width, dfreq);
-      const double2 base = Cmul(Scale(cf, wf), CisFrac(ModN(q * k1 + tf.c * k0, n, mask), n));
+      int32_t bin;
+      const double2 base = FreqBase(c, ip, q, k0, k1, &bin);
       c.term_bin[int64_t(trial) * c.cap + mi] = bin;
       double2 *val = c.term_val + int64_t(trial) * c.rows_cap * c.cap + mi;
-      val[0] = base;
+      val[0] = Cmul(cf, base);
       for (int32_t bit = 0; bit < bits; ++bit) {
-        const double2 g = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
-        val[int64_t(2 * bit + 1) * c.cap] = Cmul(base, g);
-        val[int64_t(2 * bit + 2) * c.cap] = Cmul(base, Conj(g));
+        const double2 g = BitTwiddle(c, B, bit, k1);
+        val[int64_t(2 * bit + 1) * c.cap] = Cmul(cf, Cmul(base, g));
+        val[int64_t(2 * bit + 2) * c.cap] = Cmul(cf, Cmul(base, Conj(g)));
       }
     }
     return;
   }
 
+  TileFold<Tx>(c, ip, trials, blk - kPrepBlocks, lds);
+}
+
+// The gather-fold of one tile (trial, block of R residues) of an iteration
+// with parameters ip (see BinTimeBody).
+template <typename Tx>
+__device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int32_t trials,
+                                         int32_t tile, double2 *lds) {
+  const DevState *st = c.st;
+  const int32_t B = ip.B, f = ip.f, bits = ip.bits, p2 = ip.p2, rows = ip.rows;
+  const DevTf tf = ip.tf;
+  const int64_t n = c.n, mask = c.mask;
   const int32_t R = ip.R;
   const int32_t nblk = (B + R - 1) / R;
-  int32_t tile = blk - kPrepBlocks;
   if (tile >= trials * nblk) return;
   const int32_t trial = tile / nblk;
   const int32_t rho0 = (tile - trial * nblk) * R;
@@ -1421,6 +1456,225 @@ __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
   IdentifyBody(c, p, blockIdx.x, gridDim.x);
 }
 
+// ---------------------------------------------------------------------------
+// The late iterations of a solve.  Once an iteration starts with
+// 2 (k - |found|) + 1 <= min_bins, every later one has B = min_bins (|found|
+// only grows), so iteration i's gather-fold no longer depends on iteration
+// i-1's identify.  An iteration then takes two launches instead of three:
+//   k_late_fold(i): identify of i-1 (blocks below nb_ident) beside the
+//                   gather-fold of i and, in kPrepBlocks blocks, the
+//                   BinInFreq factors of i for the modes found before i-1
+//                   (wf e^{2 pi i (q k1 + c k0)/n} times the row's phase:
+//                   the term without the coefficient, which i-1's identify
+//                   may still change);
+//   k_late_fft(i):  i's loop control (stale count, stop), then its bin DFTs;
+//                   the terms are the factors times the final coefficients,
+//                   and for the modes i-1 found the whole term.
+// Slots as in the three-launch form: iteration i (parity p) reads loop
+// state of slot 1-p and writes slot p.  cnt_base[p] is the number of modes
+// at the start of the iteration of parity p.
+
+// Iteration parameters of a late iteration (false: it does not exist).
+__device__ __forceinline__ bool LateGeometry(const Ctx &c, int p, IterParams *ip) {
+  const DevState *st = c.st;
+  const LoopCounters prev = st->L[1 - p];
+  if (prev.done) return false;
+  const int32_t it = prev.it + 1;
+  if (it >= st->max_iter) return false;
+  const BParams bp = c.btab[st->min_bins];
+  FillParams(ip, bp, bp.bits);
+  ip->bin_thr = bp.bin_thr;
+  ip->sv_thr = bp.sv_thr;
+  ip->tf = c.tfs[it];
+  ip->it = it;
+  ip->done = 0;
+  return true;
+}
+
+// The BinInFreq factor of mode k0 for (q, row) and its bin.
+__device__ __forceinline__ double2 RowFactor(const Ctx &c, const IterParams &ip, int64_t q, int row,
+                                             int64_t k0, int64_t k1, int32_t *bin) {
+  const double2 base = FreqBase(c, ip, q, k0, k1, bin);
+  if (row == 0) return base;
+  const double2 g = BitTwiddle(c, ip.B, (row - 1) / 2, k1);
+  return Cmul(base, (row & 1) ? g : Conj(g));
+}
+
+template <typename Tx>
+__global__ void __launch_bounds__(kThreads)
+    k_late_fold(Ctx c, int p, int32_t nb_ident, int32_t with_identify) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int32_t blk = blockIdx.x;
+  if (blk < nb_ident) {
+    if (with_identify) IdentifyBody(c, 1 - p, blk, nb_ident);
+    return;
+  }
+  IterParams ip;
+  if (!LateGeometry(c, p, &ip)) return;
+  DevState *st = c.st;
+  const int32_t trials = st->trials;
+  const int32_t pb = blk - nb_ident;
+  if (pb == 0 && threadIdx.x == 0) {  // publish the geometry for k_late_fft
+    st->P[p] = ip;
+    for (int t = 0; t < trials; ++t) st->P[p].q[t] = c.qs[int64_t(ip.it) * c.trials_cap + t];
+    if (c.progress) {
+      __hip_atomic_store(&c.progress[2], ip.B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[0], ip.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+  if (pb < kPrepBlocks) {
+    // factors of the modes found before iteration i-1 (final keys)
+    const int32_t cnt = min(st->cnt_base[1 - p], c.cap);
+    const int32_t rows = ip.rows;
+    const int64_t n = c.n, mask = c.mask;
+    for (int64_t i = int64_t(pb) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
+         i += int64_t(kPrepBlocks) * blockDim.x) {
+      const int32_t trial = int32_t(i / cnt);
+      const int32_t mi = int32_t(i - int64_t(trial) * cnt);
+      const int64_t k0 = c.keys[mi];
+      const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
+      const int64_t k1 = ModN(ip.tf.a * k0 + ip.tf.b, n, mask);
+      int32_t bin;
+      const double2 base = FreqBase(c, ip, q, k0, k1, &bin);
+      double2 *val = c.term_val + int64_t(trial) * c.rows_cap * c.cap + mi;
+      val[0] = base;
+      for (int32_t bit = 0; 2 * bit + 1 < rows; ++bit) {
+        const double2 g = BitTwiddle(c, ip.B, bit, k1);
+        val[int64_t(2 * bit + 1) * c.cap] = Cmul(base, g);
+        val[int64_t(2 * bit + 2) * c.cap] = Cmul(base, Conj(g));
+      }
+      c.term_bin[int64_t(trial) * c.cap + mi] = bin;
+    }
+    return;
+  }
+  TileFold<Tx>(c, ip, trials, pb - kPrepBlocks, lds);
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  constexpr int T = kFftThreads;
+  constexpr int kPerTerm = 2;  // factors prefetched per thread
+  DevState *st = c.st;
+  // Loop control of this iteration (NextIteration's; identify of the
+  // previous iteration has finished).
+  const LoopCounters prev = st->L[1 - p];
+  if (prev.done) {
+    // Past the stop: carry it into this parity's slots, so the launches
+    // still queued behind this one (the next k_late_fold's identify and
+    // geometry read them) see a stopped loop too.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->L[p] = prev;
+      st->P[p].done = 1;
+      st->found_iter[p] = 0;
+    }
+    return;
+  }
+  LoopCounters lc;
+  lc.it = prev.it + 1;
+  lc.stale = prev.stale + (st->found_iter[1 - p] ? 0 : 1);
+  lc.pad = 0;
+  const bool done =
+      (st->flags & kFlagOverflow) || lc.stale >= st->max_stale || lc.it >= st->max_iter;
+  lc.done = done ? 1 : 0;
+  const int32_t cnt_pre = min(st->cnt_base[1 - p], c.cap);  // modes with factors
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // PublishIteration's bookkeeping, for the slot k_late_fold filled.
+    st->L[p] = lc;
+    st->P[p].done = lc.done;  // read by the next launch's identify
+    st->found_iter[p] = 0;
+    st->cnt_base[p] = st->count;
+    if (st->mode == kModeSolve && prev.it >= 0 && prev.it < c.log_cap) {
+      c.log[prev.it] = st->P[1 - p].B;
+      c.log[c.log_cap + prev.it] = st->count;
+    }
+    if (lc.done) {
+      st->iters = lc.it;
+      st->stop_stale = lc.stale;
+    }
+    if (c.progress && lc.done) {
+      __hip_atomic_store(&c.progress[3], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[4], lc.stale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[5], st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+  if (done) return;
+  const IterParams &ip = st->P[p];
+  const int32_t trials = st->trials, rows = ip.rows;
+  const int32_t trial = blockIdx.x / c.rows_cap;
+  const int32_t row = blockIdx.x - trial * c.rows_cap;
+  if (trial >= trials || row >= rows) return;
+  const int32_t B = ip.B;
+  const int tid = threadIdx.x;
+  const int64_t n = c.n, mask = c.mask;
+  const double2 *fold = c.fold + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
+  const int32_t cnt = min(st->count, c.cap);
+  const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
+  const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
+  // prefetch: factor, bin and key of the first modes (coefficients after)
+  double2 fac[kPerTerm];
+  int tbin[kPerTerm];
+  int64_t kk[kPerTerm];
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) {
+    const int i = tid + r * T;
+    fac[r] = make_double2(0.0, 0.0);
+    tbin[r] = 0;
+    kk[r] = -1;
+    if (i < cnt) kk[r] = c.keys[i];
+    if (i < cnt_pre) {
+      fac[r] = tv[i];
+      tbin[r] = tb[i];
+    }
+  }
+  const int64_t q = ip.q[trial];
+  const int64_t tau = ModN(q + RowOffset(row) * B, n, mask);
+  const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+  double2 *a = lds;
+  const bool small = ip.logM <= kFftSmallLogM;
+  if (small)
+    BinFftSmall<false>(c, ip, fold, rp, a);
+  else
+    BinFftLarge<false>(c, ip, fold, rp, a);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) {
+    const int i = tid + r * T;
+    if (i < cnt) {
+      double2 v;
+      int32_t bin = tbin[r];
+      if (i < cnt_pre) {
+        v = Cmul(c.coef[kk[r]], fac[r]);
+      } else {
+        const int64_t k1 = ModN(ip.tf.a * kk[r] + ip.tf.b, n, mask);
+        v = Cmul(c.coef[kk[r]], RowFactor(c, ip, q, row, kk[r], k1, &bin));
+      }
+      const int j = small ? bin : Pad(bin);
+      atomicAdd(&a[j].x, -v.x);
+      atomicAdd(&a[j].y, -v.y);
+    }
+  }
+  for (int i = tid + kPerTerm * T; i < cnt; i += T) {
+    const int64_t k0 = c.keys[i];
+    double2 v;
+    int32_t bin;
+    if (i < cnt_pre) {
+      v = Cmul(c.coef[k0], tv[i]);
+      bin = tb[i];
+    } else {
+      const int64_t k1 = ModN(ip.tf.a * k0 + ip.tf.b, n, mask);
+      v = Cmul(c.coef[k0], RowFactor(c, ip, q, row, k0, k1, &bin));
+    }
+    const int j = small ? bin : Pad(bin);
+    atomicAdd(&a[j].x, -v.x);
+    atomicAdd(&a[j].y, -v.y);
+  }
+  __syncthreads();
+  double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
+  for (int k = tid; k < B; k += T) A[k] = a[small ? k : Pad(k)];
+}
+
 // BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix: one
 // thread per mode, subtracting from every row with device atomics.
 __global__ void __launch_bounds__(kThreads)
@@ -2034,6 +2288,20 @@ hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s) {
   hipLaunchKernelGGL(k_big_cols, dim3((rows + 1) * c.big_gpr), dim3(kThreads), 0, s, c, parity);
   hipLaunchKernelGGL(k_big_rows, dim3(rows * c.big_gpr), dim3(kThreads), 0, s, c, parity);
   hipLaunchKernelGGL(k_big_out, dim3(rows * c.big_gpr), dim3(kThreads), 0, s, c, parity);
+  return hipGetLastError();
+}
+
+hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32_t tiles,
+                               size_t lds_time, size_t lds_fft, int with_identify,
+                               hipStream_t s) {
+  if (c.mixed)
+    hipLaunchKernelGGL(k_late_fold<float2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
+                       lds_time, s, c, parity, nb_ident, with_identify);
+  else
+    hipLaunchKernelGGL(k_late_fold<double2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
+                       lds_time, s, c, parity, nb_ident, with_identify);
+  hipLaunchKernelGGL(k_late_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds_fft, s, c,
+                     parity);
   return hipGetLastError();
 }
 

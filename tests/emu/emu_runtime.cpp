@@ -6,8 +6,13 @@
 #include <ucontext.h>
 
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace emu {
@@ -17,7 +22,7 @@ thread_local dim3 bdim, gdim;
 
 namespace {
 
-// A workgroup runs as one fiber per thread on the launching OS thread;
+// A workgroup runs as one fiber per thread on the device worker thread;
 // fibers switch only at barriers and wave operations, so every access is
 // sequential and ASan sees one coherent stack per fiber.
 void Yield();
@@ -225,6 +230,88 @@ struct EmuGraph {
 namespace {
 thread_local EmuGraph *g_capture = nullptr;
 thread_local hipStream_t g_capture_stream = nullptr;
+
+// The device: one worker thread runs every stream's work in submission
+// order while the host goes on, as a GPU runs queued launches after the
+// launching call has returned.  Host code that reads what kernels wrote
+// without waiting for them (pinned progress words) sees it late, and
+// launches a host queues past a loop's stop run after the host has moved
+// on.  Synchronous calls (hipMemcpy, hipMemset, every *Synchronize,
+// hipFree) first wait until the queue is empty.
+class Device {
+ public:
+  Device() : worker_([this] { Loop(); }) { worker_.detach(); }
+  void Submit(std::function<void()> task) {
+    std::lock_guard<std::mutex> lk(mu_);
+    q_.push_back(std::move(task));
+    cv_.notify_all();
+  }
+  void Wait() {
+    if (std::this_thread::get_id() == id_) return;  // called from a kernel body
+    std::unique_lock<std::mutex> lk(mu_);
+    idle_cv_.wait(lk, [this] { return q_.empty() && !busy_; });
+  }
+
+ private:
+  void Loop() {
+    id_ = std::this_thread::get_id();
+    for (;;) {
+      std::function<void()> task;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [this] { return !q_.empty(); });
+        task = std::move(q_.front());
+        q_.pop_front();
+        busy_ = true;
+      }
+      task();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        busy_ = false;
+        if (q_.empty()) idle_cv_.notify_all();
+      }
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_, idle_cv_;
+  std::deque<std::function<void()>> q_;
+  bool busy_ = false;
+  std::thread::id id_;
+  std::thread worker_;
+};
+
+// Device (hipMalloc) and pinned (hipHostMalloc) allocations: an async copy
+// between two of them is queued; one with a pageable host side waits for
+// the queue and copies before returning, as HIP stages pageable memory.
+std::mutex g_alloc_mu;
+std::map<uintptr_t, size_t> g_allocs;
+
+void Track(void *p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_allocs[reinterpret_cast<uintptr_t>(p)] = bytes;
+}
+void Untrack(void *p) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_allocs.erase(reinterpret_cast<uintptr_t>(p));
+}
+bool Tracked(const void *p) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = g_allocs.upper_bound(a);
+  if (it == g_allocs.begin()) return false;
+  --it;
+  return a < it->first + it->second;
+}
+
+Device &Dev() {
+  // Never destroyed: the worker thread may outlive static destructors.
+  static Device *d = [] {
+    Device *p = new Device;
+    std::atexit([] { Dev().Wait(); });
+    return p;
+  }();
+  return *d;
+}
 }  // namespace
 
 void emu::Launch(void *stream, dim3 grid, dim3 block, size_t lds, std::function<void()> body) {
@@ -233,7 +320,7 @@ void emu::Launch(void *stream, dim3 grid, dim3 block, size_t lds, std::function<
     g_capture->nodes.push_back(std::move(nd));
     return;
   }
-  Run(nd);
+  Dev().Submit([nd = std::move(nd)] { Run(nd); });
 }
 
 hipError_t hipGetLastError() { return hipSuccess; }
@@ -245,9 +332,12 @@ hipError_t hipMalloc(void **p, size_t bytes) {
   *p = nullptr;
   if (posix_memalign(p, 256, bytes ? bytes : 1) != 0) return hipErrorOutOfMemory;
   std::memset(*p, 0xa5, bytes);  // device memory starts as garbage
+  Track(*p, bytes ? bytes : 1);
   return hipSuccess;
 }
 hipError_t hipFree(void *p) {
+  Dev().Wait();
+  if (p) Untrack(p);
   std::free(p);
   return hipSuccess;
 }
@@ -258,40 +348,55 @@ hipError_t hipHostGetDevicePointer(void **dev, void *host, unsigned) {
   return hipSuccess;
 }
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind) {
+  Dev().Wait();
   if (bytes) std::memcpy(dst, src, bytes);
   return hipSuccess;
 }
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind k,
                           hipStream_t) {
   if (g_capture) return hipErrorNotSupported;
+  if (Tracked(dst) && Tracked(src)) {
+    if (bytes) Dev().Submit([=] { std::memcpy(dst, src, bytes); });
+    return hipSuccess;
+  }
   return hipMemcpy(dst, src, bytes, k);
 }
 hipError_t hipMemcpy2D(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
                        size_t height, hipMemcpyKind) {
+  Dev().Wait();
   for (size_t r = 0; r < height; ++r)
     std::memcpy(static_cast<char *>(dst) + r * dpitch,
                 static_cast<const char *>(src) + r * spitch, width);
   return hipSuccess;
 }
 hipError_t hipMemset(void *dst, int value, size_t bytes) {
+  Dev().Wait();
   if (bytes) std::memset(dst, value, bytes);
   return hipSuccess;
 }
 hipError_t hipMemsetAsync(void *dst, int value, size_t bytes, hipStream_t) {
   if (g_capture) return hipErrorNotSupported;
-  return hipMemset(dst, value, bytes);
+  if (bytes) Dev().Submit([=] { std::memset(dst, value, bytes); });
+  return hipSuccess;
 }
-hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipDeviceSynchronize() {
+  Dev().Wait();
+  return hipSuccess;
+}
 hipError_t hipStreamCreate(hipStream_t *s) {
   *s = new EmuStream{0};
   return hipSuccess;
 }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
 hipError_t hipStreamDestroy(hipStream_t s) {
+  Dev().Wait();
   delete s;
   return hipSuccess;
 }
-hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) {
+  Dev().Wait();
+  return hipSuccess;
+}
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   if (g_capture || !s) return hipErrorInvalidValue;
   g_capture = new EmuGraph;
@@ -319,7 +424,7 @@ hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
   return hipSuccess;
 }
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
-  for (const auto &nd : e->nodes) emu::Run(nd);
+  for (const auto &nd : e->nodes) Dev().Submit([nd] { emu::Run(nd); });
   return hipSuccess;
 }
 hipError_t hipEventCreate(hipEvent_t *e) {
@@ -327,14 +432,18 @@ hipError_t hipEventCreate(hipEvent_t *e) {
   return hipSuccess;
 }
 hipError_t hipEventDestroy(hipEvent_t e) {
+  Dev().Wait();
   delete e;
   return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
-  e->t = std::chrono::steady_clock::now();
+  Dev().Submit([e] { e->t = std::chrono::steady_clock::now(); });
   return hipSuccess;
 }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) {
+  Dev().Wait();
+  return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) {
   *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
   return hipSuccess;

@@ -4,13 +4,15 @@
 // mismatches without a GPU).  It is never part of the shipped library.
 //
 // Model: a launch runs its workgroups one after another; the threads of a
-// workgroup are fibers on the launching thread, switched at barriers.
+// workgroup are fibers on one device worker thread, switched at barriers.
 // __syncthreads is a barrier over the threads still running; wave
 // operations (__shfl, __ballot, __any) are a barrier over the 64 lanes of
 // the wave plus an exchange buffer.  Dynamic LDS is a
 // heap block of exactly the launch's byte count, so ASan reports any read
-// or write past it.  Streams run synchronously; stream capture records
-// launches into a graph that hipGraphLaunch replays.
+// or write past it.  Launches, graph launches, hipMemsetAsync and event
+// records queue on the worker, which runs them in order while the host
+// goes on; synchronous calls and hipMemcpyAsync wait for the queue first.
+// Stream capture records launches into a graph that hipGraphLaunch queues.
 #pragma once
 
 #include <algorithm>

@@ -42,6 +42,14 @@ def solve(n, k, sigma, seed, precision=0, repeat=1, x_path=None, **kw):
         bins, found = s.log()
         got = s.modes_dict()
         out[f"iters{r}"] = it
+    # Back to back, nothing read in between, the host launching up to 6
+    # iterations ahead and never falling back to waiting on the stream: each
+    # solve starts while launches the previous one queued past its stop
+    # still wait on the device, and must neither see nor be disturbed by them.
+    os.environ.update(MPSFT_LOOKAHEAD="6", MPSFT_STALL_MS="600000")
+    out["back_to_back_iters"] = [s.solve(xd, seed) for _ in range(3)]
+    del os.environ["MPSFT_LOOKAHEAD"], os.environ["MPSFT_STALL_MS"]
+    out["back_to_back_same"] = s.modes_dict() == got and s.log()[0].tolist() == bins.tolist()
     orc.seed(seed)
     ref, rb, rf = orc.demo1_run(xo, k, sigma, orc.Demo1Options(**kw))
     ref = ref.to_dict()

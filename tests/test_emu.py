@@ -52,6 +52,7 @@ CASES = {
     "iterate_four_step": dict(kind="iterate", n=1 << 16, k=2100, sigma=1e-6, seed=3, bins=4201),
     "solve_0": dict(kind="solve", n=1 << 14, k=20, sigma=1e-6, seed=7, precision=0, repeat=2),
     "solve_1": dict(kind="solve", n=1 << 14, k=20, sigma=3e-6, seed=7, precision=1, repeat=2),
+    "solve_2": dict(kind="solve", n=1 << 16, k=300, sigma=1e-6, seed=5, precision=0, repeat=1),
     "batch": dict(kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8], sigma=1e-6,
                   seeds=[3, 4, 5, 6]),
     "adjoint_16384_20_1": dict(kind="adjoint", n=1 << 14, k=20, sigma=1e-6, seed=7, trials=1),
@@ -101,12 +102,18 @@ def test_emu_iterate_four_step_with_found_modes(emu_runs):
     assert r["same_support"] and r["n_found"] > 1000 and r["max_abs_err"] < 1e-10
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_emu_solve_graph(emu_runs, precision):
-    """Demo1::Run through the captured two-iteration graph, twice on one
-    solver."""
-    r = result(emu_runs, f"solve_{precision}")
-    assert r["iters0"] == r["iters1"] == r["ref_iters"]
+@pytest.mark.parametrize("case", ["solve_0", "solve_1", "solve_2"])
+def test_emu_solve_graph(emu_runs, case):
+    """Demo1::Run through the captured two-iteration graphs, on one solver
+    (twice for the first two cases), then three times back to back, each
+    solve starting behind the launches the previous one queued past its
+    stop.  solve_2 (k = 300) starts at B = 601 and goes over from the
+    three-launch iterations to the two-launch late form at B = min_bins."""
+    r = result(emu_runs, case)
+    assert all(r[f"iters{i}"] == r["ref_iters"] for i in range(2) if f"iters{i}" in r)
+    assert r["bins"] == r["ref_bins"] and r["same_support"]
+    assert r["max_rel_err"] < 1e-10
+    assert r["back_to_back_iters"] == [r["iters0"]] * 3 and r["back_to_back_same"]
     assert r["bins"] == r["ref_bins"] and r["same_support"]
     assert r["max_rel_err"] < 1e-10
 

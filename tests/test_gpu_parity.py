@@ -292,6 +292,29 @@ def test_solve_parity_config2_mixed(gpu, orc):
     assert len(hits) >= 0.95 * k
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_back_to_back(gpu, precision):
+    """Ten solves of the N=2^22, S=1000 complex64-rounded signal queued back
+    to back on one solver with nothing read in between: each starts while
+    launches the previous one queued past its stop are still on the stream
+    (they must neither report progress nor touch the loop state).  Every
+    solve reports the first one's iteration count, and the last leaves the
+    first one's iteration log and support."""
+    n, k, sigma = 1 << 22, 1000, 1e-7
+    x, _ = make_signal(n, k, sigma, seed=2, magnitude="loguniform")
+    x64 = x.astype(np.complex64)
+    sig_eff = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    s = gpu.Solver(n, gpu.solve_options(k, sig_eff, window_delta=1e-6), precision)
+    xd = s.upload(x64 if precision else x64.astype(np.complex128))
+    it0 = s.solve(xd, 11)
+    m0, b0 = s.modes_dict(), s.log()[0].tolist()
+    its = [s.solve(xd, 11) for _ in range(10)]
+    assert its == [it0] * 10
+    m1 = s.modes_dict()
+    assert s.log()[0].tolist() == b0
+    assert_coefs_close(m1, m0, 1e-9)
+
+
 def test_solve_trials3_prime_n(gpu, orc):
     n, k, sigma = 1048573, 100, 1e-6
     x, _ = make_signal(n, k, sigma, seed=3)
