@@ -293,6 +293,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the instrumented MIXED solve (for rocprofv3 runs)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the timed MIXED solves, as the metric runs them (for rocprofv3 "
+                         "kernel statistics of the solve loop)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
     ap.add_argument("--config3", action="store_true",
@@ -338,6 +341,13 @@ def main():
                           "launches": n_launch, "timed_repeats_per_iteration": 8,
                           "algorithmic_bytes_per_launch": line_bytes / n_launch,
                           "kernel_ms_per_solve": ms}), flush=True)
+        return
+    if args.solve_only:
+        xd64 = mixed.upload(x64)
+        M.synchronize()
+        us, it = time_solves(M, mixed, xd64, args.steps, args.warmup)
+        print(json.dumps({"us_per_solve": round(us, 2), "iterations": it, "steps": args.steps,
+                          "warmup": args.warmup}), flush=True)
         return
     fp64q = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
     fp64 = M.Solver(N, M.solve_options(K, SIGMA, window_delta=DELTA), M.MPSFT_FP64)
