@@ -820,6 +820,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
     // 0: three-launch iterations; 1: the next graph opens the late form
     // (the iterations launched so far have their identify); 2: late form.
     int late = 0;
+    const int32_t first_B = int32_t(std::max<int64_t>(2 * int64_t(opt.k) + 1, opt.min_bins));
     const auto stall = StallWait();
     auto last_change = std::chrono::steady_clock::now();
     int32_t seen = 0, synced = 0;
@@ -844,7 +845,9 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
         }
         // progress[2] is the B of the latest iteration the device started;
         // B never grows, so once it is min_bins every later iteration has it.
-        const int32_t bnow = h_progress_[2];
+        // Before the first report it is the first iteration's B, 2k+1 but at
+        // least min_bins (NextIteration with nothing found).
+        const int32_t bnow = h_progress_[2] > 0 ? h_progress_[2] : first_B;
         if (late == 0 && late_B_ > 0 && bnow == late_B_) late = 1;
         hipGraphExec_t ge = (graph_exec_[1] && bnow > 0 && bnow <= kMaxLdsBins) ? graph_exec_[1]
                                                                                  : graph_exec_[0];
