@@ -116,6 +116,7 @@ inline unsigned atomicOr(unsigned *p, unsigned v) {
 }
 #define __HIP_MEMORY_SCOPE_SYSTEM 0
 #define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), (order))
+#define __builtin_nontemporal_load(p) (*(p))
 
 // ---------------------------------------------------------------- runtime
 typedef int hipError_t;
