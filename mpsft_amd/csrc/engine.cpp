@@ -91,6 +91,18 @@ int32_t ChunkIters() {
 // changes, so launching further ahead commits no graph choice early, and
 // each iteration is short enough that two queued ones do not cover the
 // host's launch latency.
+// Iterations per late graph (even): once every launch is a late iteration the
+// graph kind no longer changes, so longer graphs cost no early commitment
+// and cross fewer inter-graph gaps.
+int32_t LateGraphIters() {
+  const char *e = std::getenv("MPSFT_LATE_GRAPH");
+  if (e) {
+    const int v = std::atoi(e);
+    if (v >= 2 && v <= 64 && v % 2 == 0) return v;
+  }
+  return 6;
+}
+
 int32_t LateLookahead() {
   const char *e = std::getenv("MPSFT_LATE_LOOKAHEAD");
   if (e) {
@@ -589,8 +601,10 @@ int Engine::LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big) {
 // reports B <= kMaxLdsBins the host launches the shorter graph.
 int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
   const int want = (fft_big_ && fft_small_) ? 2 : 1;
+  const int32_t late_iters = LateGraphIters();
   if (graph_exec_[0] && graph_key_ == shape_gen_ && graph_iters_ == iters &&
-      (want == 1 || graph_exec_[1]) && graph_late_B_ == late_B_)
+      (want == 1 || graph_exec_[1]) && graph_late_B_ == late_B_ &&
+      (late_B_ == 0 || graph_late_iters_ == late_iters))
     return MPSFT_OK;
   Drain();  // a graph may still run the launches a solve queued past its stop
   for (auto &e : graph_exec_) {
@@ -635,7 +649,7 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
         return MPSFT_ERR_HIP;
       }
       hipError_t el = hipSuccess;
-      for (int32_t i = 0; i < iters && el == hipSuccess; ++i)
+      for (int32_t i = 0; i < late_iters && el == hipSuccess; ++i)
         el = LaunchLateIteration(c, i & 1, nb_ident, tiles, LdsBinTime(c.mixed != 0), LdsFft(),
                                  (k == 3 || i > 0) ? 1 : 0, cs);
       const hipError_t ec = hipStreamEndCapture(cs, &g);
@@ -650,6 +664,7 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
     }
   }
   graph_late_B_ = late_B_;
+  graph_late_iters_ = late_iters;
   graph_iters_ = iters;
   graph_key_ = shape_gen_;
   (void)s;
@@ -832,7 +847,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
       // stops the loop (logging the last iteration).
       if (h_progress_[1] || launched > max_iter) break;
       if (launched - progressed < (late == 2 ? late_lookahead : lookahead)) {
-        const int32_t need = std::min(max_iter, launched + 3);
+        const int32_t need = std::min(max_iter, launched + graph_late_iters_ + 1);
         if (need > tf_count_) {
           // Draw ahead in doubling batches.  Growing the arrays moves them,
           // and the graph holds their addresses: drain the stream first, so
@@ -858,7 +873,7 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
           ge = graph_exec_[3];
         }
         if (hipGraphLaunch(ge, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
-        launched += 2;
+        launched += late == 2 ? graph_late_iters_ : 2;
         continue;
       }
       if (progressed != seen) {

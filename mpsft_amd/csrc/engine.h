@@ -291,6 +291,7 @@ class Engine {
   int32_t graph_iters_ = 0;
   int64_t graph_key_ = -1;
   int32_t late_B_ = 0, graph_late_B_ = 0;  // min_bins of the solve (0: no late graphs)
+  int32_t graph_late_iters_ = 2;            // iterations per late graph
   int64_t shape_gen_ = 0;
   int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
 
