@@ -1627,7 +1627,7 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   const int32_t cnt = min(st->count, c.cap);
   const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
   const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
-  // prefetch: factor, bin and key of the first modes (coefficients after)
+  // prefetch: factor, bin and key of the first modes
   double2 fac[kPerTerm];
   int tbin[kPerTerm];
   int64_t kk[kPerTerm];
@@ -1646,6 +1646,11 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   const int64_t q = ip.q[trial];
   const int64_t tau = ModN(q + RowOffset(row) * B, n, mask);
   const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
+  // their coefficients, final since the previous launch (identify of the
+  // previous iteration), load while the DFT runs
+  double2 cfv[kPerTerm];
+#pragma unroll
+  for (int r = 0; r < kPerTerm; ++r) cfv[r] = kk[r] >= 0 ? c.coef[kk[r]] : make_double2(0.0, 0.0);
   double2 *a = lds;
   const bool small = ip.logM <= kFftSmallLogM;
   if (small)
@@ -1660,10 +1665,10 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
       double2 v;
       int32_t bin = tbin[r];
       if (i < cnt_pre) {
-        v = Cmul(c.coef[kk[r]], fac[r]);
+        v = Cmul(cfv[r], fac[r]);
       } else {
         const int64_t k1 = ModN(ip.tf.a * kk[r] + ip.tf.b, n, mask);
-        v = Cmul(c.coef[kk[r]], RowFactor(c, ip, q, row, kk[r], k1, &bin));
+        v = Cmul(cfv[r], RowFactor(c, ip, q, row, kk[r], k1, &bin));
       }
       const int j = small ? bin : Pad(bin);
       atomicAdd(&a[j].x, -v.x);
