@@ -1568,7 +1568,7 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   HIP_DYNAMIC_SHARED(double2, lds)
   constexpr int T = kFftThreads;
-  constexpr int kPerTerm = 2;  // factors prefetched per thread
+  constexpr int kPerTerm = 4;  // factors prefetched per thread (S up to 1024 at once)
   DevState *st = c.st;
   // Loop control of this iteration (NextIteration's; identify of the
   // previous iteration has finished).
