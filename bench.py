@@ -108,15 +108,6 @@ def large_workload(M, steps, warmup):
     }
 
 
-class _DevPtr:
-    """A torch cuda tensor seen through the C-ABI's device-pointer argument."""
-
-    def __init__(self, t):
-        import ctypes
-        self.t = t
-        self.ptr = ctypes.c_void_p(t.data_ptr())
-
-
 def batch_workload(M, steps, warmup, count=256):
     """Batched uneven instances: `count` complex64 signals, n = 2^k with k
     uniform in [16, 22], S uniform in [10, 500], unit coefficients with
@@ -134,7 +125,7 @@ def batch_workload(M, steps, warmup, count=256):
     xs, opts, truths = [], [], []
     for i, (n, k) in enumerate(zip(ns, ks)):
         x, sq, truth = make_signal_device(n, k, sigma, seed=7000 + i)
-        xs.append(_DevPtr(x))
+        xs.append(x)
         truths.append(truth)
         opts.append(M.solve_options(k, float(np.hypot(sigma, sq)), window_delta=DELTA))
     torch.cuda.synchronize()

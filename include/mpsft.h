@@ -81,6 +81,11 @@ int32_t mpsft_num_bits(int64_t n, int32_t bins);
 /* Window::p() (src/window.cc:40-50); 0 on invalid arguments. */
 int32_t mpsft_window_p(int64_t n, int32_t bins, double delta);
 
+/* The delta of Window(n, bins, delta) recovered from its p taps wt
+ * (Window::wt(), src/window.h:36); MPSFT_ERR_INVALID_ARG when the taps are
+ * not such a window's.  Host only. */
+int mpsft_window_delta(int64_t n, int32_t bins, int32_t p, const double *wt, double *delta);
+
 /* ---- plan: one window + binner pair ----
  * Replaces Window(n, bins, delta) (src/window.cc:28) together with
  * BinInTime::Create / BinInFreq::Create (src/binner.cc:58-76, 526-535).
@@ -89,6 +94,18 @@ typedef struct mpsft_plan mpsft_plan;
 int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits,
                       int precision, mpsft_plan **plan);
 int mpsft_plan_destroy(mpsft_plan *plan);
+
+/* A plan from a Window the caller already holds (src/window.h:25-55), which
+ * exposes n(), bins(), p() and its taps wt() but keeps delta private:
+ * wt[i] for i < p, t = i for i <= (p-1)/2 and i - p above (Window::t()).
+ * BinInTime then uses these taps exactly; delta is recovered from them (the
+ * taps' Gaussian factor gives sigma_f, src/window.cc:35-37) for the window's
+ * frequency response in BinInFreq and the identify step.  Taps that are not
+ * a Window(n, bins, delta)'s return MPSFT_ERR_INVALID_ARG. */
+int mpsft_plan_create_window(int64_t n, int32_t bins, int32_t p, const double *wt,
+                             int32_t bits, int precision, mpsft_plan **plan);
+/* The plan's window delta (as given, or as recovered from the taps). */
+int mpsft_plan_delta(const mpsft_plan *plan, double *delta);
 
 /* BinInTime::Run (src/binner.h:33-34; V0 definition src/binner.cc:81-111,
  * default V4 src/binner.cc:394-521).  x_dev: n complex (double2 for FP64,
@@ -105,6 +122,27 @@ int mpsft_bin_in_freq(mpsft_plan *plan, const int64_t *k0_dev,
                       const void *coef_dev, int64_t m,
                       const mpsft_transform *tf, int64_t q, void *out_dev,
                       void *stream);
+
+/* Iterate's per-bin body (src/iterate.cc:113-175: bin energy test,
+ * IdentifyFreq with MatPencil votes, src/iterate.cc:44-76, k1 rounding,
+ * window test, coefficient estimate and un-permutation) over a caller's
+ * bin matrices, one bin per wavefront.  a_dev: trials x (1+2*bits) x bins
+ * double2, row-major, after BinInFreq (src/iterate.cc:109-110); qs: the
+ * trials' offsets q (host); tf: the iteration's Transform.  Per bin b
+ * (device arrays of `bins` entries): decide_dev[b] = 0 below the bin
+ * threshold, 1 rejected by the sv threshold, 2 rejected by the window
+ * threshold, 3 a mode found; k1_dev[b] the rounded permuted location (-1
+ * when not reached); k0_dev[b], coef_dev[b] (double2) the found mode's
+ * location and coefficient (-1 and 0 otherwise); margin_dev[b] (may be
+ * NULL) the smallest relative distance of the bin's decisions from their
+ * thresholds.  A threshold <= 0 turns its test off, as in the reference;
+ * window_threshold must be > 0 (src/iterate.cc:84). */
+int mpsft_identify_estimate(mpsft_plan *plan, const void *a_dev, int32_t trials,
+                            const mpsft_transform *tf, const int64_t *qs,
+                            double bin_threshold, double sv_threshold,
+                            double window_threshold, int32_t *decide_dev,
+                            int64_t *k1_dev, int64_t *k0_dev, void *coef_dev,
+                            double *margin_dev, void *stream);
 
 /* MatPencil (src/freqid.cc:27-48), batched on the device: for i < m,
  * u_dev[3i..3i+2] = (u0, u1, u2) double2; writes sigma_dev[2i..2i+1] and
@@ -176,12 +214,25 @@ int mpsft_iterate(mpsft_solver *s, const void *x_dev,
                   const mpsft_iterate_options *opt, const mpsft_transform *tf,
                   const int64_t *qs, int32_t *found, void *stream);
 
-/* Found-mode map access (host memory).  set replaces the map. */
+/* Found-mode map access (host memory).  set replaces the map; keys are
+ * reduced modulo n, and two keys equal modulo n are refused with
+ * MPSFT_ERR_INVALID_ARG (a ModeMap, src/base.h:35, holds a key once). */
 int mpsft_solver_modes_count(mpsft_solver *s, int64_t *count);
 int mpsft_solver_modes_get(mpsft_solver *s, int64_t *keys, double *coefs,
                            int64_t cap, int64_t *count);
 int mpsft_solver_modes_set(mpsft_solver *s, const int64_t *keys,
                            const double *coefs, int64_t m);
+/* PostAnalyze (src/demo1.cc:61-112) of the last solve on the device: the
+ * found modes against the true locations truth_keys_dev[0..m) (device
+ * int64, ascending; MPSFT_ERR_INVALID_ARG otherwise) and the signal's
+ * x-hat (device, n double2, the xh_ of Demo1, src/demo1.cc:30-33).  stats
+ * (host, 9 doubles): hit count, hit mean and rms |c - xhat|, miss count,
+ * miss mean and rms, then over all n locations the mean, rms and max of
+ * |found - xhat| (found = 0 where nothing was found).  A kind with no
+ * entries gives 0/0 (NaN) for its mean and rms, as the reference prints. */
+int mpsft_solver_post_analyze(mpsft_solver *s, const int64_t *truth_keys_dev, int64_t m,
+                              const void *xhat_dev, double *stats, void *stream);
+
 /* After an mpsft_iterate: copies the last iteration's bin matrices
  * (trials x rows x bins double2) into host memory. */
 int mpsft_solver_bins_get(mpsft_solver *s, double *out, int64_t cap_doubles,

@@ -9,12 +9,12 @@ from ._lib import (MPSFT_FP64, MPSFT_MIXED, EXPORTS, LIB_PATH, IterateOptions, M
                    SolveOptions, Transform, lib)
 from .api import (BIN_TYPE_FP64, BIN_TYPE_MIXED, Batch, BinInFreq, BinInTime, DeviceArray, Plan,
                   Solver, Window, device_count, iterate_options, mat_pencil, num_bits,
-                  solve_options, synchronize, transform_make)
+                  solve_options, synchronize, transform_make, window_delta)
 
 __all__ = [
     "MPSFT_FP64", "MPSFT_MIXED", "EXPORTS", "LIB_PATH", "IterateOptions", "MpsftError",
     "SolveOptions", "Transform", "lib", "BIN_TYPE_FP64", "BIN_TYPE_MIXED", "Batch", "BinInFreq",
     "BinInTime", "DeviceArray", "Plan", "Solver", "Window", "device_count",
     "iterate_options", "mat_pencil", "num_bits", "solve_options", "synchronize",
-    "transform_make",
+    "transform_make", "window_delta",
 ]

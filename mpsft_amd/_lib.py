@@ -24,10 +24,12 @@ EXPORTS = [
     "mpsft_free", "mpsft_memcpy_htod", "mpsft_memcpy_dtoh", "mpsft_memset",
     "mpsft_stream_create", "mpsft_stream_destroy", "mpsft_stream_synchronize",
     "mpsft_device_synchronize", "mpsft_transform_make", "mpsft_num_bits",
-    "mpsft_window_p", "mpsft_plan_create", "mpsft_plan_destroy", "mpsft_bin_in_time",
-    "mpsft_bin_in_freq", "mpsft_mat_pencil", "mpsft_solver_create",
+    "mpsft_window_p", "mpsft_window_delta", "mpsft_plan_create", "mpsft_plan_create_window",
+    "mpsft_plan_delta", "mpsft_plan_destroy", "mpsft_bin_in_time",
+    "mpsft_bin_in_freq", "mpsft_identify_estimate", "mpsft_mat_pencil", "mpsft_solver_create",
     "mpsft_solver_destroy", "mpsft_solve", "mpsft_solve_profiled", "mpsft_iterate", "mpsft_solver_modes_count",
-    "mpsft_solver_modes_get", "mpsft_solver_modes_set", "mpsft_solver_bins_get",
+    "mpsft_solver_modes_get", "mpsft_solver_modes_set", "mpsft_solver_post_analyze",
+    "mpsft_solver_bins_get",
     "mpsft_solver_log_get", "mpsft_solver_transforms_get", "mpsft_solver_set_tile_residues",
     "mpsft_solver_set_record", "mpsft_solver_info", "mpsft_solver_replay",
     "mpsft_solver_adjoint",
@@ -110,7 +112,13 @@ def declare(L):
         "mpsft_transform_make": (C.c_int, [i64, i64, i64, i64, C.POINTER(Transform)]),
         "mpsft_num_bits": (i32, [i64, i32]),
         "mpsft_window_p": (i32, [i64, i32, f64]),
+        "mpsft_window_delta": (C.c_int, [i64, i32, i32, pf64, pf64]),
         "mpsft_plan_create": (C.c_int, [i64, i32, f64, i32, C.c_int, C.POINTER(vp)]),
+        "mpsft_plan_create_window": (C.c_int, [i64, i32, i32, pf64, i32, C.c_int,
+                                               C.POINTER(vp)]),
+        "mpsft_plan_delta": (C.c_int, [vp, pf64]),
+        "mpsft_identify_estimate": (C.c_int, [vp, vp, i32, C.POINTER(Transform), pi64, f64, f64,
+                                              f64, vp, vp, vp, vp, vp, vp]),
         "mpsft_plan_destroy": (C.c_int, [vp]),
         "mpsft_bin_in_time": (C.c_int, [vp, vp, i64, C.POINTER(Transform), i64, vp, vp]),
         "mpsft_bin_in_freq": (C.c_int, [vp, vp, vp, i64, C.POINTER(Transform), i64, vp, vp]),
@@ -125,6 +133,7 @@ def declare(L):
         "mpsft_solver_modes_count": (C.c_int, [vp, pi64]),
         "mpsft_solver_modes_get": (C.c_int, [vp, pi64, pf64, i64, pi64]),
         "mpsft_solver_modes_set": (C.c_int, [vp, pi64, pf64, i64]),
+        "mpsft_solver_post_analyze": (C.c_int, [vp, vp, i64, vp, pf64, vp]),
         "mpsft_solver_bins_get": (C.c_int, [vp, pf64, i64, pi32, pi32, pi32]),
         "mpsft_solver_log_get": (C.c_int, [vp, pi32, pi32, i32, pi32]),
         "mpsft_solver_transforms_get": (C.c_int, [vp, C.POINTER(Transform), pi64, i32, pi32]),

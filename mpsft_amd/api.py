@@ -125,16 +125,42 @@ def _as_device(x, precision):
     return DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=dt))
 
 
+def window_delta(n, bins, wt):
+    """The delta of Window(n, bins, delta) recovered from its taps wt
+    (Window::wt(), src/window.h:36)."""
+    wt = np.ascontiguousarray(wt, dtype=np.float64)
+    d = C.c_double()
+    check(lib().mpsft_window_delta(n, bins, wt.size, wt.ctypes.data_as(C.POINTER(C.c_double)),
+                                   C.byref(d)), "mpsft_window_delta")
+    return d.value
+
+
 class Plan:
     """mpsft_plan: one window + binner pair on the device."""
 
-    def __init__(self, n, bins, delta, bits, precision=MPSFT_FP64):
-        self.n, self.bins, self.delta, self.bits = int(n), int(bins), float(delta), int(bits)
+    def __init__(self, n, bins, delta, bits, precision=MPSFT_FP64, taps=None):
+        """delta: the window's delta; or taps: the p taps Window::wt() of a
+        window whose delta the caller does not hold (delta is then None)."""
+        self.n, self.bins, self.bits = int(n), int(bins), int(bits)
         self.rows = 1 + 2 * self.bits
         self.precision = precision
         self.h = C.c_void_p()
-        check(lib().mpsft_plan_create(n, bins, delta, bits, precision, C.byref(self.h)),
-              "mpsft_plan_create")
+        if taps is None:
+            check(lib().mpsft_plan_create(n, bins, delta, bits, precision, C.byref(self.h)),
+                  "mpsft_plan_create")
+        else:
+            self._taps = np.ascontiguousarray(taps, dtype=np.float64)
+            check(lib().mpsft_plan_create_window(
+                n, bins, self._taps.size, self._taps.ctypes.data_as(C.POINTER(C.c_double)), bits,
+                precision, C.byref(self.h)), "mpsft_plan_create_window")
+        d = C.c_double()
+        check(lib().mpsft_plan_delta(self.h, C.byref(d)), "mpsft_plan_delta")
+        self.delta = d.value
+
+    @classmethod
+    def from_window(cls, n, bins, taps, bits, precision=MPSFT_FP64):
+        """A plan from a Window's taps alone (mpsft_plan_create_window)."""
+        return cls(n, bins, None, bits, precision, taps=taps)
 
     def __del__(self):
         try:
@@ -171,6 +197,32 @@ class Plan:
         synchronize()
         out[...] = od.to_numpy(np.complex128, out.shape)
         return out
+
+
+    def identify_estimate(self, A, tf, qs, bin_threshold, sv_threshold, window_threshold):
+        """Iterate's per-bin body (src/iterate.cc:113-175) over bin matrices
+        A (trials, rows, bins) complex128.  Returns a dict of per-bin arrays:
+        decide (0 energy, 1 sv, 2 window, 3 found), k1, k0, coef, margin."""
+        A = np.ascontiguousarray(A, dtype=np.complex128)
+        trials = A.shape[0]
+        if A.shape != (trials, self.rows, self.bins):
+            raise MpsftError("identify_estimate: A shape", 2)
+        q = np.ascontiguousarray(qs, dtype=np.int64)
+        if q.size != trials:
+            raise MpsftError("identify_estimate: one q per trial", 2)
+        ad = DeviceArray.from_numpy(A)
+        outs = dict(decide=DeviceArray(4 * self.bins), k1=DeviceArray(8 * self.bins),
+                    k0=DeviceArray(8 * self.bins), coef=DeviceArray(16 * self.bins),
+                    margin=DeviceArray(8 * self.bins))
+        check(lib().mpsft_identify_estimate(
+            self.h, ad.ptr, trials, C.byref(tf), q.ctypes.data_as(C.POINTER(C.c_int64)),
+            bin_threshold, sv_threshold, window_threshold, outs["decide"].ptr, outs["k1"].ptr,
+            outs["k0"].ptr, outs["coef"].ptr, outs["margin"].ptr, None),
+            "mpsft_identify_estimate")
+        synchronize()
+        types = dict(decide=np.int32, k1=np.int64, k0=np.int64, coef=np.complex128,
+                     margin=np.float64)
+        return {k: d.to_numpy(types[k], (self.bins,)) for k, d in outs.items()}
 
 
 class BinInTime:
@@ -277,7 +329,7 @@ class Solver:
     def solve(self, x_dev, seed, stream=None):
         """Demo1::Run; returns the iteration count."""
         it = C.c_int32(0)
-        check(lib().mpsft_solve(self.h, x_dev.ptr, seed, C.byref(it), stream), "mpsft_solve")
+        check(lib().mpsft_solve(self.h, _ptr(x_dev), seed, C.byref(it), stream), "mpsft_solve")
         return it.value
 
     def solve_profiled(self, x_dev, seed, stream=None):
@@ -285,7 +337,7 @@ class Solver:
         (iterations, [ms gather-fold, ms bin FFT, ms identify])."""
         it = C.c_int32(0)
         ms = (C.c_double * 3)()
-        check(lib().mpsft_solve_profiled(self.h, x_dev.ptr, seed, C.byref(it), ms, stream),
+        check(lib().mpsft_solve_profiled(self.h, _ptr(x_dev), seed, C.byref(it), ms, stream),
               "mpsft_solve_profiled")
         return it.value, list(ms)
 
@@ -348,6 +400,21 @@ class Solver:
             self.h, keys.ctypes.data_as(C.POINTER(C.c_int64)),
             coefs.ctypes.data_as(C.POINTER(C.c_double)), keys.size), "modes_set")
 
+    def post_analyze(self, truth_keys, xhat):
+        """PostAnalyze (src/demo1.cc:61-112) of the last solve on the device.
+        truth_keys: the true locations; xhat: the signal's x-hat (n complex,
+        a host array or a device array / tensor of complex128).  Returns the
+        figures by name."""
+        keys = np.ascontiguousarray(np.sort(np.asarray(truth_keys, np.int64)))
+        kd = DeviceArray.from_numpy(keys) if keys.size else None
+        xd = xhat if not isinstance(xhat, np.ndarray) else \
+            DeviceArray.from_numpy(np.ascontiguousarray(xhat, np.complex128))
+        st = (C.c_double * 9)()
+        check(lib().mpsft_solver_post_analyze(self.h, _ptr(kd) if kd else None, keys.size,
+                                              _ptr(xd), st, None), "mpsft_solver_post_analyze")
+        names = ("hit", "hit_l1", "hit_l2", "miss", "miss_l1", "miss_l2", "l1", "l2", "max")
+        return dict(zip(names, list(st)))
+
     def bins_matrix(self, trials, rows, bins):
         out = np.zeros(2 * trials * rows * bins, np.float64)
         t, r, b = C.c_int32(), C.c_int32(), C.c_int32()
@@ -406,7 +473,7 @@ class Batch:
         return [_as_device(x, self.precision) for x in xs]
 
     def _ptrs(self, xds, seeds):
-        ptrs = (C.c_void_p * self.count)(*[d.ptr.value for d in xds])
+        ptrs = (C.c_void_p * self.count)(*[_ptr(d).value for d in xds])
         sd = (C.c_uint32 * self.count)(*[int(v) for v in seeds])
         return ptrs, sd
 

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpsft_hip.so")
 TORCH_LIB = os.path.join(HERE, "libmpsft_torch.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "harness.hip", "engine.cpp", "batch.cpp", "capi.cpp"]
 ARCH = os.environ.get("MPSFT_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions stay off (they are produced by hipcc -O3 from
@@ -91,6 +91,28 @@ def build_torch(verbose=False):
     return TORCH_LIB
 
 
+CLIENT_SRC = os.path.join(os.path.dirname(HERE), "tests", "capi_client.cpp")
+CLIENT = os.path.join(os.path.dirname(HERE), "tests", "capi_client")
+
+
+def build_capi_client(verbose=False):
+    """Compile tests/capi_client.cpp, a C++11 caller of the C-ABI, with g++
+    against include/mpsft.h only (no HIP headers) and link it to
+    libmpsft_hip.so, as a reference-side C++ caller would.  The GPU tests
+    run it.  Returns its path."""
+    out_tmp = CLIENT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++11", "-Wall", f"-I{INCLUDE}", CLIENT_SRC, f"-L{HERE}",
+           "-l:libmpsft_hip.so", "-Wl,-rpath,$ORIGIN/../mpsft_amd", "-o", out_tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on capi_client.cpp:\n{r.stderr[-4000:]}")
+    os.replace(out_tmp, CLIENT)
+    return CLIENT
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_torch(verbose=True))
+    print(build_capi_client(verbose=True))

@@ -101,7 +101,6 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
     if ((e.precision_ == MPSFT_MIXED) != (precision_ == MPSFT_MIXED)) return MPSFT_ERR_INVALID_ARG;
     ctxs[i] = e.MakeCtx();
     ctxs[i].batch_done = d_counters_ + 1;
-    ctxs[i].batch_host = d_host_;
   }
   HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
   HIP_OK(hipMemcpyAsync(d_args_, args_.data(), args_.size() * sizeof(InitArgs),
@@ -226,7 +225,6 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
           if (rc != MPSFT_OK) return fail(rc);
           ctxs[i] = eng_[i]->MakeCtx();
           ctxs[i].batch_done = d_counters_ + 1;
-          ctxs[i].batch_host = d_host_;
         }
         if (grow &&
             hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
@@ -340,7 +338,6 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   for (int32_t i = 0; i < K; ++i) {
     ctxs[i] = eng_[i]->MakeCtx();
     ctxs[i].batch_done = d_counters_ + 1;
-    ctxs[i].batch_host = d_host_;
   }
   HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
   std::vector<AdjSig> sig(K);

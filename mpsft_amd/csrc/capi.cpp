@@ -139,6 +139,33 @@ int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits, int p
   return MPSFT_OK;
 }
 
+int mpsft_window_delta(int64_t n, int32_t bins, int32_t p, const double *wt, double *delta) {
+  if (!delta) return MPSFT_ERR_INVALID_ARG;
+  return mpsft::WindowDeltaFromTaps(n, bins, p, wt, delta) ? MPSFT_OK : MPSFT_ERR_INVALID_ARG;
+}
+
+int mpsft_plan_create_window(int64_t n, int32_t bins, int32_t p, const double *wt, int32_t bits,
+                             int precision, mpsft_plan **plan) {
+  if (!plan || !wt) return MPSFT_ERR_INVALID_ARG;
+  *plan = nullptr;
+  double delta = 0;
+  if (!mpsft::WindowDeltaFromTaps(n, bins, p, wt, &delta)) return MPSFT_ERR_INVALID_ARG;
+  int rc = mpsft_plan_create(n, bins, delta, bits, precision, plan);
+  if (rc != MPSFT_OK) return rc;
+  rc = (*plan)->eng->SetWindowTaps(bins, p, wt);
+  if (rc != MPSFT_OK) {
+    mpsft_plan_destroy(*plan);
+    *plan = nullptr;
+  }
+  return rc;
+}
+
+int mpsft_plan_delta(const mpsft_plan *plan, double *delta) {
+  if (!plan || !delta) return MPSFT_ERR_INVALID_ARG;
+  *delta = plan->delta;
+  return MPSFT_OK;
+}
+
 int mpsft_plan_destroy(mpsft_plan *plan) {
   if (!plan) return MPSFT_ERR_INVALID_ARG;
   delete plan->eng;
@@ -167,6 +194,20 @@ int mpsft_bin_in_freq(mpsft_plan *plan, const int64_t *k0_dev, const void *coef_
   return Hip(mpsft::LaunchBinInFreqDirect(n, mask, plan->bp, Reduce(n, tf), qq, k0_dev,
                                           static_cast<const double2 *>(coef_dev), m,
                                           static_cast<double2 *>(out_dev), S(stream)));
+}
+
+int mpsft_identify_estimate(mpsft_plan *plan, const void *a_dev, int32_t trials,
+                            const mpsft_transform *tf, const int64_t *qs, double bin_threshold,
+                            double sv_threshold, double window_threshold, int32_t *decide_dev,
+                            int64_t *k1_dev, int64_t *k0_dev, void *coef_dev, double *margin_dev,
+                            void *stream) {
+  if (!plan || !a_dev || !qs || !decide_dev || !k1_dev || !k0_dev || !coef_dev)
+    return MPSFT_ERR_INVALID_ARG;
+  if (!TfOk(plan->n, tf)) return MPSFT_ERR_INVALID_ARG;
+  mpsft::IdOut out{decide_dev, k1_dev, k0_dev, static_cast<double2 *>(coef_dev), margin_dev};
+  return plan->eng->IdentifyOnly(static_cast<const double2 *>(a_dev), trials, plan->bins,
+                                 plan->delta, plan->bits, Reduce(plan->n, tf), qs, bin_threshold,
+                                 sv_threshold, window_threshold, out, S(stream));
 }
 
 int mpsft_mat_pencil(const void *u_dev, int64_t m, double *sigma_dev, uint8_t *vote_dev,
@@ -222,6 +263,13 @@ int mpsft_iterate(mpsft_solver *s, const void *x_dev, const mpsft_iterate_option
   if (!s || !x_dev || !opt || !qs || !found) return MPSFT_ERR_INVALID_ARG;
   if (!TfOk(s->eng->n(), tf)) return MPSFT_ERR_INVALID_ARG;
   return s->eng->IterateOnce(x_dev, *opt, Reduce(s->eng->n(), tf), qs, found, S(stream));
+}
+
+int mpsft_solver_post_analyze(mpsft_solver *s, const int64_t *truth_keys_dev, int64_t m,
+                              const void *xhat_dev, double *stats, void *stream) {
+  if (!s) return MPSFT_ERR_INVALID_ARG;
+  return s->eng->PostAnalyze(truth_keys_dev, m, static_cast<const double2 *>(xhat_dev), stats,
+                             S(stream));
 }
 
 int mpsft_solver_modes_count(mpsft_solver *s, int64_t *count) {

@@ -164,6 +164,53 @@ bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp) {
   return true;
 }
 
+// Window(n, bins, delta) exposes its taps but not delta (src/window.h:25-55).
+// Its taps are wt_t = w sinc(pi t w) e^{-2 (pi t sigma_f)^2} with w = 1/(2
+// bins) (src/window.cc:71-75), so each tap away from the sinc's zeros gives
+// sigma_f^2 = -ln(wt_t / (w sinc(pi t w))) / (2 pi^2 t^2), and delta follows
+// from sigma_f = 1/(4 sqrt(2) bins sqrt(-ln delta)) (src/window.cc:35-37).
+// The taps whose Gaussian factor lies in [1e-8, 0.5] are averaged (each
+// carries a relative error of about 1e-16 / |ln factor|).  The window of the
+// recovered delta must have p taps and reproduce every tap to 1e-9 of w.
+bool WindowDeltaFromTaps(int64_t n, int32_t bins, int32_t p, const double *wt, double *delta) {
+  if (!wt || bins <= 0 || bins % 2 != 1 || p < bins || p % bins != 0 || (p / bins) % 2 != 1)
+    return false;
+  const double w = 1.0 / (2.0 * bins);
+  if (!(std::fabs(wt[0] - w) <= 1e-12 * w)) return false;
+  const int32_t p2 = (p - 1) / 2;
+  // Tens of thousands of nearly equal estimates: summed with Neumaier's
+  // compensation (a plain running sum drifts by ~1e-12 relative here).
+  double sum = 0, comp = 0;
+  int32_t used = 0;
+  for (int32_t t = 1; t <= p2; ++t) {
+    const double s = std::sin(kPi * t * w);
+    if (std::fabs(s) < 0.05) continue;  // near a zero of the sinc
+    const double g = wt[t] / (w * s / (kPi * t * w));
+    if (!(g >= 1e-8 && g <= 0.5)) continue;
+    const double e = -std::log(g) / (2.0 * kPi * kPi * double(t) * double(t));
+    const double nsum = sum + e;
+    comp += std::fabs(sum) >= std::fabs(e) ? (sum - nsum) + e : (e - nsum) + sum;
+    sum = nsum;
+    ++used;
+  }
+  if (used == 0 || !(sum > 0)) return false;
+  const double sigma_f2 = (sum + comp) / used;
+  const double c = 1.0 / (32.0 * double(bins) * double(bins) * sigma_f2);
+  const double d = std::exp(-c);
+  BParams bp;
+  if (!(d > 0 && d < 1) || !WindowParams(n, bins, d, &bp) || bp.P != p) return false;
+  // every tap of the recovered window against the given ones
+  for (int32_t i = 1; i < p; ++i) {
+    const int32_t t = i <= p2 ? i : i - p;
+    const double x = kPi * std::fabs(double(t)) * w;
+    const double u = kPi * double(t) * bp.sigma_f;
+    const double tap = w * (std::sin(x) / x) * std::exp(-2.0 * u * u);
+    if (!(std::fabs(tap - wt[i]) <= 1e-9 * w)) return false;
+  }
+  *delta = d;
+  return true;
+}
+
 // Inverse modulo n by extended Euclid; 0 when a is not a unit.
 int64_t InvMod(int64_t a, int64_t n) {
   int64_t r0 = n, r1 = PosMod(a, n);
@@ -264,6 +311,9 @@ Engine::~Engine() {
   (void)hipFree(d_log_);
   (void)hipFree(d_tmp_);
   (void)hipFree(d_tmpk_);
+  (void)hipFree(d_wtab_);
+  (void)hipFree(d_idout_);
+  (void)hipFree(d_pa_);
   if (h_state_) (void)hipHostFree(h_state_);
   if (h_progress_) (void)hipHostFree(const_cast<int32_t *>(h_progress_));
 }
@@ -562,7 +612,8 @@ Ctx Engine::MakeCtx() const {
   c.big_gpr = fft_big_ ? big_M_cap_ / kBigTile : 0;
   c.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
   c.batch_done = nullptr;
-  c.batch_host = nullptr;
+  c.idout = nullptr;
+  c.wtab = nullptr;
   // The recording's rows are rec_B_cap_ apart; a context of another B_cap
   // (a later Reserve) must not index them.
   c.rec = ((record_ || rec_valid_) && rec_B_cap_ == B_cap_) ? d_rec_ : nullptr;
@@ -764,9 +815,9 @@ int Engine::Solve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
   const int32_t max_iter = a.max_iter;
   const int32_t chunk = ChunkIters();
   Ctx c = MakeCtx();
-  h_progress_[0] = 0;  // the previous solve has finished; the device writes
-  h_progress_[1] = 0;  // these again only after k_init below
-  h_progress_[2] = 0;
+  // The previous solve has finished; the device writes these again only
+  // after k_reset below (which clears them too).
+  for (int i = 0; i < 6; ++i) h_progress_[i] = 0;
   HIP_OK(LaunchReset(c, a, s));
 
   int32_t launched = 0;
@@ -1108,7 +1159,9 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   if (rc != MPSFT_OK) return rc;
   rc = Reserve(1, bs);
   if (rc != MPSFT_OK) return rc;
-  if (B_cap_ != B || rows_cap_ != 1 + 2 * bits || trials_cap_ != 1) return MPSFT_ERR_SHAPE;
+  // out's layout is rows x B: the engine's row stride must be B (a larger
+  // trials_cap_ from an earlier IdentifyOnly leaves trial 0's rows alone)
+  if (B_cap_ != B || rows_cap_ != 1 + 2 * bits) return MPSFT_ERR_SHAPE;
   bool grown = false;
   rc = GrowTransforms(1, &grown);
   if (rc != MPSFT_OK) return rc;
@@ -1118,6 +1171,7 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   HIP_OK(hipMemcpyAsync(d_qs_, &hq, sizeof(int64_t), hipMemcpyHostToDevice, s));
   A_override_ = out;
   Ctx c = MakeCtx();
+  if (d_wtab_ && wtab_B_ == B) c.wtab = d_wtab_;
   InitArgs a{};
   a.x = x;
   a.mode = kModeBinOnly;
@@ -1132,6 +1186,96 @@ int Engine::BinOnly(const void *x, int32_t B, double delta, int32_t bits, const 
   rc = LaunchFft(c, 0, s);
   A_override_ = nullptr;
   if (rc != MPSFT_OK) return rc;
+  return MPSFT_OK;
+}
+
+int Engine::SetWindowTaps(int32_t B, int32_t P, const double *wt_host) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!wt_host || P <= 0) return MPSFT_ERR_INVALID_ARG;
+  Drain();
+  if (Alloc(&d_wtab_, size_t(P)) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemcpy(d_wtab_, wt_host, size_t(P) * sizeof(double), hipMemcpyHostToDevice));
+  wtab_B_ = B;
+  return MPSFT_OK;
+}
+
+int Engine::IdentifyOnly(const double2 *A, int32_t trials, int32_t B, double delta, int32_t bits,
+                         const DevTf &tf, const int64_t *qs, double bin_thr, double sv_thr,
+                         double win_thr, const IdOut &out, hipStream_t s) {
+  Drain();
+  rec_valid_ = false;
+  if (status_ != MPSFT_OK) return status_;
+  if (!A || !qs || trials <= 0 || trials % 2 != 1 || trials > kMaxTrials || bits < 1 ||
+      bits > 30 || !(win_thr > 0) || !out.decide || !out.k1 || !out.k0 || !out.coef)
+    return MPSFT_ERR_INVALID_ARG;
+  std::vector<int32_t> bs{B};
+  int rc = EnsureTables(bs, delta, bits);
+  if (rc != MPSFT_OK) return rc;
+  rc = Reserve(trials, bs);
+  if (rc != MPSFT_OK) return rc;
+  // A's layout is [trial][row][bin] with rows x B per trial: the engine's
+  // strides must be exactly those.
+  if (B_cap_ != B || rows_cap_ != 1 + 2 * bits) return MPSFT_ERR_SHAPE;
+  bool grown = false;
+  rc = GrowTransforms(1, &grown);
+  if (rc != MPSFT_OK) return rc;
+  tf_valid_ = false;
+  std::vector<int64_t> hq(trials_cap_, 0);
+  for (int32_t t = 0; t < trials; ++t) hq[t] = PosMod(qs[t], n_);
+  if (!d_idout_ && Alloc(&d_idout_, 1) != MPSFT_OK) return MPSFT_ERR_NOMEM;
+  HIP_OK(hipMemcpyAsync(d_tfs_, &tf, sizeof(DevTf), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_qs_, hq.data(), hq.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIP_OK(hipMemcpyAsync(d_idout_, &out, sizeof(IdOut), hipMemcpyHostToDevice, s));
+  A_override_ = const_cast<double2 *>(A);
+  Ctx c = MakeCtx();
+  A_override_ = nullptr;
+  c.idout = d_idout_;
+  InitArgs a{};
+  a.x = A;  // not read: no gather runs
+  a.mode = kModeIterate;
+  a.B = B;
+  a.trials = trials;
+  a.max_stale = 1;
+  a.max_iter = 1;
+  a.bits_override = bits;
+  a.bin_thr = bin_thr;
+  a.sv_thr = sv_thr;
+  a.win_thr = win_thr;
+  HIP_OK(LaunchReset(c, a, s));
+  HIP_OK(LaunchPublish(c, 0, s));
+  HIP_OK(LaunchIdentify(c, 0, IdentifyBlocks(), s));
+  // the host copies of out and the q's are read by the copies above
+  HIP_OK(hipStreamSynchronize(s));
+  return MPSFT_OK;
+}
+
+// Workgroups of a PostAnalyze pass through the engine's own scratch.
+constexpr int32_t kPaBlocks = 512;
+
+int Engine::PostAnalyzeDev(const int64_t *truth, int64_t m, const double2 *xhat, double *partial,
+                           int32_t blocks, double *stats_dev, hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (m < 0 || (m > 0 && !truth) || !xhat || !partial || !stats_dev || blocks < 1)
+    return MPSFT_ERR_INVALID_ARG;
+  if (tail_stream_ != s) Drain();  // a solve's queued launches run before, in stream order
+  HIP_OK(LaunchPostAnalyze(MakeCtx(), truth, m, xhat, partial, blocks, stats_dev, s));
+  return MPSFT_OK;
+}
+
+int Engine::PostAnalyze(const int64_t *truth, int64_t m, const double2 *xhat, double *stats,
+                        hipStream_t s) {
+  if (status_ != MPSFT_OK) return status_;
+  if (!stats) return MPSFT_ERR_INVALID_ARG;
+  if (!d_pa_ && Alloc(&d_pa_, size_t(kPaBlocks + 1) * kPaFields) != MPSFT_OK)
+    return MPSFT_ERR_NOMEM;
+  double *dstats = d_pa_ + size_t(kPaBlocks) * kPaFields;
+  int rc = PostAnalyzeDev(truth, m, xhat, d_pa_, kPaBlocks, dstats, s);
+  if (rc != MPSFT_OK) return rc;
+  double h[kPaFields];
+  HIP_OK(hipMemcpyAsync(h, dstats, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (h[9] != 0) return MPSFT_ERR_INVALID_ARG;  // truth keys not ascending
+  for (int i = 0; i < 9; ++i) stats[i] = h[i];
   return MPSFT_OK;
 }
 
@@ -1162,6 +1306,12 @@ int Engine::ModesSet(const int64_t *keys, const double *coefs, int64_t m) {
   if (m < 0 || m > cap_ || (m > 0 && (!keys || !coefs))) return MPSFT_ERR_INVALID_ARG;
   std::vector<int64_t> hk(keys, keys + m);
   for (auto &k : hk) k = PosMod(k, n_);
+  // A ModeMap (src/base.h:35) holds each location once: two keys equal
+  // modulo n are refused rather than entered twice.
+  std::vector<int64_t> sorted(hk);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    return MPSFT_ERR_INVALID_ARG;
   Ctx c = MakeCtx();
   HIP_OK(LaunchClearModes(c, 0, nullptr));
   if (m > 0) {

@@ -70,6 +70,30 @@ struct DevTf {
 enum : int32_t { kModeSolve = 0, kModeIterate = 1, kModeBinOnly = 2, kModeReplay = 3 };
 enum : int32_t { kFlagOverflow = 1 };
 
+// Per-bin decisions of Iterate (src/iterate.cc:113-175): below the bin
+// energy threshold, rejected by the pencil's sv threshold, rejected by the
+// window threshold, or a mode found.
+enum : int32_t { kDecideEnergy = 0, kDecideSv = 1, kDecideWindow = 2, kDecideFound = 3 };
+
+// Where k_identify writes each bin's outcome instead of updating the
+// found-mode map (device memory, B entries each; margin may be null).
+struct IdOut {
+  int32_t *decide;
+  int64_t *k1;      // the rounded location, -1 when not reached
+  int64_t *k0;      // the un-permuted location of a found mode, else -1
+  double2 *coef;    // its coefficient estimate, else 0
+  double *margin;   // smallest relative distance of a decision from its threshold
+  __device__ void Record(int32_t b, int32_t d, int64_t k1v, double m) const {
+    decide[b] = d;
+    k1[b] = k1v;
+    if (margin) margin[b] = m;
+    if (d != kDecideFound) {
+      k0[b] = -1;
+      coef[b] = make_double2(0.0, 0.0);
+    }
+  }
+};
+
 // The parameters of one iteration.  Every k_bin_time workgroup derives them
 // from the previous iteration's slot and the counters k_identify left;
 // workgroup 0 publishes them in slot (iteration & 1) for the iteration's
@@ -139,7 +163,9 @@ struct Ctx {
   int64_t *rec;          // recorded solve: k1 of each (iteration, bin), -1 if none; or null
   int32_t rec_cap;       // iterations rec holds (B_cap entries each)
   int32_t *batch_done;   // batched form: device count of stopped signals, or null
-  int32_t *batch_host;   // batched form: host-mapped {iterations, stopped}, or null
+  const IdOut *idout;    // per-bin outputs of k_identify instead of map updates, or null
+  const double *wtab;    // the caller's window taps wt[i] (src/window.h:36), or null:
+                         // computed on the device
 };
 
 struct AdjSig;
@@ -162,6 +188,27 @@ class Engine {
   // BinInTime (bin_only) for a plan: out gets rows x B, row stride B.
   int BinOnly(const void *x, int32_t B, double delta, int32_t bits,
               const DevTf &tf, int64_t q, double2 *out, hipStream_t s);
+  // The per-bin body of Iterate (src/iterate.cc:113-175) on a caller's bin
+  // matrices A (trials x rows x B, row-major, rows = 1 + 2 bits): each bin's
+  // decision, k1, k0, coefficient and margin into out (device arrays).
+  int IdentifyOnly(const double2 *A, int32_t trials, int32_t B, double delta, int32_t bits,
+                   const DevTf &tf, const int64_t *qs, double bin_thr, double sv_thr,
+                   double win_thr, const IdOut &out, hipStream_t s);
+  // Window taps given by the caller (Window::wt(), src/window.h:36) for the
+  // window of B bins and P taps: BinInTime then reads them instead of
+  // computing them on the device.
+  int SetWindowTaps(int32_t B, int32_t P, const double *wt_host);
+
+  // PostAnalyze (src/demo1.cc:61-112) of the found modes against the true
+  // locations truth[0..m) (device, ascending) and x-hat (device, n double2):
+  // stats_dev gets 10 doubles (the nine figures, then 1 if truth was not
+  // ascending); partial: `blocks` x kPaFields doubles of scratch.  Writes
+  // device memory only (no allocation, no synchronisation).
+  int PostAnalyzeDev(const int64_t *truth, int64_t m, const double2 *xhat, double *partial,
+                     int32_t blocks, double *stats_dev, hipStream_t s);
+  // The same into host memory (9 figures), with scratch the engine holds.
+  int PostAnalyze(const int64_t *truth, int64_t m, const double2 *xhat, double *stats,
+                  hipStream_t s);
 
   int ModesCount(int64_t *count);
   int ModesGet(int64_t *keys, double *coefs, int64_t cap, int64_t *count);
@@ -266,6 +313,10 @@ class Engine {
   volatile int32_t *h_progress_ = nullptr;  // pinned, written by k_bin_time
   int32_t *d_progress_ = nullptr;           // device address of h_progress_
   double2 *A_override_ = nullptr;
+  double *d_wtab_ = nullptr;     // the caller's window taps (SetWindowTaps), or null
+  int32_t wtab_B_ = 0;           // the window they belong to
+  IdOut *d_idout_ = nullptr;     // IdentifyOnly's output pointers (device copy)
+  double *d_pa_ = nullptr;       // PostAnalyze scratch: partial sums, then the stats
   double2 *d_tmp_ = nullptr;     // cap double2 staging for mode I/O
   int64_t *d_tmpk_ = nullptr;    // cap int64 staging for mode I/O
 
@@ -334,6 +385,9 @@ class Engine {
 // Host-side helpers shared with the C-ABI.
 int32_t NumBits(int64_t n, int32_t bins);
 bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp);
+// The delta of a window given by its taps (Window::wt(), p() taps of
+// Window(n, bins, delta)); false when the taps are not such a window's.
+bool WindowDeltaFromTaps(int64_t n, int32_t bins, int32_t p, const double *wt, double *delta);
 int64_t InvMod(int64_t a, int64_t n);
 DevTf ReduceTf(int64_t n, int64_t a, int64_t b, int64_t c, int64_t a_inv);
 std::chrono::milliseconds StallWait();

@@ -38,6 +38,9 @@ hipError_t LaunchBinFft(const Ctx &c, int parity, size_t lds, hipStream_t s);
 // The four-step bin FFT (B > kMaxLdsBins): three kernels.
 hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s);
 hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s);
+// Publishes the next iteration's parameters in slot `parity` (k_bin_time's
+// first workgroup alone: no gather, no terms of found modes).
+hipError_t LaunchPublish(const Ctx &c, int parity, hipStream_t s);
 
 // The batched form: per-stage launches over every signal of a batch.
 struct BatchLaunch {
@@ -109,5 +112,12 @@ hipError_t LaunchBinInFreqDirect(int64_t n, int64_t mask, const BParams &bp,
                                  hipStream_t s);
 hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma,
                            uint8_t *vote, hipStream_t s);
+
+// PostAnalyze (harness.hip): `blocks` workgroups write their partial sums
+// (kPaFields doubles each) to partial, one more reduces them into
+// stats[0..9] (device memory).
+constexpr int kPaFields = 10;
+hipError_t LaunchPostAnalyze(const Ctx &c, const int64_t *truth, int64_t m, const double2 *xhat,
+                             double *partial, int32_t blocks, double *stats, hipStream_t s);
 
 }  // namespace mpsft

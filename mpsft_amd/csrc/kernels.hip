@@ -31,6 +31,16 @@ __device__ __forceinline__ int64_t ModN(int64_t v, int64_t n, int64_t mask) {
 __device__ __forceinline__ double2 Cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// a * b with each product rounded on its own (no fused multiply-add), as
+// the reference's complex<double> product rounds (src/freqid.cc:27-48 via
+// Eigen, restated by the oracle without contraction).  The pencil's vote
+// Im(conj(u0) u2 + conj(u1) u0) and its sigma_2 = |u0^2 - u1 u2| / sigma_1
+// cancel heavily on a bin holding one strong mode, and both are compared
+// against thresholds; Cmul (contracted) serves everything else.
+__device__ __forceinline__ double2 CmulRn(double2 a, double2 b) {
+#pragma clang fp contract(off)
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 __device__ __forceinline__ double2 Conj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double2 Add(double2 a, double2 b) {
   return make_double2(a.x + b.x, a.y + b.y);
@@ -334,11 +344,9 @@ __device__ __forceinline__ void InitBody(const Ctx &c, const InitArgs &a) {
   st->cnt_base[0] = st->cnt_base[1] = 0;
   st->P[1].B = a.B;
   st->P[1].done = 0;
-  if (c.progress) {
-    __hip_atomic_store(&c.progress[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&c.progress[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&c.progress[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
+  if (c.progress)
+    for (int i = 0; i < 6; ++i)
+      __hip_atomic_store(&c.progress[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void k_init(Ctx c, InitArgs a) {
@@ -400,6 +408,19 @@ __device__ void NextIteration(const Ctx &c, int p, IterParams *ip, LoopCounters 
   // their own trial, keeping the struct out of private memory.
 }
 
+// The solve's result in the host-mapped progress words: iterations, stale
+// count and flags, then the iteration and the stop flag, which the host
+// polls.  The release store orders the result words before the flag, so a
+// host that sees the flag reads this solve's result.
+__device__ __forceinline__ void PublishStop(const Ctx &c, int32_t it, int32_t stale,
+                                            int32_t flags) {
+  __hip_atomic_store(&c.progress[3], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&c.progress[4], stale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&c.progress[5], flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&c.progress[0], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&c.progress[1], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Workgroup 0 of k_bin_time: publish the iteration, log the previous one,
 // report progress to the host.
 __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
@@ -421,21 +442,17 @@ __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
   if (lc.done) {
     st->iters = lc.it;
     st->stop_stale = lc.stale;
-    if (c.batch_done) {  // a batch counts its stopped signals for the host
-      const int32_t d = atomicAdd(c.batch_done, 1) + 1;
-      __hip_atomic_store(&c.batch_host[1], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    // A batch counts its stopped signals; the next batched launch's first
+    // workgroup alone copies the count to the host.
+    if (c.batch_done) atomicAdd(c.batch_done, 1);
   }
   if (c.progress) {
     if (!lc.done) {
       __hip_atomic_store(&c.progress[2], ip.B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    } else {  // the solve's result, read by the host once the stream is idle
-      __hip_atomic_store(&c.progress[3], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[4], lc.stale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[5], st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+      PublishStop(c, lc.it, lc.stale, st->flags);
     }
-    __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&c.progress[1], lc.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -633,7 +650,9 @@ __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int
         const int ri = idx & (R - 1);
         if (ri < nrho) {
           const int64_t t = int64_t(k) * B + rho0 + ri - p2;
-          const double w = (t == 0) ? width : SampleInTime(t, width, sigma_f);
+          // the caller's tap table is indexed as Window::wt() (t < 0 at t + P)
+          const double w = c.wtab ? c.wtab[t >= 0 ? t : t + ip.P]
+                                  : (t == 0) ? width : SampleInTime(t, width, sigma_f);
           const int64_t r = ModN(tf.b * t, n, mask);
           const double d = delta * double(t);
           const double turns = double(r) / double(n) + (d - floor(d));
@@ -1323,6 +1342,11 @@ __device__ __forceinline__ double WaveSum(double v) {
   return v;
 }
 
+__device__ __forceinline__ double WaveMin(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
+}
+
 // Entry `src` of a column held as col[s] = entry (lane + 64 s).  Every lane
 // of the wave must call it (the shuffles read other lanes' registers).
 __device__ __forceinline__ double2 ColFetch(const double2 (&col)[kColSlices], int src) {
@@ -1363,6 +1387,10 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
 
   const bool replay = st->mode == kModeReplay;
   int64_t *rec = (c.rec && ip.it < c.rec_cap) ? c.rec + int64_t(ip.it) * c.B_cap : nullptr;
+  // With per-bin outputs (the identify-and-estimate entry point) every bin
+  // reports its decision, k1, k0, coefficient and decision margin, and the
+  // found-mode map is left alone.
+  const IdOut *const io = c.idout;
   for (int32_t b = blk * waves + (threadIdx.x >> 6); b < B; b += nblk * waves) {
     const double2 *Ab = c.A + b;
     double2 col[kColSlices];
@@ -1384,6 +1412,10 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
       wf = SampleInFreq(xi0, width, dfreq);
     } else {
     if (rec && lane == 0) rec[b] = -1;
+    // the smallest relative distance of a decision from its threshold
+    // (Iterate's comparisons, src/iterate.cc:121,134,142 and
+    // src/iterate.cc:58,62,68); only with per-bin outputs
+    double margin = 1e300;
     if (bin_thr > 0) {
       double e = 0;
       for (int t = 0; t < trials; ++t) {
@@ -1391,7 +1423,12 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
         e += v.x * v.x + v.y * v.y;
       }
       e /= trials;
-      if (e < bin_thr * bin_thr) continue;
+      const double thr2 = bin_thr * bin_thr;
+      if (io) margin = fmin(margin, fabs(e - thr2) / thr2);
+      if (e < thr2) {
+        if (io && lane == 0) io->Record(b, kDecideEnergy, -1, margin);
+        continue;
+      }
     }
     // IdentifyFreq: lane j < bits takes bit j (offset B 2^j).  MatPencil
     // (src/freqid.cc:27-48) in closed form: the vote is
@@ -1405,30 +1442,49 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
       const double2 u2 = ColFetch(col, t * rows + 2 * jb + 2);
       const double alpha = (u0.x * u0.x + u0.y * u0.y) + (u1.x * u1.x + u1.y * u1.y);
       const double gamma = (u2.x * u2.x + u2.y * u2.y) + (u0.x * u0.x + u0.y * u0.y);
-      const double2 beta = Add(Cmul(Conj(u0), u2), Cmul(Conj(u1), u0));
+      const double2 beta = Add(CmulRn(Conj(u0), u2), CmulRn(Conj(u1), u0));
       if (beta.y > 0) ++count;
+      double s2 = 0.0;
       if (sv_thr > 0) {
         const double hh = 0.5 * (alpha - gamma);
         const double lam1 =
             0.5 * (alpha + gamma) + sqrt(hh * hh + (beta.x * beta.x + beta.y * beta.y));
         const double s1 = sqrt(lam1);
-        const double2 det = Sub(Cmul(u0, u0), Cmul(u1, u2));
-        const double s2 = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
+        const double2 det = Sub(CmulRn(u0, u0), CmulRn(u1, u2));
+        s2 = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
         if (s2 * 0.5 > sv_thr) rej = true;
       }
+      if (io && lane < bits) {
+        const double ab = hypot(beta.x, beta.y);
+        if (ab > 0) margin = fmin(margin, fabs(beta.y) / ab);
+        if (sv_thr > 0) margin = fmin(margin, fabs(s2 * 0.5 - sv_thr) / sv_thr);
+      }
     }
+    if (io) margin = WaveMin(margin);
     const bool vote = lane < bits && count > trials / 2;
-    if (__any(lane < bits && rej)) continue;
+    if (__any(lane < bits && rej)) {
+      if (io && lane == 0) io->Record(b, kDecideSv, -1, margin);
+      continue;
+    }
     const uint64_t vm = __ballot(vote);
     int32_t xi_sum = 0;
     for (int j = 0; j < bits; ++j) xi_sum = (xi_sum << 1) | int32_t((vm >> j) & 1);
     const double xi1 = double(2 * xi_sum + 1) / double(1 << (bits + 1));
     // k1 = std::round(n (b + xi1) / B)  (round half away from zero).
-    k1 = int64_t(round(double(n) * (double(b) + xi1) / double(B)));
+    const double kr = double(n) * (double(b) + xi1) / double(B);
+    k1 = int64_t(round(kr));
     const double xi0 = double(k1) / double(n) - (double(b) + 0.5) / double(B);
     wf = SampleInFreq(xi0, width, dfreq);
-    if (fabs(wf) < win_thr) continue;
+    if (io) {
+      margin = fmin(margin, fabs(fabs(kr - floor(kr)) - 0.5));
+      margin = fmin(margin, fabs(fabs(wf) - win_thr) / win_thr);
+    }
+    if (fabs(wf) < win_thr) {
+      if (io && lane == 0) io->Record(b, kDecideWindow, k1, margin);
+      continue;
+    }
     if (rec && lane == 0) rec[b] = k1;
+    if (io && lane == 0) io->Record(b, kDecideFound, k1, margin);
     }
     // Coefficient estimate (src/iterate.cc:147-166): sum over trials and rows
     // of A * e^{-2 pi i (q + off*B) k1 / n}.
@@ -1459,7 +1515,12 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
       const int64_t k0 = ModN(tf.a_inv * ModN(k1 - tf.b, n, mask), n, mask);
       coef = Cmul(coef, Conj(CisFrac(ModN(tf.c * k0, n, mask), n)));
       coef = make_double2(coef.x / wf, coef.y / wf);
-      InsertMode(c, k0, coef, &ncnt, nkeys);
+      if (io) {
+        io->k0[b] = k0;
+        io->coef[b] = coef;
+      } else {
+        InsertMode(c, k0, coef, &ncnt, nkeys);
+      }
       nany = 1;
     }
   }
@@ -1606,13 +1667,7 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
       st->iters = lc.it;
       st->stop_stale = lc.stale;
     }
-    if (c.progress && lc.done) {
-      __hip_atomic_store(&c.progress[3], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[4], lc.stale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[5], st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[0], lc.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&c.progress[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    if (c.progress && lc.done) PublishStop(c, lc.it, lc.stale, st->flags);
   }
   if (done) return;
   const IterParams &ip = st->P[p];
@@ -1733,11 +1788,11 @@ __global__ void __launch_bounds__(kThreads)
   const double2 u0 = u[3 * i], u1 = u[3 * i + 1], u2 = u[3 * i + 2];
   const double alpha = (u0.x * u0.x + u0.y * u0.y) + (u1.x * u1.x + u1.y * u1.y);
   const double gamma = (u2.x * u2.x + u2.y * u2.y) + (u0.x * u0.x + u0.y * u0.y);
-  const double2 beta = Add(Cmul(Conj(u0), u2), Cmul(Conj(u1), u0));
+  const double2 beta = Add(CmulRn(Conj(u0), u2), CmulRn(Conj(u1), u0));
   const double hh = 0.5 * (alpha - gamma);
   const double lam1 = 0.5 * (alpha + gamma) + sqrt(hh * hh + (beta.x * beta.x + beta.y * beta.y));
   const double s1 = sqrt(lam1);
-  const double2 det = Sub(Cmul(u0, u0), Cmul(u1, u2));
+  const double2 det = Sub(CmulRn(u0, u0), CmulRn(u1, u2));
   sigma[2 * i] = s1;
   sigma[2 * i + 1] = s1 > 0 ? hypot(det.x, det.y) / s1 : 0.0;
   vote[i] = beta.y > 0 ? 1 : 0;
@@ -1766,7 +1821,12 @@ __global__ void __launch_bounds__(kThreads)
                      int32_t *host) {
   HIP_DYNAMIC_SHARED(double2, lds)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // The single writer of the host's copy: batched iterations started and
+    // signals stopped by the earlier launches (counters[1] only grows, and
+    // each launch publishes after the previous one has finished).
     const int32_t v = atomicAdd(&counters[0], 1) + 1;
+    const int32_t d = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&host[1], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[0], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   const int2 m = map[blockIdx.x];
@@ -2322,6 +2382,11 @@ hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32
                        lds_time, s, c, parity, nb_ident, with_identify);
   hipLaunchKernelGGL(k_late_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds_fft, s, c,
                      parity);
+  return hipGetLastError();
+}
+
+hipError_t LaunchPublish(const Ctx &c, int parity, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_time<double2>, dim3(1), dim3(kThreads), 0, s, c, parity);
   return hipGetLastError();
 }
 

@@ -14,8 +14,10 @@ import numpy as np
 import torch
 
 
-def make_signal(n, k, sigma, seed, magnitude="unit", clusters=None, dtype=np.complex128):
-    """Returns (x, xhat_modes) with x a length-n signal of k modes.
+def make_signal(n, k, sigma, seed, magnitude="unit", clusters=None, dtype=np.complex128,
+                with_xhat=False):
+    """Returns (x, xhat_modes) with x a length-n signal of k modes, and with
+    with_xhat also its x-hat (modes plus noise, complex128) last.
 
     magnitude: "unit" (|c| = 1) or "loguniform" (|c| = 10^U[-1, 1]).
     clusters: None, or (n_centres, fraction, halfwidth): that fraction of
@@ -49,6 +51,8 @@ def make_signal(n, k, sigma, seed, magnitude="unit", clusters=None, dtype=np.com
     scale = sigma / np.sqrt(2.0 * n)
     xh += (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * scale
     x = torch.fft.ifft(torch.from_numpy(xh)).numpy() * n  # unnormalised backward DFT
+    if with_xhat:
+        return x.astype(dtype), dict(zip(locs.tolist(), coef.tolist())), xh
     return x.astype(dtype), dict(zip(locs.tolist(), coef.tolist()))
 
 

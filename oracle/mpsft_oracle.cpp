@@ -841,9 +841,23 @@ int orc_iterate_fixed(const double *x, int32_t n, const orc_iterate_options *opt
   return found ? 1 : 0;
 }
 
-int32_t orc_demo1_run(const double *x, int32_t n, int32_t k, double sigma,
-                      const orc_demo1_options *opt, orc_modemap *found,
-                      int32_t *bins_log, int32_t *found_log, int32_t log_cap) {
+}  // extern "C"
+
+namespace {
+
+// The loop of Demo1::Run (src/demo1.cc:36-59) with its Iterate call given
+// as step(io, &res, &size): step runs one Iterate with the options io and
+// reports its return value and the map size after it.  Before iteration i:
+// bins = max((k - size)*2 + 1, min_bins) (src/demo1.cc:48-49), computed in
+// size_t there and converted to int32, which equals the signed value for
+// any size < 2^30; thresholds 3 sigma/sqrt(bins) and sigma/sqrt(bins)
+// (src/demo1.cc:50-51).  stale counts the iterations whose Iterate returned
+// false and is never reset (src/demo1.cc:55-57); the loop runs while
+// stale < max_stale_iter.  Both the real solve (orc_demo1_run*) and the
+// scripted schedule (orc_demo1_schedule) run this one loop.
+template <typename Step>
+int32_t Demo1Loop(int32_t k, double sigma, const orc_demo1_options *opt, int32_t start_size,
+                  orc_demo1_log *log, int32_t max_iter, Step step) {
   IterOpts io;
   io.window_delta = opt->window_delta;
   io.window_threshold = opt->window_threshold;
@@ -851,24 +865,68 @@ int32_t orc_demo1_run(const double *x, int32_t n, int32_t k, double sigma,
   io.bin_in_time_type = 1;
   io.bin_in_freq_type = 1;
   int32_t stale_iter = 0;
-  found->m.clear();
+  int32_t size = start_size;
   int32_t i = 0;
   for (; stale_iter < opt->max_stale_iter; ++i) {
-    // src/demo1.cc:48-51; (k - size)*2+1 is computed in size_t there and
-    // converted to int32, which equals the signed value for any size < 2^30.
-    io.bins = std::max<int32_t>(int32_t((int64_t(k) - int64_t(found->m.size())) * 2 + 1),
-                                opt->min_bins);
+    if (max_iter >= 0 && i >= max_iter) break;
+    io.bins = std::max<int32_t>(int32_t((int64_t(k) - int64_t(size)) * 2 + 1), opt->min_bins);
     io.bin_threshold = 3.0 * sigma / std::sqrt(double(io.bins));
     io.sv_threshold = 1.0 * sigma / std::sqrt(double(io.bins));
     bool res = false;
-    const int rc = IterateImpl(reinterpret_cast<const Cplex *>(x), n, io, nullptr,
-                               nullptr, &found->m, nullptr, &res);
+    const int rc = step(io, &res, &size);
     if (rc != ORC_OK) return -rc;
-    if (bins_log && i < log_cap) bins_log[i] = io.bins;
-    if (found_log && i < log_cap) found_log[i] = int32_t(found->m.size());
+    if (log && i < log->cap) {
+      if (log->bins) log->bins[i] = io.bins;
+      if (log->found) log->found[i] = size;
+      if (log->res) log->res[i] = res ? 1 : 0;
+      if (log->stale) log->stale[i] = stale_iter + (res ? 0 : 1);
+      if (log->bin_thr) log->bin_thr[i] = io.bin_threshold;
+      if (log->sv_thr) log->sv_thr[i] = io.sv_threshold;
+    }
     if (!res) ++stale_iter;
   }
   return i;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t orc_demo1_run_log(const double *x, int32_t n, int32_t k, double sigma,
+                          const orc_demo1_options *opt, orc_modemap *found,
+                          orc_demo1_log *log) {
+  found->m.clear();
+  return Demo1Loop(k, sigma, opt, 0, log, -1,
+                   [&](const IterOpts &io, bool *res, int32_t *size) {
+                     const int rc = IterateImpl(reinterpret_cast<const Cplex *>(x), n, io,
+                                                nullptr, nullptr, &found->m, nullptr, res);
+                     *size = int32_t(found->m.size());
+                     return rc;
+                   });
+}
+
+int32_t orc_demo1_run(const double *x, int32_t n, int32_t k, double sigma,
+                      const orc_demo1_options *opt, orc_modemap *found,
+                      int32_t *bins_log, int32_t *found_log, int32_t log_cap) {
+  orc_demo1_log log{};
+  log.bins = bins_log;
+  log.found = found_log;
+  log.cap = log_cap;
+  return orc_demo1_run_log(x, n, k, sigma, opt, found, &log);
+}
+
+int32_t orc_demo1_schedule(int32_t k, double sigma, const orc_demo1_options *opt,
+                           const uint8_t *script_res, const int32_t *script_size,
+                           int32_t script_len, orc_demo1_log *log) {
+  int32_t i = 0;
+  const int32_t it = Demo1Loop(k, sigma, opt, 0, log, script_len,
+                               [&](const IterOpts &, bool *res, int32_t *size) {
+                                 *res = script_res[i] != 0;
+                                 *size = script_size[i];
+                                 ++i;
+                                 return ORC_OK;
+                               });
+  return it;
 }
 
 int orc_generate_mode_map(int32_t n, int32_t k, orc_modemap *mm) {

@@ -170,6 +170,32 @@ int32_t orc_demo1_run(const double *x, int32_t n, int32_t k, double sigma,
                       const orc_demo1_options *opt, orc_modemap *found,
                       int32_t *bins_log, int32_t *found_log, int32_t log_cap);
 
+/* Per-iteration record of the Demo1::Run loop (any array may be NULL; each
+ * holds cap entries): bins, map size after the iteration, Iterate's return
+ * value, the stale count after it, and the two thresholds it ran with. */
+typedef struct orc_demo1_log {
+  int32_t *bins;
+  int32_t *found;
+  uint8_t *res;
+  int32_t *stale;
+  double *bin_thr;
+  double *sv_thr;
+  int32_t cap;
+} orc_demo1_log;
+
+/* orc_demo1_run with the full per-iteration record. */
+int32_t orc_demo1_run_log(const double *x, int32_t n, int32_t k, double sigma,
+                          const orc_demo1_options *opt, orc_modemap *found,
+                          orc_demo1_log *log);
+
+/* The same loop with Iterate replaced by a script: iteration i returns
+ * script_res[i] and leaves the map at script_size[i] modes.  Runs until the
+ * stale stop or the script ends; returns the iterations run.  Pins the
+ * schedule (bins, thresholds, stop) independently of any signal. */
+int32_t orc_demo1_schedule(int32_t k, double sigma, const orc_demo1_options *opt,
+                           const uint8_t *script_res, const int32_t *script_size,
+                           int32_t script_len, orc_demo1_log *log);
+
 /* ---- gen, src/gen.cc:27-89 ---- */
 int orc_generate_mode_map(int32_t n, int32_t k, orc_modemap *mm);
 int orc_evaluate_modes(int32_t n, const orc_modemap *mm, double *out);

@@ -81,6 +81,32 @@ class Demo1Options(C.Structure):
             setattr(self, k, v)
 
 
+class Demo1Log(C.Structure):
+    """orc_demo1_log: per-iteration record of the Demo1::Run loop."""
+
+    _fields_ = [
+        ("bins", C.c_void_p),
+        ("found", C.c_void_p),
+        ("res", C.c_void_p),
+        ("stale", C.c_void_p),
+        ("bin_thr", C.c_void_p),
+        ("sv_thr", C.c_void_p),
+        ("cap", C.c_int32),
+    ]
+
+
+def _demo1_log(cap):
+    """A Demo1Log over fresh numpy arrays of cap entries (kept alive in .arrays)."""
+    arrays = dict(bins=np.zeros(cap, np.int32), found=np.zeros(cap, np.int32),
+                  res=np.zeros(cap, np.uint8), stale=np.zeros(cap, np.int32),
+                  bin_thr=np.zeros(cap, np.float64), sv_thr=np.zeros(cap, np.float64))
+    lg = Demo1Log()
+    for name, arr in arrays.items():
+        setattr(lg, name, arr.ctypes.data)
+    lg.cap = cap
+    return lg, arrays
+
+
 def build():
     """Compile the oracle with its Makefile (g++; no GPU involved)."""
     subprocess.check_call(["make", "-s", "-C", _HERE])
@@ -140,6 +166,13 @@ def lib():
     L.orc_demo1_run.argtypes = [_f64p, C.c_int32, C.c_int32, C.c_double,
                                 C.POINTER(Demo1Options), C.c_void_p, _i32p, _i32p, C.c_int32]
     L.orc_demo1_run.restype = C.c_int32
+    L.orc_demo1_run_log.argtypes = [_f64p, C.c_int32, C.c_int32, C.c_double,
+                                    C.POINTER(Demo1Options), C.c_void_p, C.POINTER(Demo1Log)]
+    L.orc_demo1_run_log.restype = C.c_int32
+    L.orc_demo1_schedule.argtypes = [C.c_int32, C.c_double, C.POINTER(Demo1Options),
+                                     C.POINTER(C.c_uint8), _i32p, C.c_int32,
+                                     C.POINTER(Demo1Log)]
+    L.orc_demo1_schedule.restype = C.c_int32
     L.orc_generate_mode_map.argtypes = [C.c_int32, C.c_int32, C.c_void_p]
     L.orc_evaluate_modes.argtypes = [C.c_int32, C.c_void_p, _f64p]
     L.orc_generate_xhat.argtypes = [C.c_int32, C.c_void_p, C.c_double, _f64p]
@@ -357,6 +390,37 @@ def demo1_run(x, k, sigma, opt=None, log_cap=4096):
     if it < 0:
         raise RuntimeError(f"demo1_run failed with status {-it}")
     return mm, bl[:it].copy(), fl[:it].copy()
+
+
+def demo1_run_log(x, k, sigma, opt=None, log_cap=4096):
+    """Demo1::Run with its full per-iteration record.  Returns (ModeMap, log)
+    where log maps bins / found / res / stale / bin_thr / sv_thr to arrays
+    of one entry per iteration."""
+    opt = opt or Demo1Options()
+    x = _c128(x)
+    n = x.size // 2
+    mm = ModeMap()
+    lg, arrays = _demo1_log(log_cap)
+    it = lib().orc_demo1_run_log(x, n, k, sigma, C.byref(opt), mm.h, C.byref(lg))
+    if it < 0:
+        raise RuntimeError(f"demo1_run failed with status {-it}")
+    return mm, {name: arr[:it].copy() for name, arr in arrays.items()}
+
+
+def demo1_schedule(k, sigma, script_res, script_size, opt=None):
+    """The Demo1::Run loop with Iterate replaced by a script (iteration i
+    returns script_res[i] and leaves script_size[i] modes).  Returns the
+    per-iteration record as demo1_run_log does."""
+    opt = opt or Demo1Options()
+    res = np.ascontiguousarray(script_res, np.uint8)
+    size = np.ascontiguousarray(script_size, np.int32)
+    lg, arrays = _demo1_log(max(1, res.size))
+    it = lib().orc_demo1_schedule(k, sigma, C.byref(opt),
+                                  res.ctypes.data_as(C.POINTER(C.c_uint8)), size,
+                                  res.size, C.byref(lg))
+    if it < 0:
+        raise RuntimeError(f"demo1_schedule failed with status {-it}")
+    return {name: arr[:it].copy() for name, arr in arrays.items()}
 
 
 # ---- gen (src/gen.cc) ----

@@ -115,7 +115,9 @@ inline unsigned atomicOr(unsigned *p, unsigned v) {
   return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
 }
 #define __HIP_MEMORY_SCOPE_SYSTEM 0
+#define __HIP_MEMORY_SCOPE_AGENT 1
 #define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), (order))
+#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), (order))
 #define __builtin_nontemporal_load(p) (*(p))
 
 // ---------------------------------------------------------------- runtime

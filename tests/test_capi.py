@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import mpsft_amd
@@ -112,3 +113,19 @@ def test_binner_factory_rejects_reference_cpu_types():
             mpsft_amd.BinInTime.create(t, w, 2)
     with pytest.raises(mpsft_amd.MpsftError):
         mpsft_amd.Window(1109, 4, 1e-6)
+
+
+@pytest.mark.parametrize("n,bins,delta", [(1 << 20, 101, 1e-6), (1 << 20, 101, 1e-5),
+                                          (1 << 22, 2001, 1e-6), (1 << 22, 2001, 1e-5),
+                                          (4099, 11, 1e-3), (1 << 26, 8001, 1e-6)])
+def test_window_delta_from_taps(orc, n, bins, delta):
+    """A reference Window keeps delta private (src/window.h:40) but exposes
+    its taps: delta recovered from the oracle's taps (Window::wt(),
+    src/window.cc:56-75) to 1e-12, and taps that are no window's refused."""
+    wt, t = orc.window_taps(n, bins, delta)
+    got = mpsft_amd.window_delta(n, bins, wt)
+    assert abs(got - delta) <= 1e-12 * delta
+    for bad in (wt * 1.001, np.concatenate([wt, wt[:bins]]), wt[: wt.size - 2],
+                np.where(np.arange(wt.size) == wt.size // 3, wt * 1.01, wt)):
+        with pytest.raises(mpsft_amd.MpsftError):
+            mpsft_amd.window_delta(n, bins, bad)

@@ -8,6 +8,7 @@ coefficients within 1e-12 relative for FP64 and 1e-6 relative for MIXED
 import numpy as np
 import pytest
 
+from demo1_cases import STALE_THEN_FOUND, check_demo1_schedule
 from mpsft_amd.signals import make_signal, quantization_sigma
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +119,73 @@ def test_bin_in_freq_many_modes(gpu, orc):
 
 
 # ---------------------------------------------------------------- MatPencil
+@pytest.mark.parametrize("delta", [1e-5, 1e-6])
+def test_plan_from_window_taps(gpu, orc, delta):
+    """A plan built from a reference Window's taps alone (its delta is
+    private, src/window.h:40): BinInTime with the caller's taps and BinInFreq
+    with the recovered delta match the delta-built plan and the oracle."""
+    n, bins, bits, k = 1 << 16, 101, 5, 40
+    x, truth = make_signal(n, k, 1e-6, seed=60)
+    wt, _ = orc.window_taps(n, bins, delta)
+    pw = gpu.Plan.from_window(n, bins, wt, bits)
+    pd = gpu.Plan(n, bins, delta, bits)
+    assert abs(pw.delta - delta) <= 1e-12 * delta
+    orc.seed(61)
+    for _ in range(3):
+        tf_o = orc.transform_draw(n)
+        q = orc.rand_int32() % n
+        tf = _tf(gpu, tf_o, n)
+        a_w, a_d = pw.bin_in_time(x, tf, q), pd.bin_in_time(x, tf, q)
+        ref = orc.bin_in_time(x, bins, delta, bits, tf_o, q, version=0)
+        assert rel_err(a_w, a_d) < FP64_TOL and rel_err(a_w, ref) < FP64_TOL
+        keys = np.array(list(truth), np.int64)
+        coefs = np.array([truth[int(key)] for key in keys], np.complex128)
+        f_w = pw.bin_in_freq(keys, coefs, tf, q, a_w.copy())
+        f_d = pd.bin_in_freq(keys, coefs, tf, q, a_d.copy())
+        f_o = orc.bin_in_freq(keys.astype(np.int32), coefs, n, bins, delta, bits, tf_o, q,
+                              ref.copy())
+        scale = np.abs(a_d).max()
+        assert np.abs(f_w - f_d).max() < FP64_TOL * scale
+        assert np.abs(f_w - f_o).max() < FP64_TOL * scale
+
+
+@pytest.mark.parametrize("trials", [1, 3])
+def test_identify_estimate_matches_oracle_trace(gpu, orc, trials):
+    """mpsft_identify_estimate on the oracle's own bin matrices (after
+    BinInFreq) of a real iteration: every bin's decision and k1 equal the
+    oracle's Iterate trace, found coefficients within 1e-12 of what it adds
+    to the map, and the decision margins agree (for the bins whose every
+    comparison both evaluate)."""
+    n, k, sigma = 1 << 16, 300, 1e-6
+    x, _ = make_signal(n, k, sigma, seed=21 + trials, magnitude="loguniform")
+    bins = 2 * k + 1
+    kw = dict(bins=bins, window_delta=1e-6, trials=trials, bin_threshold=3 * sigma / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
+    orc.seed(70 + trials)
+    mm = orc.ModeMap()
+    for step in range(2):  # the second step runs with the first step's modes subtracted
+        before = mm.to_dict()
+        tf_o = orc.transform_draw(n)
+        qs = np.array([orc.rand_int32() % n for _ in range(trials)], np.int32)
+        _, tr = orc.iterate(x, orc.IterateOptions(**kw), mm, tf=tf_o, qs=qs, want_a=True)
+        plan = gpu.Plan(n, bins, 1e-6, tr["bits"])
+        got = plan.identify_estimate(tr["a"], _tf(gpu, tf_o, n), qs.astype(np.int64),
+                                     kw["bin_threshold"], kw["sv_threshold"],
+                                     kw["window_threshold"])
+        assert np.array_equal(got["decide"], tr["decide"]), step
+        assert np.array_equal(got["k1"], tr["k1"].astype(np.int64)), step
+        found = got["decide"] == 3
+        assert found.sum() == tr["found"] and found.any()
+        after = mm.to_dict()
+        for b in np.flatnonzero(found):
+            key = int(got["k0"][b])
+            added = after[key] - before.get(key, 0)
+            assert abs(got["coef"][b] - added) <= FP64_TOL * max(abs(added), 1e-2), (step, b)
+        full = got["decide"] != 1  # the oracle stops at the first sv rejection
+        assert np.abs(got["margin"][full] - tr["margin"][full]).max() < 1e-9
+        assert np.all(got["margin"][~full] <= tr["margin"][~full] + 1e-12)
+
+
 def test_mat_pencil_kats_and_random(gpu, orc, kats):
     cases = kats["pencil_kats"]["cases"]
     u0 = np.array([complex(*c["u0"]) for c in cases])
@@ -274,6 +342,27 @@ def test_solve_parity_config2_fp64(gpu, orc):
     assert_coefs_close(got, ref, FP64_TOL)
 
 
+def test_post_analyze_config2_fp64(gpu, orc):
+    """PostAnalyze (src/demo1.cc:61-112) on the device after the N=2^22,
+    S=1000 FP64 solve, against the oracle's on the same found modes and
+    x-hat: counts exact, errors within 1e-12; the same figures bitwise on a
+    second call (fixed summation order)."""
+    n, k, sigma = 1 << 22, 1000, 1e-7
+    x, truth, xh = make_signal(n, k, sigma, seed=2, magnitude="loguniform", with_xhat=True)
+    s = gpu.Solver(n, gpu.solve_options(k, sigma), gpu.MPSFT_FP64)
+    s.solve(s.upload(x), 11)
+    got = s.post_analyze(list(truth), xh)
+    found = orc.ModeMap(s.modes_dict())
+    ref = orc.post_analyze(n, orc.ModeMap(truth), xh, found)
+    print(f"[post_analyze] gpu {got}\n[post_analyze] orc {ref}", flush=True)
+    assert got["hit"] == ref["hit"] and got["miss"] == ref["miss"] and got["miss"] > 0
+    for key in ("hit_l1", "hit_l2", "miss_l1", "miss_l2", "l1", "l2", "max"):
+        assert abs(got[key] - ref[key]) <= 1e-12 * abs(ref[key]), key
+    assert s.post_analyze(list(truth), xh) == got
+    with pytest.raises(gpu.MpsftError):  # truth keys must be distinct
+        s.post_analyze(list(truth) + [next(iter(truth))], xh)
+
+
 def test_solve_parity_config2_mixed(gpu, orc):
     """The same signal stored as complex64; thresholds calibrated to
     sigma_eff = sqrt(sigma^2 + sigma_q^2); compared with the oracle on the
@@ -347,6 +436,14 @@ def test_solve_parity_config3_mixed(gpu, orc):
     assert_coefs_close(got, ref, MIXED_TOL)
     assert solver.log()[0][0] == 8001
     assert len([key for key in truth if key in got]) >= 0.95 * k
+    # PostAnalyze on the device against the oracle's, with x-hat the dense
+    # FFT of the complex64 samples the solve saw (the global oracle)
+    xh = np.fft.fft(x_orc) / n
+    st = solver.post_analyze(list(truth), xh)
+    st_o = orc.post_analyze(n, orc.ModeMap(truth), xh, orc.ModeMap(got))
+    assert st["hit"] == st_o["hit"] and st["miss"] == st_o["miss"]
+    for key in ("hit_l1", "hit_l2", "l1", "l2", "max"):
+        assert abs(st[key] - st_o[key]) <= 1e-12 * abs(st_o[key]), key
 
 
 # ---------------------------------------------------------------- batched form
@@ -399,6 +496,57 @@ def test_batch_matches_single_and_oracle(gpu, orc, precision):
         assert_coefs_close(b.modes_dict(i), before[i], 1e-12, f"retuned {i}")
 
 
+@pytest.mark.timeout(900)
+def test_batch_config4_full_workload(gpu, orc):
+    """The batched workload at its full size: 256 complex64 signals,
+    n = 2^U{16..22}, S = U{10..500}, unit |c|, sigma = 1e-6, synthesised on
+    the GPU; tile width from the autotuner; one batched solve.  Every
+    signal's iteration log and support equal its own single-signal solve;
+    16 signals spanning n = 2^16..2^22 equal the oracle run on the same
+    complex64 samples."""
+    import torch
+    from mpsft_amd.signals import make_signal_device
+    count, sigma = 256, 1e-6
+    rng = np.random.default_rng(4)
+    ns = [1 << int(v) for v in rng.integers(16, 23, count)]
+    ks = [int(v) for v in rng.integers(10, 501, count)]
+    seeds = [int(v) for v in rng.integers(1, 1 << 30, count)]
+    print(f"[config4] synthesising {count} signals, {sum(ns) / 1e6:.0f} M samples", flush=True)
+    xs, opts = [], []
+    for i, (n, k) in enumerate(zip(ns, ks)):
+        x, sq, _ = make_signal_device(n, k, sigma, seed=7000 + i)
+        xs.append(x)
+        opts.append(gpu.solve_options(k, float(np.hypot(sigma, sq)), window_delta=1e-6))
+    torch.cuda.synchronize()
+    b = gpu.Batch(ns, opts, gpu.MPSFT_MIXED)
+    tile = b.autotune(xs, seeds)
+    assert tile in (1, 2, 4, 8)
+    iters, stats = b.solve(xs, seeds)
+    assert stats == [0] * count
+    print(f"[config4] batched solve done (tile {tile}); single-signal solves", flush=True)
+    got = [b.modes_dict(i) for i in range(count)]
+    for i in range(count):
+        s = gpu.Solver(ns[i], opts[i], gpu.MPSFT_MIXED)
+        assert s.solve(xs[i], seeds[i]) == iters[i], i
+        assert np.array_equal(s.log()[0], b.log(i)[0]), i
+        assert_coefs_close(got[i], s.modes_dict(), 1e-12, f"single {i}")
+        del s
+    # the oracle on 16 signals: the two smallest S at every n, 2^16 .. 2^22
+    by_n = {}
+    for i in sorted(range(count), key=lambda j: ks[j]):
+        by_n.setdefault(ns[i], []).append(i)
+    subset = [i for n in sorted(by_n) for i in by_n[n][:2]]
+    subset += [i for i in by_n[max(by_n)][2:2 + 16 - len(subset)]]
+    assert len(subset) >= 16 and {ns[i] for i in subset} == {1 << e for e in range(16, 23)}
+    for i in subset:
+        print(f"[config4] oracle signal {i}: n={ns[i]} k={ks[i]}", flush=True)
+        x_orc = xs[i].cpu().numpy().astype(np.complex128)
+        orc.seed(seeds[i])
+        ref, rb, _ = orc.demo1_run(x_orc, ks[i], opts[i].sigma, orc.Demo1Options())
+        assert np.array_equal(b.log(i)[0], rb), i
+        assert_coefs_close(got[i], ref.to_dict(), MIXED_TOL, f"oracle {i}")
+
+
 # ---------------------------------------------------------------- edge cases
 def test_solve_zero_modes_and_errors(gpu):
     n = 1 << 12
@@ -436,3 +584,33 @@ def test_modes_set_get_roundtrip(gpu):
     assert s.modes_dict() == d
     s.set_modes({})
     assert s.modes_dict() == {}
+    # keys equal modulo n are one ModeMap entry: refused, and the map is kept
+    s.set_modes(d)
+    import ctypes as C
+    keys = np.array([3, 3 + n], np.int64)
+    coefs = np.array([1.0, 0.0, 2.0, 0.0])
+    rc = gpu.lib().mpsft_solver_modes_set(s.h, keys.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          coefs.ctypes.data_as(C.POINTER(C.c_double)), 2)
+    assert rc == 1  # MPSFT_ERR_INVALID_ARG
+    assert s.modes_dict() == d
+
+
+# ---------------------------------------------------------------- Demo1::Run schedule
+@pytest.mark.parametrize("case", range(len(STALE_THEN_FOUND)))
+def test_solve_schedule_stale_then_found(gpu, orc, case):
+    """Solves whose schedule finds modes after a stale iteration: the device
+    loop's per-iteration log (bins, |found|) equals the oracle's, satisfies
+    the Demo1::Run recurrences on its own, and ends where the oracle's
+    never-reset stale count ends it."""
+    n, k, sigma, min_bins, max_stale, sseed, seed = STALE_THEN_FOUND[case]
+    x, _ = make_signal(n, k, sigma, seed=sseed)
+    kw = dict(min_bins=min_bins, max_stale_iter=max_stale)
+    got, ref, solver, _ = _solve_both(gpu, orc, x, x, n, k, sigma, seed=seed, **kw)
+    assert_coefs_close(got, ref, FP64_TOL)
+    g_bins, g_found = solver.log()
+    check_demo1_schedule(k, sigma, min_bins, max_stale, g_bins, g_found)
+    orc.seed(seed)
+    _, lg = orc.demo1_run_log(x, k, sigma, orc.Demo1Options(**kw))
+    res = lg["res"].astype(np.int64)
+    assert res[int(np.argmin(res)):].any()  # a finding iteration after a stale one
+    check_demo1_schedule(k, sigma, min_bins, max_stale, g_bins, g_found, res)

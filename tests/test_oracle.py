@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import approx
+from demo1_cases import HAND_SCHEDULES, STALE_THEN_FOUND, check_demo1_schedule
 from mpsft_amd.signals import make_signal
 
 
@@ -351,6 +352,56 @@ def test_demo1_recovers_against_dense_fft(orc):
         assert abs(d[key] - xh[key]) <= 1e-5 * abs(xh[key])
     assert bins_log[0] == max(2 * k + 1, 101)
     assert np.all(np.diff(found_log) >= 0)
+
+
+# ---------------------------------------------------------------- Demo1::Run schedule
+# The solve loop of src/demo1.cc:36-59: before iteration i,
+#   B_i = max(2 (k - |found after i-1|) + 1, min_bins)     (demo1.cc:48-49)
+#   bin threshold 3 sigma / sqrt(B_i), sv threshold sigma / sqrt(B_i)  (demo1.cc:50-51)
+# and stale counts the iterations whose Iterate returned false, never reset
+# (demo1.cc:55-57); the loop ends at the max_stale_iter-th such iteration.
+# The hand-written schedules (demo1_cases.py) fail on each plausible slip: a stale count
+# reset by a finding iteration, B from the count two iterations back, the
+# two thresholds swapped, or B not clamped to min_bins once |found| > k.
+
+
+@pytest.mark.parametrize("case", range(len(HAND_SCHEDULES)))
+def test_demo1_schedule_hand_written(orc, case):
+    k, sigma, min_bins, max_stale, script, want_bins = HAND_SCHEDULES[case]
+    opt = orc.Demo1Options(min_bins=min_bins, max_stale_iter=max_stale)
+    res = [r for r, _ in script]
+    size = [s for _, s in script]
+    lg = orc.demo1_schedule(k, sigma, res, size, opt)
+    assert lg["bins"].tolist() == want_bins
+    it = len(want_bins)
+    # thresholds written out from the stated B's
+    assert lg["bin_thr"].tolist() == [3.0 * sigma / np.sqrt(float(b)) for b in want_bins]
+    assert lg["sv_thr"].tolist() == [1.0 * sigma / np.sqrt(float(b)) for b in want_bins]
+    # the loop ends at the max_stale-th false, counted without reset
+    assert lg["res"].tolist() == res[:it]
+    assert lg["stale"].tolist() == list(np.cumsum([1 - r for r in res[:it]]))
+    assert lg["stale"][-1] == max_stale and res[it - 1] == 0
+
+
+@pytest.mark.parametrize("case", range(len(STALE_THEN_FOUND)))
+def test_demo1_schedule_real_solves(orc, case):
+    n, k, sigma, min_bins, max_stale, sseed, seed = STALE_THEN_FOUND[case]
+    x, _ = make_signal(n, k, sigma, seed=sseed)
+    orc.seed(seed)
+    opt = orc.Demo1Options(min_bins=min_bins, max_stale_iter=max_stale)
+    _, lg = orc.demo1_run_log(x, k, sigma, opt)
+    res = lg["res"].astype(np.int64)
+    check_demo1_schedule(k, sigma, min_bins, max_stale, lg["bins"], lg["found"], res)
+    assert np.allclose(lg["bin_thr"], 3.0 * sigma / np.sqrt(lg["bins"].astype(float)), rtol=0,
+                       atol=0)
+    assert np.allclose(lg["sv_thr"], sigma / np.sqrt(lg["bins"].astype(float)), rtol=0, atol=0)
+    # a finding iteration after a stale one: a reset stale count would run longer
+    first_stale = int(np.argmin(res))
+    assert res[first_stale] == 0 and res[first_stale:].any()
+    # the scripted loop replays the real one from its own record
+    sched = orc.demo1_schedule(k, sigma, lg["res"], lg["found"], opt)
+    for key in ("bins", "res", "stale", "bin_thr", "sv_thr"):
+        assert sched[key].tolist() == lg[key].tolist()
 
 
 # ---------------------------------------------------------------- gen
