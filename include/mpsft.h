@@ -232,6 +232,16 @@ int mpsft_solver_modes_set(mpsft_solver *s, const int64_t *keys,
  * entries gives 0/0 (NaN) for its mean and rms, as the reference prints. */
 int mpsft_solver_post_analyze(mpsft_solver *s, const int64_t *truth_keys_dev, int64_t m,
                               const void *xhat_dev, double *stats, void *stream);
+/* No counterpart in the reference: mpsft_solver_post_analyze for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  It calls no allocator and does not synchronise: its
+ * partial sums live in a fixed 40 KiB device array of the library (one per
+ * device, kept until the process ends), and stats_dev (device, 10 doubles)
+ * gets the nine figures and then 1.0 if the truth keys were not ascending
+ * (0.0 otherwise).  Any n and m; calls that overlap on one device race. */
+int mpsft_solver_post_analyze_noalloc(mpsft_solver *s, const int64_t *truth_keys_dev,
+                                      int64_t m, const void *xhat_dev, double *stats_dev,
+                                      void *stream);
 
 /* After an mpsft_iterate: copies the last iteration's bin matrices
  * (trials x rows x bins double2) into host memory. */

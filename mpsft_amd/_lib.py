@@ -29,6 +29,7 @@ EXPORTS = [
     "mpsft_bin_in_freq", "mpsft_identify_estimate", "mpsft_mat_pencil", "mpsft_solver_create",
     "mpsft_solver_destroy", "mpsft_solve", "mpsft_solve_profiled", "mpsft_iterate", "mpsft_solver_modes_count",
     "mpsft_solver_modes_get", "mpsft_solver_modes_set", "mpsft_solver_post_analyze",
+    "mpsft_solver_post_analyze_noalloc",
     "mpsft_solver_bins_get",
     "mpsft_solver_log_get", "mpsft_solver_transforms_get", "mpsft_solver_set_tile_residues",
     "mpsft_solver_set_record", "mpsft_solver_info", "mpsft_solver_replay",
@@ -134,6 +135,7 @@ def declare(L):
         "mpsft_solver_modes_get": (C.c_int, [vp, pi64, pf64, i64, pi64]),
         "mpsft_solver_modes_set": (C.c_int, [vp, pi64, pf64, i64]),
         "mpsft_solver_post_analyze": (C.c_int, [vp, vp, i64, vp, pf64, vp]),
+        "mpsft_solver_post_analyze_noalloc": (C.c_int, [vp, vp, i64, vp, vp, vp]),
         "mpsft_solver_bins_get": (C.c_int, [vp, pf64, i64, pi32, pi32, pi32]),
         "mpsft_solver_log_get": (C.c_int, [vp, pi32, pi32, i32, pi32]),
         "mpsft_solver_transforms_get": (C.c_int, [vp, C.POINTER(Transform), pi64, i32, pi32]),

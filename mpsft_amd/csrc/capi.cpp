@@ -272,6 +272,15 @@ int mpsft_solver_post_analyze(mpsft_solver *s, const int64_t *truth_keys_dev, in
                              S(stream));
 }
 
+int mpsft_solver_post_analyze_noalloc(mpsft_solver *s, const int64_t *truth_keys_dev, int64_t m,
+                                      const void *xhat_dev, double *stats_dev, void *stream) {
+  if (!s || !stats_dev) return MPSFT_ERR_INVALID_ARG;
+  double *partial = nullptr;
+  if (mpsft::PostAnalyzeScratch(&partial) != hipSuccess) return MPSFT_ERR_HIP;
+  return s->eng->PostAnalyzeDev(truth_keys_dev, m, static_cast<const double2 *>(xhat_dev),
+                                partial, mpsft::kPaBlocks, stats_dev, S(stream));
+}
+
 int mpsft_solver_modes_count(mpsft_solver *s, int64_t *count) {
   if (!s || !count) return MPSFT_ERR_INVALID_ARG;
   return s->eng->ModesCount(count);

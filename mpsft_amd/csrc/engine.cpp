@@ -1249,9 +1249,6 @@ int Engine::IdentifyOnly(const double2 *A, int32_t trials, int32_t B, double del
   return MPSFT_OK;
 }
 
-// Workgroups of a PostAnalyze pass through the engine's own scratch.
-constexpr int32_t kPaBlocks = 512;
-
 int Engine::PostAnalyzeDev(const int64_t *truth, int64_t m, const double2 *xhat, double *partial,
                            int32_t blocks, double *stats_dev, hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;

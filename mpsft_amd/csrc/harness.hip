@@ -15,9 +15,46 @@
 // coef and x-hat (2 x 16 B per location).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.h"
 
 namespace mpsft {
+
+// The no-allocation form (mpsft_solver_post_analyze_noalloc): the partial
+// sums of kPaBlocks workgroups in one file-scope device array, 40 KiB on
+// every device the library runs on, held until the process ends.  Every
+// call on a device shares it (each call rewrites all of it before reading
+// it); calls that overlap on one device race.
+static __device__ __align__(256) double scratch[kPaBlocks * kPaFields];
+
+namespace {
+
+// The array's device address on each device, looked up once per device on
+// its first call (the array's name is a host-side stand-in, not a device
+// address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+hipError_t PostAnalyzeScratch(double **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 namespace {
 
 // Partial sums per workgroup (kPaFields of them): hit, hit l1, hit l2,

@@ -117,7 +117,11 @@ hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma,
 // (kPaFields doubles each) to partial, one more reduces them into
 // stats[0..9] (device memory).
 constexpr int kPaFields = 10;
+constexpr int kPaBlocks = 512;  // workgroups of a PostAnalyze pass
 hipError_t LaunchPostAnalyze(const Ctx &c, const int64_t *truth, int64_t m, const double2 *xhat,
                              double *partial, int32_t blocks, double *stats, hipStream_t s);
+// The device address of the static PostAnalyze scratch (kPaBlocks x
+// kPaFields doubles) on the current device, looked up on its first use.
+hipError_t PostAnalyzeScratch(double **ptr);
 
 }  // namespace mpsft

@@ -156,6 +156,11 @@ typedef struct EmuGraphNode *hipGraphNode_t;
 
 hipError_t hipGetLastError();
 hipError_t hipGetDeviceCount(int *count);
+hipError_t hipGetDevice(int *device);
+// A file-scope __device__ array is host memory here: its address is itself.
+#define HIP_SYMBOL(sym) (reinterpret_cast<const void *>(&(sym)))
+#define __align__(n) __attribute__((aligned(n)))
+hipError_t hipGetSymbolAddress(void **p, const void *symbol);
 hipError_t hipMalloc(void **p, size_t bytes);
 hipError_t hipFree(void *p);
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned flags = 0);
