@@ -86,6 +86,18 @@ int32_t mpsft_window_p(int64_t n, int32_t bins, double delta);
  * not such a window's.  Host only. */
 int mpsft_window_delta(int64_t n, int32_t bins, int32_t p, const double *wt, double *delta);
 
+/* The signal of Demo1(opt, n, k, sigma) (src/demo1.cc:27-34) after
+ * RandomSeed(seed) (src/base.cc:52): GenerateModeMap (src/gen.cc:27-38) on
+ * the host, k distinct locations with unit-modulus coefficients, into keys
+ * (k int64) and coefs (k interleaved complex128), in draw order; then
+ * GenerateXhatAlt (src/gen.cc:78-89) on the device into xhat_dev (n
+ * double2): the modes plus complex Gaussian noise of sigma/sqrt(2n) per
+ * component, from the same std::mt19937 stream (normals by libstdc++'s
+ * polar method).  The signal is x = the unnormalised backward DFT of x-hat
+ * (src/demo1.cc:32-33).  n < 2^31.  Synchronises the stream. */
+int mpsft_generate_signal(int64_t n, int32_t k, double sigma, uint32_t seed, void *xhat_dev,
+                          int64_t *keys, double *coefs, void *stream);
+
 /* ---- plan: one window + binner pair ----
  * Replaces Window(n, bins, delta) (src/window.cc:28) together with
  * BinInTime::Create / BinInFreq::Create (src/binner.cc:58-76, 526-535).

@@ -24,7 +24,7 @@ EXPORTS = [
     "mpsft_free", "mpsft_memcpy_htod", "mpsft_memcpy_dtoh", "mpsft_memset",
     "mpsft_stream_create", "mpsft_stream_destroy", "mpsft_stream_synchronize",
     "mpsft_device_synchronize", "mpsft_transform_make", "mpsft_num_bits",
-    "mpsft_window_p", "mpsft_window_delta", "mpsft_plan_create", "mpsft_plan_create_window",
+    "mpsft_window_p", "mpsft_window_delta", "mpsft_generate_signal", "mpsft_plan_create", "mpsft_plan_create_window",
     "mpsft_plan_delta", "mpsft_plan_destroy", "mpsft_bin_in_time",
     "mpsft_bin_in_freq", "mpsft_identify_estimate", "mpsft_mat_pencil", "mpsft_solver_create",
     "mpsft_solver_destroy", "mpsft_solve", "mpsft_solve_profiled", "mpsft_iterate", "mpsft_solver_modes_count",
@@ -89,8 +89,22 @@ def lib():
         raise ImportError(
             f"{LIB_PATH} is not built; run `python -m mpsft_amd.build` "
             "(the HIP library is required, there is no CPU fallback)")
+    _start_torch_hip()
     _lib = declare(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _start_torch_hip():
+    """PyTorch for ROCm carries its own HIP runtime beside the system one this
+    library links.  In one process the two coexist (they share the device's
+    address space, so tensors and this library's buffers mix) only when
+    torch's runtime opens the device first: started after ours, torch finds
+    no GPU.  So torch, when installed, is started before this library."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
 
 
 def declare(L):
@@ -114,6 +128,7 @@ def declare(L):
         "mpsft_num_bits": (i32, [i64, i32]),
         "mpsft_window_p": (i32, [i64, i32, f64]),
         "mpsft_window_delta": (C.c_int, [i64, i32, i32, pf64, pf64]),
+        "mpsft_generate_signal": (C.c_int, [i64, i32, f64, C.c_uint32, vp, pi64, pf64, vp]),
         "mpsft_plan_create": (C.c_int, [i64, i32, f64, i32, C.c_int, C.POINTER(vp)]),
         "mpsft_plan_create_window": (C.c_int, [i64, i32, i32, pf64, i32, C.c_int,
                                                C.POINTER(vp)]),

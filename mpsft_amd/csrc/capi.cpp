@@ -272,6 +272,12 @@ int mpsft_solver_post_analyze(mpsft_solver *s, const int64_t *truth_keys_dev, in
                              S(stream));
 }
 
+int mpsft_generate_signal(int64_t n, int32_t k, double sigma, uint32_t seed, void *xhat_dev,
+                          int64_t *keys, double *coefs, void *stream) {
+  return mpsft::GenerateSignal(n, k, sigma, seed, static_cast<double2 *>(xhat_dev), keys, coefs,
+                               S(stream));
+}
+
 int mpsft_solver_post_analyze_noalloc(mpsft_solver *s, const int64_t *truth_keys_dev, int64_t m,
                                       const void *xhat_dev, double *stats_dev, void *stream) {
   if (!s || !stats_dev) return MPSFT_ERR_INVALID_ARG;

@@ -15,7 +15,12 @@
 // coef and x-hat (2 x 16 B per location).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <mutex>
+#include <random>
+#include <sstream>
+#include <unordered_set>
+#include <vector>
 
 #include "kernels.h"
 
@@ -155,7 +160,307 @@ __global__ void __launch_bounds__(kThreads)
   stats[9] = v[9] > 0 ? 1.0 : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// The signal of Demo1 (src/demo1.cc:27-34): GenerateModeMap then
+// GenerateXhatAlt (src/gen.cc:27-38, 78-89) from the reference's global
+// std::mt19937 (src/base.cc:32-36).  x-hat = the modes plus
+// Cplex(RandomNormal(), RandomNormal()) * sigma/sqrt(2n) at every location,
+// the imaginary part drawn first (g++ evaluates the arguments right to
+// left).  std::normal_distribution<double> is the Marsaglia polar method of
+// libstdc++: an attempt draws two uniforms (generate_canonical<double, 53>:
+// two 32-bit words each, (w0 + w1 2^32) / 2^64 in double arithmetic, 1
+// replaced by the double below it), x = 2u1 - 1, y = 2u2 - 1, accepted when
+// 0 < x^2 + y^2 <= 1; an accepted attempt gives y*m, then (cached) x*m with
+// m = sqrt(-2 ln r2 / r2).  With an even number of normals drawn before
+// (two per mode), location i's sample is exactly the i-th accepted attempt
+// of the engine's stream after the mode map: re = x m, im = y m.
+//
+// The whole file is compiled without FP contraction (build.py): the host's
+// normal_distribution computes r2 = x x + y y, which a fused multiply-add
+// would round differently from the reference's g++ build.
+//
+// On the device: one wavefront runs the engine from its state (624 words
+// and a position) and writes the raw 32-bit stream; then every attempt (4
+// words) is tested in parallel, the accepted ones are ranked by a prefix
+// count, and the i-th accepted attempt writes sample i.  Bound: the
+// engine's twist, 624 words per step of four dependent phases.
+
+constexpr uint32_t kMtN = 624, kMtM = 397;
+constexpr uint32_t kMtUpper = 0x80000000u, kMtLower = 0x7fffffffu, kMtA = 0x9908b0dfu;
+
+__device__ __forceinline__ uint32_t MtTwist(uint32_t cur, uint32_t next, uint32_t far) {
+  const uint32_t y = (cur & kMtUpper) | (next & kMtLower);
+  return far ^ (y >> 1) ^ ((y & 1u) ? kMtA : 0u);
+}
+
+__device__ __forceinline__ uint32_t MtTemper(uint32_t y) {
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+// words out[0..count) of the stream of an engine in state (mt, pos); one
+// workgroup of 64 threads.
+__global__ void __launch_bounds__(64)
+    k_mt19937(const uint32_t *state, uint32_t pos, uint32_t *out, int64_t count) {
+  __shared__ uint32_t mt[kMtN];
+  for (uint32_t i = threadIdx.x; i < kMtN; i += blockDim.x) mt[i] = state[i];
+  __syncthreads();
+  int64_t done = 0;
+  // the rest of the current block
+  for (uint32_t i = pos + threadIdx.x; i < kMtN && done + (i - pos) < count; i += blockDim.x)
+    out[i - pos] = MtTemper(mt[i]);
+  done = min<int64_t>(count, int64_t(kMtN - pos));
+  while (done < count) {
+    // The twist in four phases; each reads only words that are final
+    // before it (new for index < i, old for index > i), then writes.
+    //   [0, 227): mt[i+397] old;  [227, 454): mt[i-227] new from the first;
+    //   [454, 623): mt[i-227] new from the second;  623: mt[0], mt[396] new.
+    const uint32_t lo[4] = {0, kMtN - kMtM, 2 * (kMtN - kMtM), kMtN - 1};
+    const uint32_t hi[4] = {kMtN - kMtM, 2 * (kMtN - kMtM), kMtN - 1, kMtN};
+    for (int ph = 0; ph < 4; ++ph) {
+      uint32_t v[4];
+      int nv = 0;
+      for (uint32_t i = lo[ph] + threadIdx.x; i < hi[ph]; i += blockDim.x, ++nv) {
+        const uint32_t far = mt[i < kMtN - kMtM ? i + kMtM : i - (kMtN - kMtM)];
+        v[nv] = MtTwist(mt[i], mt[(i + 1) % kMtN], far);
+      }
+      __syncthreads();
+      nv = 0;
+      for (uint32_t i = lo[ph] + threadIdx.x; i < hi[ph]; i += blockDim.x, ++nv) mt[i] = v[nv];
+      __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < kMtN && done + i < count; i += blockDim.x)
+      out[done + i] = MtTemper(mt[i]);
+    done += kMtN;
+    __syncthreads();
+  }
+}
+
+// generate_canonical<double, 53> over two 32-bit words.
+__device__ __forceinline__ double Canonical(uint32_t w0, uint32_t w1) {
+#pragma clang fp contract(off)
+  const double sum = double(w0) + double(w1) * 4294967296.0;
+  const double r = sum / 18446744073709551616.0;  // 2^64
+  return r >= 1.0 ? 0.99999999999999988898 : r;   // the double below 1
+}
+
+__device__ __forceinline__ bool PolarAttempt(const uint32_t *w, double *x, double *y, double *r2) {
+#pragma clang fp contract(off)
+  *x = 2.0 * Canonical(w[0], w[1]) - 1.0;
+  *y = 2.0 * Canonical(w[2], w[3]) - 1.0;
+  *r2 = *x * *x + *y * *y;
+  return !(*r2 > 1.0 || *r2 == 0.0);
+}
+
+constexpr int kScanTile = 4 * kThreads;  // attempts per workgroup of the ranking scan
+
+// acc[t] = accepted attempts in tile t (kScanTile attempts per workgroup)
+__global__ void __launch_bounds__(kThreads)
+    k_polar_count(const uint32_t *words, int64_t attempts, int32_t *acc) {
+  __shared__ int32_t sh[kThreads];
+  int32_t c = 0;
+  const int64_t base = int64_t(blockIdx.x) * kScanTile;
+  for (int j = threadIdx.x; j < kScanTile; j += blockDim.x) {
+    double x, y, r2;
+    if (base + j < attempts && PolarAttempt(words + 4 * (base + j), &x, &y, &r2)) ++c;
+  }
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[blockIdx.x] = sh[0];
+}
+
+// acc[t] <- accepted attempts before tile t (exclusive prefix over tiles),
+// acc[tiles] <- the total; one thread (a few thousand tiles per 2^22
+// samples).
+__global__ void k_polar_offsets(int32_t *acc, int32_t tiles) {
+  if (threadIdx.x != 0) return;
+  int32_t s = 0;
+  for (int32_t t = 0; t < tiles; ++t) {
+    const int32_t v = acc[t];
+    acc[t] = s;
+    s += v;
+  }
+  acc[tiles] = s;
+}
+
+// xhat[r] = (x m, y m) * scale for the r-th accepted attempt, r < n.  Each
+// workgroup ranks its tile's attempts in order (a prefix count in LDS).
+__global__ void __launch_bounds__(kThreads)
+    k_polar_samples(const uint32_t *words, int64_t attempts, const int32_t *acc, int64_t n,
+                    double scale, double2 *xhat) {
+#pragma clang fp contract(off)
+  __shared__ int32_t sh[kThreads];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile;
+  constexpr int kPer = kScanTile / kThreads;  // consecutive attempts per thread
+  double xs[kPer], ys[kPer], r2s[kPer];
+  bool ok[kPer];
+  int32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int64_t j = base + int64_t(threadIdx.x) * kPer + e;
+    ok[e] = j < attempts && PolarAttempt(words + 4 * j, &xs[e], &ys[e], &r2s[e]);
+    c += ok[e] ? 1 : 0;
+  }
+  // exclusive prefix of the per-thread counts (Hillis-Steele in LDS)
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const int32_t v = int(threadIdx.x) >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t r = int64_t(acc[blockIdx.x]) + sh[threadIdx.x] - c;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    if (!ok[e]) continue;
+    if (r < n) {
+      const double m = sqrt(-2.0 * log(r2s[e]) / r2s[e]);
+      xhat[r] = make_double2(xs[e] * m * scale, ys[e] * m * scale);
+    }
+    ++r;
+  }
+}
+
+// xhat[k] += coef for every mode (location k: the mode plus its noise).
+__global__ void __launch_bounds__(kThreads)
+    k_add_modes(const int64_t *keys, const double2 *coefs, int64_t m, double2 *xhat) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) {
+    const double2 c = coefs[i];
+    const double2 v = xhat[keys[i]];
+    xhat[keys[i]] = make_double2(c.x + v.x, c.y + v.y);
+  }
+}
+
 }  // namespace
+
+namespace {
+
+// RandomInt32 (src/base.cc:53) as libstdc++ computed it before GCC 11:
+// the engine's outputs below 2^31.
+int32_t RandomInt32(std::mt19937 &eng) {
+  uint32_t r;
+  do {
+    r = uint32_t(eng());
+  } while (r >= 0x80000000u);
+  return int32_t(r);
+}
+
+template <typename T>
+int Alloc(T **p, size_t count) {
+  return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T)) == hipSuccess
+             ? MPSFT_OK
+             : MPSFT_ERR_NOMEM;
+}
+
+}  // namespace
+
+int GenerateSignal(int64_t n, int32_t k, double sigma, uint32_t seed, double2 *xhat,
+                   int64_t *keys, double *coefs, hipStream_t s) {
+  if (n < 1 || n > 0x7fffffffLL || k < 0 || k > n || !(sigma >= 0) || !xhat ||
+      (k > 0 && (!keys || !coefs)))
+    return MPSFT_ERR_INVALID_ARG;
+  // GenerateModeMap (src/gen.cc:27-38) on the host: k distinct locations
+  // UnsafePosMod(RandomInt32(), n), coefficients Cplex(RandomNormal(),
+  // RandomNormal()) (imaginary part first) over their modulus.
+  std::mt19937 eng(seed);
+  std::normal_distribution<double> nd;
+  std::unordered_set<int64_t> seen;
+  for (int32_t i = 0; i < k;) {
+    const int64_t idx = RandomInt32(eng) % n;
+    if (seen.count(idx)) continue;
+    const double im = nd(eng);
+    const double re = nd(eng);
+    const double a = std::hypot(re, im);  // std::abs of a complex<double> (glibc cabs)
+    seen.insert(idx);
+    keys[i] = idx;
+    coefs[2 * i] = re / a;
+    coefs[2 * i + 1] = im / a;
+    ++i;
+  }
+  // The engine's state after the mode map: 624 words, then the position.
+  std::vector<uint32_t> state(kMtN);
+  uint32_t pos = 0;
+  {
+    std::ostringstream os;
+    os << eng;
+    std::istringstream is(os.str());
+    for (auto &w : state) is >> w;
+    is >> pos;
+    if (!is || pos > kMtN) return MPSFT_ERR_UNSUPPORTED;
+  }
+  const double scale = sigma / std::sqrt(2.0 * double(n));
+  int rc = MPSFT_OK;
+  uint32_t *d_state = nullptr, *d_words = nullptr;
+  int32_t *d_acc = nullptr;
+  int64_t *d_keys = nullptr;
+  double2 *d_coefs = nullptr;
+  // Attempts for n accepted ones (acceptance pi/4): 1.35 n plus a margin,
+  // more when the draw falls short.
+  int64_t attempts = int64_t(1.35 * double(n)) + 4096;
+  for (int tries = 0; tries < 4 && rc == MPSFT_OK; ++tries, attempts *= 2) {
+    const int32_t tiles = int32_t((attempts + kScanTile - 1) / kScanTile);
+    if ((rc = Alloc(&d_state, kMtN)) != MPSFT_OK || (rc = Alloc(&d_words, 4 * attempts)) != MPSFT_OK ||
+        (rc = Alloc(&d_acc, size_t(tiles) + 1)) != MPSFT_OK)
+      break;
+    int32_t total = 0;
+    if (hipMemcpyAsync(d_state, state.data(), kMtN * sizeof(uint32_t), hipMemcpyHostToDevice, s) !=
+        hipSuccess) {
+      rc = MPSFT_ERR_HIP;
+      break;
+    }
+    hipLaunchKernelGGL(k_mt19937, dim3(1), dim3(64), 0, s, d_state, pos, d_words, 4 * attempts);
+    hipLaunchKernelGGL(k_polar_count, dim3(tiles), dim3(kThreads), 0, s, d_words, attempts, d_acc);
+    hipLaunchKernelGGL(k_polar_offsets, dim3(1), dim3(64), 0, s, d_acc, tiles);
+    if (hipMemcpyAsync(&total, d_acc + tiles, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = MPSFT_ERR_HIP;
+      break;
+    }
+    if (total >= n) {
+      hipLaunchKernelGGL(k_polar_samples, dim3(tiles), dim3(kThreads), 0, s, d_words, attempts,
+                         d_acc, n, scale, xhat);
+      if (k > 0) {
+        if ((rc = Alloc(&d_keys, k)) != MPSFT_OK || (rc = Alloc(&d_coefs, k)) != MPSFT_OK) break;
+        if (hipMemcpyAsync(d_keys, keys, size_t(k) * sizeof(int64_t), hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(d_coefs, coefs, size_t(k) * sizeof(double2), hipMemcpyHostToDevice,
+                           s) != hipSuccess) {
+          rc = MPSFT_ERR_HIP;
+          break;
+        }
+        hipLaunchKernelGGL(k_add_modes, dim3((k + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                           d_keys, d_coefs, int64_t(k), xhat);
+      }
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        rc = MPSFT_ERR_HIP;
+      break;
+    }
+    (void)hipFree(d_state);
+    (void)hipFree(d_words);
+    (void)hipFree(d_acc);
+    d_state = nullptr;
+    d_words = nullptr;
+    d_acc = nullptr;
+    if (tries == 3) rc = MPSFT_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d_state);
+  (void)hipFree(d_words);
+  (void)hipFree(d_acc);
+  (void)hipFree(d_keys);
+  (void)hipFree(d_coefs);
+  return rc;
+}
 
 hipError_t LaunchPostAnalyze(const Ctx &c, const int64_t *truth, int64_t m, const double2 *xhat,
                              double *partial, int32_t blocks, double *stats, hipStream_t s) {

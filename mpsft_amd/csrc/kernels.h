@@ -120,6 +120,12 @@ constexpr int kPaFields = 10;
 constexpr int kPaBlocks = 512;  // workgroups of a PostAnalyze pass
 hipError_t LaunchPostAnalyze(const Ctx &c, const int64_t *truth, int64_t m, const double2 *xhat,
                              double *partial, int32_t blocks, double *stats, hipStream_t s);
+// The signal of Demo1 (src/demo1.cc:27-34) from the reference's RNG seeded
+// with seed: GenerateModeMap on the host (k keys and interleaved complex
+// coefficients into host arrays, in draw order), GenerateXhatAlt into xhat
+// (n double2, device).  Returns an mpsft_status.
+int GenerateSignal(int64_t n, int32_t k, double sigma, uint32_t seed, double2 *xhat,
+                   int64_t *keys, double *coefs, hipStream_t s);
 // The device address of the static PostAnalyze scratch (kPaBlocks x
 // kPaFields doubles) on the current device, looked up on its first use.
 hipError_t PostAnalyzeScratch(double **ptr);

@@ -78,6 +78,26 @@ def make_signal_device(n, k, sigma, seed):
     return x64, sigma_q, truth
 
 
+def demo1_signal_device(n, k, sigma, seed):
+    """The signal of Demo1(opt, n, k, sigma) (src/demo1.cc:27-34) on the GPU
+    from the reference's RNG stream seeded with seed: GenerateModeMap and
+    GenerateXhatAlt by mpsft_generate_signal, then x = the unnormalised
+    backward DFT of x-hat with torch's GPU FFT.  Returns (x, xhat, truth):
+    complex128 cuda tensors and {location: coefficient}."""
+    import ctypes as C
+    from ._lib import check, lib
+    xh = torch.empty(n, dtype=torch.complex128, device="cuda")
+    keys = np.zeros(k, np.int64)
+    coefs = np.zeros(2 * k, np.float64)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().mpsft_generate_signal(n, k, sigma, seed, C.c_void_p(xh.data_ptr()),
+                                      keys.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      coefs.ctypes.data_as(C.POINTER(C.c_double)), stream),
+          "mpsft_generate_signal")
+    x = torch.fft.ifft(xh) * n
+    return x, xh, dict(zip(keys.tolist(), coefs.view(np.complex128).tolist()))
+
+
 def quantization_sigma(x128, x64):
     """RMS per-sample error of storing x in complex64: the noise the mixed
     form sees on top of sigma."""

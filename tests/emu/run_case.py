@@ -178,8 +178,81 @@ def batch_adjoint(ns, ks, sigma, seeds, precision=0):
     return dict(iters=iters, status=stats, dot_rel=dot, vs_single=single)
 
 
+def gen(n, k, sigma, seed):
+    """mpsft_generate_signal against the oracle's GenerateModeMap /
+    GenerateXhatAlt on the same seed (the emulator's log and sqrt are the
+    host's, so x-hat must agree bit for bit)."""
+    xd = gpu.DeviceArray(16 * n)
+    keys = np.zeros(k, np.int64)
+    coefs = np.zeros(2 * k)
+    rc = gpu.lib().mpsft_generate_signal(n, k, sigma, seed, xd.ptr,
+                                         keys.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         coefs.ctypes.data_as(C.POINTER(C.c_double)), None)
+    got = xd.to_numpy(np.complex128, (n,))
+    orc.seed(seed)
+    mm = orc.generate_mode_map(n, k)
+    xh = orc.generate_xhat_alt(n, mm, sigma)
+    return dict(rc=rc, same_modes=dict(zip(keys.tolist(), coefs.view(np.complex128).tolist()))
+                == mm.to_dict(), same_xhat=bool(np.array_equal(got, xh)))
+
+
+def analyze(n, k, sigma, seed):
+    """A solve, then PostAnalyze on the device (allocating and static-scratch
+    forms) against the oracle's on the same found modes and x-hat."""
+    rng = np.random.default_rng(seed)
+    locs = rng.choice(n, size=k, replace=False)
+    xh = np.zeros(n, np.complex128)
+    xh[locs] = np.exp(2j * np.pi * rng.uniform(0, 1, k))
+    truth = {int(q): complex(xh[q]) for q in locs}
+    xh += (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * sigma / np.sqrt(2 * n)
+    x = np.fft.ifft(xh) * n
+    s = gpu.Solver(n, gpu.solve_options(k, sigma), 0)
+    s.solve(s.upload(x), seed)
+    st = s.post_analyze(list(truth), xh)
+    st_o = orc.post_analyze(n, orc.ModeMap(truth), xh, orc.ModeMap(s.modes_dict()))
+    kd = gpu.DeviceArray.from_numpy(np.sort(np.array(list(truth), np.int64)))
+    xd = gpu.DeviceArray.from_numpy(xh)
+    out = gpu.DeviceArray(80)
+    rc = gpu.lib().mpsft_solver_post_analyze_noalloc(s.h, kd.ptr, k, xd.ptr, out.ptr, None)
+    gpu.synchronize()
+    na = out.to_numpy(np.float64, (10,))
+    names = ("hit", "hit_l1", "hit_l2", "miss", "miss_l1", "miss_l2", "l1", "l2", "max")
+    rel = max(abs(st[m] - st_o[m]) / abs(st_o[m]) for m in names if st_o[m] and st_o[m] == st_o[m])
+    return dict(hit=st["hit"], hit_o=st_o["hit"], miss=st["miss"], miss_o=st_o["miss"], rel=rel,
+                noalloc_rc=rc, noalloc_same=all(na[i] == st[m] or (na[i] != na[i] and st[m] != st[m])
+                                                for i, m in enumerate(names)),
+                noalloc_flag=float(na[9]))
+
+
+def identify(n, k, sigma, seed, trials=1):
+    """mpsft_identify_estimate on the oracle's bin matrices of one Iterate:
+    decisions, k1 and margins against the oracle's trace."""
+    x = signal(n, k, sigma, seed)
+    bins = 2 * k + 1
+    kw = dict(bins=bins, window_delta=1e-6, trials=trials, bin_threshold=3 * sigma / np.sqrt(bins),
+              window_threshold=0.1, sv_threshold=sigma / np.sqrt(bins))
+    orc.seed(seed)
+    mm = orc.ModeMap()
+    tf_o = orc.transform_draw(n)
+    qs = np.array([orc.rand_int32() % n for _ in range(trials)], np.int32)
+    _, tr = orc.iterate(x, orc.IterateOptions(**kw), mm, tf=tf_o, qs=qs, want_a=True)
+    plan = gpu.Plan(n, bins, 1e-6, tr["bits"])
+    tf = gpu.transform_make(n, int(tf_o[0]), int(tf_o[1]), int(tf_o[2]))
+    got = plan.identify_estimate(tr["a"], tf, qs.astype(np.int64), kw["bin_threshold"],
+                                 kw["sv_threshold"], kw["window_threshold"])
+    found = got["decide"] == 3
+    full = got["decide"] != 1
+    ref = mm.to_dict()
+    return dict(same_decide=bool(np.array_equal(got["decide"], tr["decide"])),
+                same_k1=bool(np.array_equal(got["k1"], tr["k1"].astype(np.int64))),
+                n_found=int(found.sum()),
+                coef_err=max(abs(got["coef"][b] - ref[int(got["k0"][b])]) for b in np.flatnonzero(found)),
+                margin_err=float(np.abs(got["margin"][full] - tr["margin"][full]).max()))
+
+
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
     fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch,
-          "adjoint": adjoint, "batch_adjoint": batch_adjoint}[case.pop("kind")]
+          "adjoint": adjoint, "batch_adjoint": batch_adjoint, "gen": gen, "analyze": analyze,
+          "identify": identify}[case.pop("kind")]
     print("RESULT " + json.dumps(fn(**case)), flush=True)

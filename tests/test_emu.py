@@ -61,6 +61,11 @@ CASES = {
     "adjoint_10007_10_1": dict(kind="adjoint", n=10007, k=10, sigma=1e-6, seed=7, trials=1),
     "batch_adjoint": dict(kind="batch_adjoint", ns=[4096, 8192, 16384], ks=[5, 12, 20],
                           sigma=1e-6, seeds=[3, 4, 5]),
+    "gen_4099": dict(kind="gen", n=4099, k=7, sigma=0.5, seed=77),
+    "gen_65536": dict(kind="gen", n=1 << 16, k=30, sigma=1e-6, seed=123534),
+    "analyze": dict(kind="analyze", n=1 << 14, k=40, sigma=1e-5, seed=9),
+    "identify_1": dict(kind="identify", n=1 << 14, k=60, sigma=1e-6, seed=11, trials=1),
+    "identify_3": dict(kind="identify", n=1 << 14, k=60, sigma=1e-6, seed=12, trials=3),
 }
 
 
@@ -147,6 +152,32 @@ def test_emu_batched_adjoint(emu_runs):
     r = result(emu_runs, "batch_adjoint")
     assert r["status"] == [0, 0, 0]
     assert max(r["dot_rel"]) < 1e-12 and max(r["vs_single"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["gen_4099", "gen_65536"])
+def test_emu_generate_signal(emu_runs, case):
+    """Demo1's signal from the reference's RNG stream (GenerateModeMap on the
+    host, GenerateXhatAlt on the device) equals the oracle's bit for bit."""
+    r = result(emu_runs, case)
+    assert r["rc"] == 0 and r["same_modes"] and r["same_xhat"]
+
+
+def test_emu_post_analyze(emu_runs):
+    """PostAnalyze on the device against the oracle's, and its static-scratch
+    form bit for bit against the allocating one."""
+    r = result(emu_runs, "analyze")
+    assert r["hit"] == r["hit_o"] == 40 and r["miss"] == r["miss_o"]
+    assert r["rel"] < 1e-12
+    assert r["noalloc_rc"] == 0 and r["noalloc_same"] and r["noalloc_flag"] == 0.0
+
+
+@pytest.mark.parametrize("case", ["identify_1", "identify_3"])
+def test_emu_identify_estimate(emu_runs, case):
+    """mpsft_identify_estimate on the oracle's bin matrices: decisions and
+    k1 equal, coefficients and decision margins to rounding."""
+    r = result(emu_runs, case)
+    assert r["same_decide"] and r["same_k1"] and r["n_found"] > 10
+    assert r["coef_err"] < 1e-12 and r["margin_err"] < 1e-12
 
 
 @pytest.mark.skipif(os.environ.get("MPSFT_SLOW_TESTS") != "1",
