@@ -23,6 +23,7 @@ if ROOT not in sys.path:
 
 N, K, SIGMA, DELTA = 1 << 22, 1000, 1e-7, 1e-6
 SIGNAL_SEED, SOLVE_SEED = 2, 11
+BATCH_SEED0 = 100  # signal seeds of the batched form
 HBM_PEAK_GBS = 8000.0
 METRIC = "µs per MPSFT solve N=2^22 S=1000: mixed vs FP64 GPU at 1e-6 coef err"
 
@@ -70,6 +71,51 @@ def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
     return line_bytes, logical, len(bins_log)
 
 
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz (CDNA LDS read/write port)
+FP64_PEAK_TFS = 78.6
+
+
+def fft_lds_bytes(B):
+    """LDS bytes one k_bin_fft workgroup moves for one row's Bluestein DFT
+    of B points over M = 2^ceil(log2(2B-1)) (the count of its loads and
+    stores of 16-byte elements).  M <= 1024: twiddles staged (M/2), the
+    bit-reversed input (M), radix-4 passes (each M loads + M stores + 3M/4
+    twiddle loads) forward and inverse, the filter product (2M), the result
+    (2B) and its read-out (B).  M = 2048, 4096: three Stockham passes each
+    way (the forward one starts from registers: 6 stores, 5 loads of M), the
+    chirp (B stored, B read), the result (2B) and its read-out (B)."""
+    logM = max(1, int(np.ceil(np.log2(2 * B - 1))))
+    Mm = 1 << logM
+    if logM <= 10:
+        passes = (logM + 1) // 2
+        el = Mm / 2 + Mm + 2 * passes * (2 * Mm + 0.75 * Mm) + 2 * Mm + 3 * B
+    else:
+        el = 11 * Mm + 5 * B
+    return el * 16, 2 * 5 * Mm * logM  # bytes, FFT flops (two M-point FFTs)
+
+
+def fft_throughput(M, solver, fft_ms):
+    """The bin FFT of the last (instrumented) solve against the LDS and FP64
+    peaks: model LDS bytes and FFT flops of every launch (rows x trials
+    Bluestein DFTs) over the event-timed k_bin_fft time."""
+    bins_log, _ = solver.log()
+    lds = flops = 0.0
+    for B in bins_log:
+        B = int(B)
+        rows = 1 + 2 * M.num_bits(N, B)
+        b, f = fft_lds_bytes(B)
+        lds += rows * b
+        flops += rows * f
+    t = fft_ms / 1000.0
+    busy_cus = 1 + 2 * M.num_bits(N, int(bins_log[0]))
+    return {"launches": len(bins_log), "avg_launch_us": round(fft_ms * 1000.0 / len(bins_log), 2),
+            "lds_model_bytes_per_solve": int(lds), "lds_GBs": round(lds / t / 1e9, 1),
+            "lds_peak_GBs": LDS_PEAK_GBS, "lds_frac": round(lds / t / 1e9 / LDS_PEAK_GBS, 4),
+            "lds_frac_of_busy_cus": round(lds / t / 1e9 / (LDS_PEAK_GBS * busy_cus / 256), 4),
+            "fft_GFLOPs": round(flops / t / 1e9, 1),
+            "fft_frac_of_fp64_peak": round(flops / t / 1e12 / FP64_PEAK_TFS, 5)}
+
+
 def large_workload(M, steps, warmup):
     """N=2^26 complex64, S=4000 unit modes (1000 within +-64 bins of 10
     centres), sigma=1e-6, MIXED: us per solve and the gather-fold's bytes/s."""
@@ -91,8 +137,21 @@ def large_workload(M, steps, warmup):
     _, ms = s.solve_profiled(xd, SOLVE_SEED)
     bins_log, _ = s.log()
     line_bytes, logical, n_launch = gather_bytes(M, s, 8, n=n)
+    # the FP64 form on the same complex64-rounded samples
+    del xd
+    s64 = M.Solver(n, M.solve_options(k, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
+    xd64 = s64.upload(x64.astype(np.complex128))
+    M.synchronize()
+    log("timing the N=2^26 FP64 solve on the same input")
+    us64, iters64 = time_solves(M, s64, xd64, steps, warmup)
+    got64 = s64.modes_dict()
+    rel64 = max((abs(got[q] - v) / abs(v) for q, v in got64.items() if q in got), default=None)
     t = ms[0] / 1000.0
     return {
+        "fp64_same_input_us": round(us64, 1), "fp64_same_input_iterations": iters64,
+        "mixed_vs_fp64_same_input": round(us64 / us, 3),
+        "support_identical_mixed_vs_fp64": set(got) == set(got64),
+        "max_rel_coef_diff_mixed_vs_fp64": rel64,
         "workload": "MPSFT solve N=2^26 S=4000 (25% in 10 clusters of +-64) unit |c| sigma=1e-6, "
                     "complex64 MIXED",
         "us_per_solve": round(us, 1), "solves_per_sec": round(1e6 / us, 1), "iterations": iters,
@@ -232,18 +291,43 @@ def adjoint_workload(M, steps, warmup, count=32):
     }
 
 
-def cpu_baseline(x, sample_solves):
-    """The FP64 CPU restatement of the reference (single thread, V4-order
-    binning) on the plain FP64 workload."""
+def cpu_model():
+    """The host CPU's model name (lscpu) and logical core count."""
+    import subprocess
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, os.cpu_count()
+
+
+def host_threads():
+    """Threads for the all-core CPU baseline: the CPU share this process is
+    given (OMP_NUM_THREADS where set), at most the logical cores."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_solve_time(x, k, sigma, threads, solves, seed):
+    """Mean wall time of `solves` oracle Demo1::Run solves (the FP64 CPU
+    restatement of the reference: BinInTime V1 order, BinInFreq V1,
+    closed-form pencil) on `threads` threads (OpenMP over bin rows)."""
     from oracle import pyoracle as O
-    O.set_threads(1)
+    O.set_threads(threads)
     t0 = time.perf_counter()
     iters = 0
-    for _ in range(sample_solves):
-        O.seed(SOLVE_SEED)
-        _, bl, _ = O.demo1_run(x, K, SIGMA)
+    for _ in range(solves):
+        O.seed(seed)
+        _, bl, _ = O.demo1_run(x, k, sigma)
         iters = len(bl)
-    dt = (time.perf_counter() - t0) / sample_solves
+    dt = (time.perf_counter() - t0) / solves
     O.set_threads(0)
     return dt * 1e6, iters
 
@@ -274,19 +358,34 @@ def traffic_from_profile():
     return None
 
 
+def time_batched(M, b, xd, seeds_for, steps, warmup):
+    """Mean seconds per batched solve; seeds_for(step) gives the step's
+    solve seeds (fresh ones per step)."""
+    for w in range(warmup):
+        b.solve(xd, seeds_for(-1 - w))
+    M.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b.solve(xd, seeds_for(i))
+    M.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-solves", type=int, default=5)
+    ap.add_argument("--batch-count", type=int, default=16,
+                    help="signals per batched solve (the timed form)")
+    ap.add_argument("--cpu-solves", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the instrumented MIXED solve (for rocprofv3 runs)")
     ap.add_argument("--solve-only", action="store_true",
-                    help="only the timed MIXED solves, as the metric runs them (for rocprofv3 "
-                         "kernel statistics of the solve loop)")
+                    help="only timed single-signal MIXED solves (for rocprofv3 kernel statistics "
+                         "of the solve loop)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
     ap.add_argument("--config3", action="store_true",
@@ -298,7 +397,7 @@ def main():
                          "(its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (one signal's solve per device); "
+        print("bench.py: this build measures one GPU only (every box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
@@ -340,59 +439,118 @@ def main():
         print(json.dumps({"us_per_solve": round(us, 2), "iterations": it, "steps": args.steps,
                           "warmup": args.warmup}), flush=True)
         return
+
+    # ---- the timed form: one batched solve of G config-2 signals per step
+    G = args.batch_count
+    log(f"generating {G} config-2 signals for the batched form")
+    xs64, xs128q, sigs = [], [], []
+    for g in range(G):
+        xg, _ = make_signal(N, K, SIGMA, seed=BATCH_SEED0 + g, magnitude="loguniform")
+        xg64 = xg.astype(np.complex64)
+        sigs.append(float(np.hypot(SIGMA, quantization_sigma(xg, xg64))))
+        xs64.append(xg64)
+        xs128q.append(xg64.astype(np.complex128))
+    opts = [M.solve_options(K, s_, window_delta=DELTA) for s_ in sigs]
+
+    def seeds_for(step):  # fresh solve seeds every step (warm-up steps negative)
+        return [(SOLVE_SEED + 7919 * (step + 10) + g) & 0x7fffffff for g in range(G)]
+
+    bm = M.Batch([N] * G, opts, M.MPSFT_MIXED)
+    xdb = bm.upload(xs64)
+    log("timing the batched MIXED form")
+    t_bm = time_batched(M, bm, xdb, seeds_for, args.steps, args.warmup)
+    # identical support against the FP64 single-signal form on the same
+    # complex64-rounded input, for every signal of the last step's seeds
+    last = seeds_for(args.steps - 1)
+    bm_modes = [bm.modes_dict(g) for g in range(G)]
+    del xdb
+    bf = M.Batch([N] * G, opts, M.MPSFT_FP64)
+    xdf = bf.upload(xs128q)
+    log("timing the batched FP64 form (same inputs)")
+    t_bf = time_batched(M, bf, xdf, seeds_for, args.steps, args.warmup)
+    del xdf, bf
+    same_support, max_rel = 0, 0.0
+    s_chk = M.Solver(N, opts[0], M.MPSFT_FP64)
+    for g in range(G):
+        s_chk = M.Solver(N, opts[g], M.MPSFT_FP64)
+        s_chk.solve(s_chk.upload(xs128q[g]), last[g])
+        ref = s_chk.modes_dict()
+        if set(ref) == set(bm_modes[g]):
+            same_support += 1
+            max_rel = max([max_rel] + [abs(bm_modes[g][q] - v) / abs(v) for q, v in ref.items()])
+    del s_chk
+
+    # ---- single-signal forms on the headline signal
     fp64q = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
     fp64 = M.Solver(N, M.solve_options(K, SIGMA, window_delta=DELTA), M.MPSFT_FP64)
     xd64 = mixed.upload(x64)
     xd128q = fp64q.upload(x64.astype(np.complex128))
     xd128 = fp64.upload(x)
     M.synchronize()
+    log("timing the single-signal forms")
 
-    log("timing")
-    us_mixed, it_mixed = time_solves(M, mixed, xd64, args.steps, args.warmup)
-    us_fp64q, it_fp64q = time_solves(M, fp64q, xd128q, args.steps, args.warmup)
-    us_fp64, it_fp64 = time_solves(M, fp64, xd128, args.steps, args.warmup)
+    def time_fresh(solver, xd):
+        for w in range(args.warmup):
+            solver.solve(xd, seeds_for(-1 - w)[0])
+        M.synchronize()
+        t0 = time.perf_counter()
+        iters = []
+        for i in range(args.steps):
+            iters.append(solver.solve(xd, seeds_for(i)[0]))
+        M.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e6, float(np.mean(iters))
 
-    # Accuracy of the timed result: MIXED vs FP64 on the same input.
-    gm, gq = mixed.modes_dict(), fp64q.modes_dict()
-    same_support = set(gm) == set(gq)
-    rel = [abs(gm[k] - gq[k]) / abs(gq[k]) for k in gq if k in gm]
-    max_rel = max(rel) if rel else None
-    hits = sum(1 for k in truth if k in gm)
-    g64 = fp64.modes_dict()
-    hits64 = sum(1 for k in truth if k in g64)
-    truth_err = max(abs(gm[k] - v) / abs(v) for k, v in truth.items() if k in gm)
+    us_mixed, it_mixed = time_fresh(mixed, xd64)
+    us_mixed_fixed, it_mixed_fixed = time_solves(M, mixed, xd64, args.steps, args.warmup)
+    us_fp64q, it_fp64q = time_fresh(fp64q, xd128q)
+    us_fp64, it_fp64 = time_fresh(fp64, xd128)
+
+    # accuracy at the fixed seed: MIXED vs FP64 on the same input, and truth
+    mixed.solve(xd64, SOLVE_SEED)
+    fp64q.solve(xd128q, SOLVE_SEED)
+    fp64.solve(xd128, SOLVE_SEED)
+    gm, gq, g64 = mixed.modes_dict(), fp64q.modes_dict(), fp64.modes_dict()
+    rel = [abs(gm[q] - gq[q]) / abs(gq[q]) for q in gq if q in gm]
+    hits = sum(1 for q in truth if q in gm)
+    truth_err = max(abs(gm[q] - v) / abs(v) for q, v in truth.items() if q in gm)
+    us_batch = t_bm / G * 1e6
 
     line = {
         "metric": METRIC,
-        "value": round(us_mixed, 2),
+        "value": round(us_batch, 2),
         "unit": "us/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(us_mixed / 1000.0, 4),
+        "ms_per_step": round(t_bm * 1e3, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 input; f64 phase/FFT/pencil",
+        "dtype": "c64 input; f64 arithmetic",
         "data": "synthetic",
-        "config": {"workload": "MPSFT solve N=2^22 S=1000 |c| log-uniform [0.1,10] sigma=1e-7",
-                   "n": N, "k": K, "sigma": SIGMA, "sigma_eff": sig_eff, "window_delta": DELTA,
-                   "solve_seed": SOLVE_SEED, "signal_seed": SIGNAL_SEED},
-        "solves_per_sec": round(1e6 / us_mixed, 1),
-        "iterations": it_mixed,
-        "fp64_same_input_us": round(us_fp64q, 2),
-        "fp64_same_input_iterations": it_fp64q,
-        "mixed_speedup_vs_fp64": round(us_fp64q / us_mixed, 3),
-        "fp64_c128_sigma_us": round(us_fp64, 2),
-        "fp64_c128_sigma_iterations": it_fp64,
-        "mixed_speedup_vs_plain_fp64": round(us_fp64 / us_mixed, 3),
-        "true_modes_recovered_plain_fp64": hits64,
-        "true_support_identical_mixed_vs_plain_fp64":
-            {k for k in truth if k in gm} == {k for k in truth if k in g64},
-        "support_identical_mixed_vs_fp64": same_support,
-        "max_rel_coef_diff_mixed_vs_fp64": max_rel,
+        "config": {"workload": f"MPSFT solves N=2^22 S=1000 |c| log-uniform [0.1,10] sigma=1e-7, "
+                               f"complex64 MIXED, {G} signals per batched solve (a step), fresh "
+                               "solve seeds per step",
+                   "n": N, "k": K, "sigma": SIGMA, "window_delta": DELTA, "signals_per_step": G,
+                   "signal_seeds": [BATCH_SEED0, BATCH_SEED0 + G - 1]},
+        "solves_per_sec": round(1e6 / us_batch, 1),
+        "batched_fp64_same_input_us": round(t_bf / G * 1e6, 2),
+        "mixed_vs_fp64_batched": round(t_bf / t_bm, 3),
+        "batched_support_identical_to_single_fp64": f"{same_support}/{G}",
+        "batched_max_rel_coef_diff_vs_single_fp64": max_rel,
+        "single_solve": {
+            "mixed_us_fresh_seeds": round(us_mixed, 2), "mixed_mean_iterations": it_mixed,
+            "mixed_us_fixed_seed": round(us_mixed_fixed, 2), "fixed_seed_iterations": it_mixed_fixed,
+            "fp64_same_input_us": round(us_fp64q, 2), "fp64_same_input_mean_iterations": it_fp64q,
+            "fp64_c128_sigma_us": round(us_fp64, 2), "fp64_c128_sigma_mean_iterations": it_fp64,
+            "sigma_eff": sig_eff, "signal_seed": SIGNAL_SEED,
+        },
+        "mixed_batched_speedup_vs_single_fp64_same_input": round(us_fp64q / us_batch, 3),
+        "support_identical_mixed_vs_fp64_single": set(gm) == set(gq),
+        "max_rel_coef_diff_mixed_vs_fp64_single": max(rel) if rel else None,
         "true_modes_recovered": hits,
         "max_rel_coef_err_vs_truth": truth_err,
+        "true_modes_recovered_plain_fp64": sum(1 for q in truth if q in g64),
     }
 
     if not args.no_roofline:
@@ -411,14 +569,25 @@ def main():
         }
         line["kernel_ms_per_solve"] = {"k_bin_time": round(ms[0], 4), "k_bin_fft": round(ms[1], 4),
                                        "k_identify": round(ms[2], 4)}
+        line["bin_fft"] = fft_throughput(M, mixed, ms[1])
 
     if not args.no_cpu_baseline:
-        log(f"CPU baseline: {args.cpu_solves} single-thread oracle solves")
-        us_cpu, it_cpu = cpu_baseline(x, args.cpu_solves)
+        threads = host_threads()
+        model, ncpu = cpu_model()
+        xq = x64.astype(np.complex128)
+        log(f"CPU baseline: oracle solves of the headline workload on {threads} and 1 threads")
+        us_all, it_all = cpu_solve_time(xq, K, sig_eff, threads, args.cpu_solves, SOLVE_SEED)
+        us_one, _ = cpu_solve_time(xq, K, sig_eff, 1, max(1, args.cpu_solves - 1), SOLVE_SEED)
+        us_plain, it_plain = cpu_solve_time(x, K, SIGMA, 1, 1, SOLVE_SEED)
         line["cpu_baseline"] = {
-            "value": round(us_cpu, 1), "unit": "us/solve", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_solves} full FP64 solves of the same N=2^22 S=1000 signal "
-                      f"(complex128, sigma=1e-7, {it_cpu} iterations each)",
+            "value": round(us_all, 1), "unit": "us/solve", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_solves} solves of the headline signal (complex64-rounded, "
+                      f"sigma_eff, solve seed {SOLVE_SEED}, {it_all} iterations) by the oracle, "
+                      "OpenMP over bin rows",
+            "single_core_us": round(us_one, 1),
+            "plain_fp64_single_core_us": round(us_plain, 1),
+            "plain_fp64_iterations": it_plain,
+            "cpu_model": model, "logical_cpus": ncpu,
         }
     print(json.dumps(line), flush=True)
 

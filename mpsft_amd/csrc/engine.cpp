@@ -78,6 +78,9 @@ int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f, int32_t forced) {
   return R;
 }
 
+// Transforms drawn before a solve's first launch.
+constexpr int32_t kFirstDraws = 32;
+
 int32_t ChunkIters() {
   const char *e = std::getenv("MPSFT_LOOKAHEAD");
   if (e) {
@@ -315,6 +318,8 @@ Engine::~Engine() {
   (void)hipFree(d_idout_);
   (void)hipFree(d_pa_);
   if (h_state_) (void)hipHostFree(h_state_);
+  if (p_tfs_) (void)hipHostFree(p_tfs_);
+  if (p_qs_) (void)hipHostFree(p_qs_);
   if (h_progress_) (void)hipHostFree(const_cast<int32_t *>(h_progress_));
 }
 
@@ -526,6 +531,17 @@ int Engine::GrowTransforms(int32_t count, bool *grown) {
   const int32_t cap = std::max(count, std::max(1024, 2 * tf_cap_));
   if (Alloc(&d_tfs_, cap) != MPSFT_OK || Alloc(&d_qs_, size_t(cap) * trials_cap_) != MPSFT_OK)
     return MPSFT_ERR_NOMEM;
+  // The pinned staging copies: no upload from the old ones is in flight
+  // (callers drain the stream before the arrays grow).
+  if (p_tfs_) (void)hipHostFree(p_tfs_);
+  if (p_qs_) (void)hipHostFree(p_qs_);
+  p_tfs_ = nullptr;
+  p_qs_ = nullptr;
+  if (hipHostMalloc(reinterpret_cast<void **>(&p_tfs_), size_t(cap) * sizeof(DevTf)) !=
+          hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&p_qs_),
+                    size_t(cap) * trials_cap_ * sizeof(int64_t)) != hipSuccess)
+    return MPSFT_ERR_NOMEM;
   tf_cap_ = cap;
   if (tf_qstride_ != trials_cap_) tf_valid_ = false;  // the host copies' q layout changed
   tf_qstride_ = trials_cap_;
@@ -540,7 +556,11 @@ int Engine::GrowTransforms(int32_t count, bool *grown) {
 // For n = 2^k the multiplier is drawn odd, a = 2 (r mod n/2) + 1; for other
 // composite n the draw of a repeats until gcd(a, n) = 1.  Draws already made
 // for the same seed are kept (host copies) and only the new ones are
-// uploaded, unless the device arrays were reallocated.
+// uploaded, unless the device arrays were reallocated.  The uploads read
+// pinned copies and are not waited for: the range they read is rewritten
+// only by a later solve's draws (this solve's uploads precede its kernels,
+// and its stop is reported by a kernel) or after the arrays grow (callers
+// drain the stream first).
 int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s) {
   bool grown = false;
   const int rc = GrowTransforms(count, &grown);
@@ -572,13 +592,13 @@ int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipSt
       h_qs_.push_back(t < trials ? int64_t(RandomInt32() % n) : 0);
   }
   const int32_t from = (fresh || grown) ? 0 : tf_count_;
-  HIP_OK(hipMemcpyAsync(d_tfs_ + from, h_tfs_.data() + from,
-                        size_t(target - from) * sizeof(DevTf), hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(d_qs_ + size_t(from) * trials_cap_, h_qs_.data() + size_t(from) * trials_cap_,
-                        size_t(target - from) * trials_cap_ * sizeof(int64_t),
+  const size_t q0 = size_t(from) * trials_cap_, q1 = size_t(target) * trials_cap_;
+  std::copy(h_tfs_.begin() + from, h_tfs_.begin() + target, p_tfs_ + from);
+  std::copy(h_qs_.begin() + q0, h_qs_.begin() + q1, p_qs_ + q0);
+  HIP_OK(hipMemcpyAsync(d_tfs_ + from, p_tfs_ + from, size_t(target - from) * sizeof(DevTf),
                         hipMemcpyHostToDevice, s));
-  // The host vectors may move on the next draw: the copies finish first.
-  HIP_OK(hipStreamSynchronize(s));
+  HIP_OK(hipMemcpyAsync(d_qs_ + q0, p_qs_ + q0, (q1 - q0) * sizeof(int64_t),
+                        hipMemcpyHostToDevice, s));
   tf_count_ = target;
   tf_valid_ = true;
   return MPSFT_OK;
@@ -774,7 +794,10 @@ int Engine::PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t
     prep_epoch_ = tab_epoch_;
   }
   const int32_t max_iter = opt.max_iter > 0 ? opt.max_iter : 100000;
-  rc = EnsureTransforms(seed, std::min(max_iter, std::max(256, ChunkIters())), opt.trials, s);
+  // A few graphs' worth up front; the launch loop draws ahead in doubling
+  // batches while the device runs.
+  rc = EnsureTransforms(seed, std::min(max_iter, std::max(kFirstDraws, ChunkIters())), opt.trials,
+                        s);
   if (rc != MPSFT_OK) return rc;
   *a = InitArgs{};
   a->x = x;

@@ -334,6 +334,10 @@ class Engine {
   }
   std::vector<DevTf> h_tfs_;
   std::vector<int64_t> h_qs_;
+  // pinned copies of the draws (tf_cap_ and tf_cap_ x trials_cap_ entries):
+  // the uploads read them asynchronously, so a solve never waits for them
+  DevTf *p_tfs_ = nullptr;
+  int64_t *p_qs_ = nullptr;
 
   // graph of graph_iters_ iterations for the current shapes
   // [1]: without the four-step FFT; [2], [3]: the late iterations at

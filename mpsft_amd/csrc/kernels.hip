@@ -74,15 +74,24 @@ __device__ __forceinline__ double2 Widen(double2 v) { return v; }
 // gathers are random 8- or 16-byte reads with little reuse within a launch,
 // and loading them as streaming data measured faster (N=2^22 S=1000 solve
 // 521 -> 507 us, N=2^26 S=4000 1227 -> 1186 us).
+// x comes to the kernels as a pointer stored in device memory (the loop
+// state), which the compiler cannot place in an address space: its loads
+// would be flat loads, which also count against the LDS wait counter, so
+// every LDS wait after the gathers are issued would wait for them too.
+// They are made global loads explicitly, through pointers cast to the
+// global address space.
+typedef const unsigned long long __attribute__((address_space(1))) *GlobalU64;
+typedef const double __attribute__((address_space(1))) *GlobalF64;
+
 __device__ __forceinline__ float2 LoadStream(const float2 *p) {
-  const unsigned long long v =
-      __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+  const unsigned long long v = __builtin_nontemporal_load((GlobalU64)p);
   float2 r;
   __builtin_memcpy(&r, &v, sizeof(r));
   return r;
 }
 __device__ __forceinline__ double2 LoadStream(const double2 *p) {
-  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+  const GlobalF64 q = (GlobalF64)p;
+  return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 __device__ __forceinline__ double2 Widen(float2 v) { return make_double2(double(v.x), double(v.y)); }
 
