@@ -291,6 +291,46 @@ def adjoint_workload(M, steps, warmup, count=32):
     }
 
 
+NEAR_MARGIN = 1e-9
+
+
+def first_iteration_decisions(M, x64, sig, seed=SOLVE_SEED):
+    """The first iteration's per-bin decisions on the device (B = 2K + 1,
+    thresholds 3 sigma/sqrt(B) and sigma/sqrt(B) from sigma_eff, one trial,
+    a fixed transform and q) for the MIXED form on x64 and the FP64 form on
+    the same complex64-rounded input: each form's decisions and relative
+    margins (distance from the nearest threshold a decision compared with)."""
+    rng = np.random.default_rng(seed)
+    a = int(rng.integers(0, N // 2)) * 2 + 1
+    b, c, q = (int(v) for v in rng.integers(0, N, 3))
+    B = 2 * K + 1
+    bits = M.num_bits(N, B)
+    tf = M.transform_make(N, a, b, c)
+    out = {}
+    for name, prec, xin in (("mixed", M.MPSFT_MIXED, x64),
+                            ("fp64", M.MPSFT_FP64, x64.astype(np.complex128))):
+        p = M.Plan(N, B, DELTA, bits, prec)
+        A = p.bin_in_time(xin, tf, q)
+        out[name] = p.identify_estimate(A[None], tf, [q], 3 * sig / np.sqrt(B),
+                                        sig / np.sqrt(B), 0.1)
+        del p
+    return dict(tf=(a, b, c), q=q, B=B, **out)
+
+
+def margin_summary(d):
+    m = np.asarray(d["margin"])
+    return {"near_margin_decisions": int(np.sum(m < NEAR_MARGIN)),
+            "min_margin": float(m.min()), "found": int(np.sum(d["decide"] == 3))}
+
+
+def decision_flips(d0, d1):
+    """Bins whose decision or located frequency differ between two records."""
+    diff = (np.asarray(d0["decide"]) != np.asarray(d1["decide"]))
+    found = (np.asarray(d0["decide"]) == 3) & (np.asarray(d1["decide"]) == 3)
+    diff |= found & (np.asarray(d0["k1"]) != np.asarray(d1["k1"]))
+    return int(np.sum(diff))
+
+
 def cpu_model():
     """The host CPU's model name (lscpu) and logical core count."""
     import subprocess
@@ -318,18 +358,34 @@ def host_threads():
 def cpu_solve_time(x, k, sigma, threads, solves, seed):
     """Mean wall time of `solves` oracle Demo1::Run solves (the FP64 CPU
     restatement of the reference: BinInTime V1 order, BinInFreq V1,
-    closed-form pencil) on `threads` threads (OpenMP over bin rows)."""
+    closed-form pencil) on `threads` threads (OpenMP over bin rows).
+    Returns (us per solve, iterations, the last solve's modes)."""
     from oracle import pyoracle as O
     O.set_threads(threads)
     t0 = time.perf_counter()
-    iters = 0
+    iters, modes = 0, {}
     for _ in range(solves):
         O.seed(seed)
-        _, bl, _ = O.demo1_run(x, k, sigma)
+        mm, bl, _ = O.demo1_run(x, k, sigma)
         iters = len(bl)
+        modes = mm.to_dict()
     dt = (time.perf_counter() - t0) / solves
     O.set_threads(0)
-    return dt * 1e6, iters
+    return dt * 1e6, iters, modes
+
+
+def oracle_first_iteration(xq, sig, dec):
+    """The oracle's Iterate (one trial) on the complex64-rounded input with
+    the transform and q of first_iteration_decisions: its decisions and
+    margins, for counting flips against the device's."""
+    from oracle import pyoracle as O
+    B = dec["B"]
+    opt = O.IterateOptions(bins=B, window_delta=DELTA, trials=1,
+                           bin_threshold=3 * sig / np.sqrt(B), window_threshold=0.1,
+                           sv_threshold=sig / np.sqrt(B))
+    tf = O.transform_make(N, *dec["tf"])
+    _, tr = O.iterate(xq, opt, O.ModeMap(), tf=tf, qs=np.array([dec["q"]], np.int32))
+    return tr
 
 
 def copy_calibration():
@@ -386,6 +442,8 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="only timed single-signal MIXED solves (for rocprofv3 kernel statistics "
                          "of the solve loop)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the timed batched MIXED solves (for rocprofv3 kernel statistics)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales rocprofv3 byte counters)")
     ap.add_argument("--config3", action="store_true",
@@ -459,6 +517,10 @@ def main():
     xdb = bm.upload(xs64)
     log("timing the batched MIXED form")
     t_bm = time_batched(M, bm, xdb, seeds_for, args.steps, args.warmup)
+    if args.batched_only:
+        print(json.dumps({"us_per_solve": round(t_bm / G * 1e6, 2), "signals": G,
+                          "steps": args.steps, "warmup": args.warmup}), flush=True)
+        return
     # identical support against the FP64 single-signal form on the same
     # complex64-rounded input, for every signal of the last step's seeds
     last = seeds_for(args.steps - 1)
@@ -553,6 +615,15 @@ def main():
         "true_modes_recovered_plain_fp64": sum(1 for q in truth if q in g64),
     }
 
+    log("first-iteration decision margins (MIXED and FP64 on the same input)")
+    dec = first_iteration_decisions(M, x64, sig_eff)
+    line["decisions"] = {
+        "what": f"first iteration, B={dec['B']}, one trial, fixed transform; near = relative "
+                f"margin below {NEAR_MARGIN:g}",
+        "mixed": margin_summary(dec["mixed"]), "fp64_same_input": margin_summary(dec["fp64"]),
+        "flips_mixed_vs_fp64": decision_flips(dec["mixed"], dec["fp64"]),
+    }
+
     if not args.no_roofline:
         log("profiled solve for the roofline")
         it_p, ms = mixed.solve_profiled(xd64, SOLVE_SEED)
@@ -576,9 +647,22 @@ def main():
         model, ncpu = cpu_model()
         xq = x64.astype(np.complex128)
         log(f"CPU baseline: oracle solves of the headline workload on {threads} and 1 threads")
-        us_all, it_all = cpu_solve_time(xq, K, sig_eff, threads, args.cpu_solves, SOLVE_SEED)
-        us_one, _ = cpu_solve_time(xq, K, sig_eff, 1, max(1, args.cpu_solves - 1), SOLVE_SEED)
-        us_plain, it_plain = cpu_solve_time(x, K, SIGMA, 1, 1, SOLVE_SEED)
+        us_all, it_all, orc_modes = cpu_solve_time(xq, K, sig_eff, threads, args.cpu_solves,
+                                                   SOLVE_SEED)
+        us_one, _, _ = cpu_solve_time(xq, K, sig_eff, 1, max(1, args.cpu_solves - 1), SOLVE_SEED)
+        us_plain, it_plain, _ = cpu_solve_time(x, K, SIGMA, 1, 1, SOLVE_SEED)
+        # the same solves' results against the device's (same seed, same input)
+        sym = {f: len(set(orc_modes) ^ set(g)) for f, g in (("mixed", gm), ("fp64", gq))}
+        rel_o = [abs(gq[q] - v) / abs(v) for q, v in orc_modes.items() if q in gq]
+        tr = oracle_first_iteration(xq, sig_eff, dec)
+        line["decisions"]["oracle"] = {
+            "near_margin_decisions": int(np.sum(tr["margin"] < NEAR_MARGIN)),
+            "min_margin": float(tr["margin"].min()),
+            "flips_vs_mixed": decision_flips(tr, dec["mixed"]),
+            "flips_vs_fp64": decision_flips(tr, dec["fp64"]),
+            "solve_support_differences": sym,
+            "solve_max_rel_coef_diff_fp64": max(rel_o) if rel_o else None,
+        }
         line["cpu_baseline"] = {
             "value": round(us_all, 1), "unit": "us/solve", "cores": threads, "kind": "port",
             "sample": f"{args.cpu_solves} solves of the headline signal (complex64-rounded, "

@@ -39,10 +39,15 @@ typedef enum mpsft_status {
   MPSFT_ERR_NOT_CONVERGED = 7 /* solve hit its iteration cap */
 } mpsft_status;
 
-/* Arithmetic forms.  FP64: every stage in double, input complex128, equal
- * to the reference.  MIXED: input complex64; gathers, window products and
- * the fold in FP32 with compensated accumulation; phases, bin FFT, pencil
- * and rounding in FP64. */
+/* Arithmetic forms.  FP64: input complex128, every stage in double, equal
+ * to the reference.  MIXED: input complex64 (half the bytes in device
+ * memory); each gathered sample is widened exactly to double and every
+ * stage after it (window products, fold, bin FFT, pencil, estimate, rounding)
+ * runs in double, so MIXED equals FP64 run on the complex64-rounded samples.
+ * (FP32 window products or bin FFTs leave errors at the noise level per bin
+ * and change the pencil's decisions; a double-float form that does not costs
+ * more than FP64 on this part.)  Thresholds: pass sigma_eff = sqrt(sigma^2 +
+ * sigma_q^2), sigma_q the rms rounding error of the complex64 samples. */
 typedef enum mpsft_precision {
   MPSFT_FP64 = 0,
   MPSFT_MIXED = 1

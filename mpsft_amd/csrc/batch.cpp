@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 
 namespace mpsft {
 
@@ -14,6 +15,16 @@ namespace {
   do {                                               \
     if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
   } while (0)
+
+// Workgroups of each packed batched launch (default: 8 per CU of MI355X's
+// 256), and bins per k_identify unit (default 16: four per wavefront), each
+// overridable for measurement by MPSFT_PACKED_GRID / MPSFT_IDENT_BINS.
+int32_t EnvInt(const char *name, int32_t def, int32_t lo, int32_t hi) {
+  const char *e = std::getenv(name);
+  if (!e) return def;
+  const int v = std::atoi(e);
+  return v >= lo && v <= hi ? v : def;
+}
 
 template <typename T>
 int AllocDev(T **p, size_t count) {
@@ -73,6 +84,8 @@ Batch::~Batch() {
   (void)hipFree(d_args_);
   (void)hipFree(d_map_);
   (void)hipFree(d_nblk_ident_);
+  (void)hipFree(d_work_map_);
+  (void)hipFree(d_work_i32_);
   (void)hipFree(d_counters_);
   (void)hipFree(d_results_);
   (void)hipFree(d_adj_sig_);
@@ -142,7 +155,9 @@ int Batch::Build(hipStream_t s) {
                    hipMemcpyHostToDevice));
   adj_nb_fft_ = int32_t(map_adj.size());
   const size_t total = map_time.size() + map_fft.size() + map_ident.size();
-  if (AllocDev(&d_map_, total) != MPSFT_OK || AllocDev(&d_nblk_ident_, eng_.size()) != MPSFT_OK)
+  const size_t K = eng_.size();
+  if (AllocDev(&d_map_, total) != MPSFT_OK || AllocDev(&d_nblk_ident_, K) != MPSFT_OK ||
+      AllocDev(&d_work_map_, total) != MPSFT_OK || AllocDev(&d_work_i32_, 7 * K + 3) != MPSFT_OK)
     return MPSFT_ERR_NOMEM;
   HIP_OK(hipMemcpy(d_map_, map_time.data(), map_time.size() * sizeof(int2),
                    hipMemcpyHostToDevice));
@@ -160,6 +175,21 @@ int Batch::Build(hipStream_t s) {
   bl_.nb_time = int32_t(map_time.size());
   bl_.nb_fft = int32_t(map_fft.size());
   bl_.nb_ident = int32_t(map_ident.size());
+  // The packed launches: at most kPackedGrid workgroups each (8 per CU),
+  // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
+  const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 2048, 64, 1 << 20);
+  bl_.ident_bins = EnvInt("MPSFT_IDENT_BINS", 16, 4, kIdentKeys) / 4 * 4;
+  bl_.grid_time = std::max(1, std::min(bl_.nb_time, grid));
+  bl_.grid_fft = std::max(1, std::min(bl_.nb_fft, grid));
+  bl_.grid_ident = std::max(1, std::min(bl_.nb_ident, grid));
+  bl_.count = int32_t(K);
+  bl_.work.cnt = d_work_i32_;
+  bl_.work.off = d_work_i32_ + 3 * K;
+  bl_.work.rows = d_work_i32_ + 6 * K;
+  bl_.work.total = d_work_i32_ + 7 * K;
+  bl_.work.map[0] = d_work_map_;
+  bl_.work.map[1] = d_work_map_ + map_time.size();
+  bl_.work.map[2] = d_work_map_ + map_time.size() + map_fft.size();
   bl_.lds_time = lds_time;
   bl_.lds_fft = lds_fft;
   bl_.counters = d_counters_;

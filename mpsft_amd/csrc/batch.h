@@ -62,6 +62,8 @@ class Batch {
   InitArgs *d_args_ = nullptr;
   int2 *d_map_ = nullptr;           // map_time | map_fft | map_ident
   int32_t *d_nblk_ident_ = nullptr;
+  int2 *d_work_map_ = nullptr;      // the packed maps of BatchWork
+  int32_t *d_work_i32_ = nullptr;   // BatchWork cnt | off | rows | total
   int32_t *d_counters_ = nullptr;   // {iterations started, signals stopped}
   BatchResultDev *d_results_ = nullptr;
   BatchResultDev *h_results_ = nullptr;  // pinned

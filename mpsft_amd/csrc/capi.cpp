@@ -18,6 +18,9 @@ struct mpsft_plan {
   int32_t bins = 0, bits = 0;
   double delta = 0;
   mpsft::BParams bp{};
+  // BinInFreq scratch: 4 integers per (row, bin) and the grid exponent
+  unsigned long long *d_limbs = nullptr;
+  int32_t *d_emax = nullptr;
 };
 
 struct mpsft_batch {
@@ -135,6 +138,13 @@ int mpsft_plan_create(int64_t n, int32_t bins, double delta, int32_t bits, int p
   p->delta = delta;
   p->bp = bp;
   p->bp.bits = bits;
+  const size_t limbs = size_t(1 + 2 * bits) * bins * 4;
+  if (hipMalloc(reinterpret_cast<void **>(&p->d_limbs), limbs * sizeof(unsigned long long)) !=
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&p->d_emax), sizeof(int32_t)) != hipSuccess) {
+    mpsft_plan_destroy(p);
+    return MPSFT_ERR_NOMEM;
+  }
   *plan = p;
   return MPSFT_OK;
 }
@@ -168,7 +178,9 @@ int mpsft_plan_delta(const mpsft_plan *plan, double *delta) {
 
 int mpsft_plan_destroy(mpsft_plan *plan) {
   if (!plan) return MPSFT_ERR_INVALID_ARG;
-  delete plan->eng;
+  delete plan->eng;  // waits for the device
+  (void)hipFree(plan->d_limbs);
+  (void)hipFree(plan->d_emax);
   delete plan;
   return MPSFT_OK;
 }
@@ -193,7 +205,8 @@ int mpsft_bin_in_freq(mpsft_plan *plan, const int64_t *k0_dev, const void *coef_
   if (qq < 0) qq += n;
   return Hip(mpsft::LaunchBinInFreqDirect(n, mask, plan->bp, Reduce(n, tf), qq, k0_dev,
                                           static_cast<const double2 *>(coef_dev), m,
-                                          static_cast<double2 *>(out_dev), S(stream)));
+                                          static_cast<double2 *>(out_dev), plan->d_limbs,
+                                          plan->d_emax, S(stream)));
 }
 
 int mpsft_identify_estimate(mpsft_plan *plan, const void *a_dev, int32_t trials,

@@ -467,8 +467,10 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     if (bp.logM <= kMaxLogM) {
       small = true;
       // k_bin_fft: M <= 1024 array + M/2 twiddles; larger M the padded
-      // array, then the chirp
-      const size_t el = bp.logM <= kFftSmallLogM ? size_t(bp.M + bp.M / 2) : size_t(bp.M + bp.M / 16 + B);
+      // array (M + M/16).  The BinInFreq limbs of the B bins (four 8-byte
+      // integers per bin: 2B <= M double2 slots) later take the array's place.
+      const size_t el = bp.logM <= kFftSmallLogM ? size_t(bp.M + bp.M / 2)
+                                                 : size_t(bp.M + bp.M / 16);
       lfft = std::max(lfft, el * sizeof(double2));
     } else {
       big = true;

@@ -42,6 +42,7 @@ constexpr int kFftSmallLogM = 10;  // k_bin_fft's in-place radix-4 path up to M 
 constexpr int kMaxLogMBig = 16;    // four-step Bluestein FFT up to M = 65536
 constexpr int kTwLogM = kMaxLogMBig;  // twiddle table e^{-2 pi i j / 2^kTwLogM}
 constexpr int kBigTile = 1024;     // elements of one four-step workgroup
+constexpr int kIdentKeys = 64;     // new keys a k_identify workgroup lists before appending
 constexpr int kPrepBlocks = 16;    // k_bin_time workgroups computing mode terms
 constexpr int kMaxBins = 32767;    // largest B (2B-1 <= 2^kMaxLogMBig)
 constexpr int kTargetTiles = 512;  // k_bin_time tiles aimed for per trial

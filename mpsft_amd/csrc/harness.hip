@@ -8,7 +8,8 @@
 //   over every location i: err_i = |xh2_i - xhat_i| with xh2 the found
 //   coefficients (0 elsewhere): mean, rms and max.
 // The solver's found-mode map is dense (coef over all n locations, zero
-// where nothing was found), so xh2 is read directly.  Each workgroup sums
+// where nothing was found, and a presence bit each), so xh2 is read
+// directly.  Each workgroup sums
 // a fixed, strided share of the work and reduces it in a fixed tree; one
 // workgroup then adds the workgroups' partial sums in index order, so the
 // result is the same on every run (bitwise).  Bound: HBM, one pass over
@@ -108,20 +109,20 @@ __global__ void __launch_bounds__(kThreads)
   for (int f = 0; f < kPaFields; ++f) v[f] = 0.0;
   const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t gsize = int64_t(gridDim.x) * blockDim.x;
-  const int64_t cnt = min<int64_t>(c.st->count, c.cap);
-  for (int64_t i = gid; i < cnt; i += gsize) {
-    const int64_t k = c.keys[i];
-    const double e = Err(c.coef[k], xhat[k]);
-    const int o = InSorted(truth, m, k) ? 0 : 3;
-    v[o] += 1.0;
-    v[o + 1] += e;
-    v[o + 2] += e * e;
-  }
+  // One pass over the locations in a fixed order (not the key list, whose
+  // order depends on how identify's workgroups raced), so the sums do not
+  // depend on the run.
   for (int64_t i = gid; i < c.n; i += gsize) {
     const double e = Err(c.coef[i], xhat[i]);
     v[6] += e;
     v[7] += e * e;
     v[8] = fmax(v[8], e);
+    if ((c.present[i >> 5] >> (i & 31)) & 1u) {  // a found mode
+      const int o = InSorted(truth, m, i) ? 0 : 3;
+      v[o] += 1.0;
+      v[o + 1] += e;
+      v[o + 2] += e * e;
+    }
   }
   for (int64_t i = gid; i + 1 < m; i += gsize)
     if (!(truth[i] < truth[i + 1])) v[9] = 1.0;

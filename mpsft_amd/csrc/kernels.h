@@ -43,11 +43,25 @@ hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t 
 hipError_t LaunchPublish(const Ctx &c, int parity, hipStream_t s);
 
 // The batched form: per-stage launches over every signal of a batch.
+// BatchWork (device) is rewritten by each batched iteration's plan: per
+// launch k (0 k_bin_time, 1 k_bin_fft, 2 k_identify) the units of signal i
+// (cnt[k count + i] of them from off[k count + i]) and the packed map of
+// {signal, unit} entries, total[k] in all.
+struct BatchWork {
+  int32_t *cnt, *off;  // 3 x count each
+  int32_t *rows;       // count: the iteration's rows per signal
+  int32_t *total;      // 3
+  int2 *map[3];        // capacities: the units of every signal at its largest B
+};
 struct BatchLaunch {
   const Ctx *ctxs;                      // device, one per signal
   const int2 *map_time, *map_fft, *map_ident;  // device, {signal, local block} per block
   const int32_t *nblk_ident;            // device, k_identify blocks per signal
-  int32_t nb_time, nb_fft, nb_ident;    // blocks per launch
+  int32_t nb_time, nb_fft, nb_ident;    // blocks of the largest iteration of every signal
+  int32_t grid_time, grid_fft, grid_ident;  // workgroups of the packed launches
+  int32_t ident_bins;                   // bins per k_identify unit (a multiple of 4)
+  int32_t count;                        // signals
+  BatchWork work;
   size_t lds_time, lds_fft;             // dynamic LDS (max over signals)
   int32_t *counters;                    // device {iterations started, signals stopped}
   int32_t *host;                        // host-mapped copy of counters
@@ -106,10 +120,12 @@ hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s);
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);
+// BinInFreq on a caller's rows x B matrix; limbs: rows x B x 4 integers and
+// emax one int of scratch (the exact per-bin term sums).
 hipError_t LaunchBinInFreqDirect(int64_t n, int64_t mask, const BParams &bp,
                                  const DevTf &tf, int64_t q, const int64_t *k0,
                                  const double2 *coef, int64_t m, double2 *out,
-                                 hipStream_t s);
+                                 unsigned long long *limbs, int32_t *emax, hipStream_t s);
 hipError_t LaunchMatPencil(const double2 *u, int64_t m, double *sigma,
                            uint8_t *vote, hipStream_t s);
 

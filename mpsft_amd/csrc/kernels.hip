@@ -431,12 +431,18 @@ __device__ __forceinline__ void PublishStop(const Ctx &c, int32_t it, int32_t st
 }
 
 // Workgroup 0 of k_bin_time: publish the iteration, log the previous one,
-// report progress to the host.
+// report progress to the host.  PublishLoop: all of it but the copy of ip
+// into slot p (for a caller that built ip there).
+__device__ void PublishLoop(const Ctx &c, int p, const IterParams &ip, const LoopCounters &lc);
 __device__ void PublishIteration(const Ctx &c, int p, const IterParams &ip,
                                  const LoopCounters &lc) {
+  c.st->P[p] = ip;
+  PublishLoop(c, p, ip, lc);
+}
+
+__device__ void PublishLoop(const Ctx &c, int p, const IterParams &ip, const LoopCounters &lc) {
   DevState *st = c.st;
   const LoopCounters prev = st->L[1 - p];
-  st->P[p] = ip;
   if (!ip.done)
     for (int t = 0; t < st->trials; ++t)
       st->P[p].q[t] = c.qs[int64_t(lc.it) * c.trials_cap + t];
@@ -558,6 +564,38 @@ __device__ __forceinline__ double2 BitTwiddle(const Ctx &c, int32_t B, int bit, 
   return CisFrac(ModN((int64_t(B) << bit) * k1, c.n, c.mask), c.n);
 }
 
+// BinInFreqV1 (src/binner.cc:565-591) per found mode and trial: bin,
+// base = wf e^{2 pi i (q k1 + c k0)/n}, and per bit
+// g = e^{2 pi i (B 2^bit k1)/n}; the terms are coef base (row 0),
+// coef (base g) (row 2bit+1) and coef (base conj(g)) (row 2bit+2).
+// Workgroup blk of nblk.
+__device__ __forceinline__ void FreqTerms(const Ctx &c, const IterParams &ip, int32_t trials,
+                                          int32_t blk, int32_t nblk) {
+  const int32_t B = ip.B, bits = ip.bits;
+  const DevTf tf = ip.tf;
+  const int64_t n = c.n, mask = c.mask;
+  const int32_t cnt = min(c.st->count, c.cap);
+  for (int64_t i = int64_t(blk) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
+       i += int64_t(nblk) * blockDim.x) {
+    const int32_t trial = int32_t(i / cnt);
+    const int32_t mi = int32_t(i - int64_t(trial) * cnt);
+    const int64_t k0 = c.keys[mi];
+    const double2 cf = c.coef[k0];
+    const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
+    const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
+    int32_t bin;
+    const double2 base = FreqBase(c, ip, q, k0, k1, &bin);
+    c.term_bin[int64_t(trial) * c.cap + mi] = bin;
+    double2 *val = c.term_val + int64_t(trial) * c.rows_cap * c.cap + mi;
+    val[0] = Cmul(cf, base);
+    for (int32_t bit = 0; bit < bits; ++bit) {
+      const double2 g = BitTwiddle(c, B, bit, k1);
+      val[int64_t(2 * bit + 1) * c.cap] = Cmul(cf, Cmul(base, g));
+      val[int64_t(2 * bit + 2) * c.cap] = Cmul(cf, Cmul(base, Conj(g)));
+    }
+  }
+}
+
 template <typename Tx>
 __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double2 *lds) {
   const DevState *st = c.st;
@@ -567,35 +605,8 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
   if (blk == 0 && threadIdx.x == 0) PublishIteration(c, p, ip, lc);
   if (ip.done) return;
   const int32_t trials = st->trials;
-  const int32_t B = ip.B, bits = ip.bits;
-  const DevTf tf = ip.tf;
-  const int64_t n = c.n, mask = c.mask;
-
   if (blk < kPrepBlocks) {
-    // BinInFreqV1 (src/binner.cc:565-591) per found mode and trial: bin,
-    // base = wf e^{2 pi i (q k1 + c k0)/n}, and per bit
-    // g = e^{2 pi i (B 2^bit k1)/n}; the terms are coef base (row 0),
-    // coef (base g) (row 2bit+1) and coef (base conj(g)) (row 2bit+2).
-    const int32_t cnt = min(st->count, c.cap);
-    for (int64_t i = int64_t(blk) * blockDim.x + threadIdx.x; i < int64_t(cnt) * trials;
-         i += int64_t(kPrepBlocks) * blockDim.x) {
-      const int32_t trial = int32_t(i / cnt);
-      const int32_t mi = int32_t(i - int64_t(trial) * cnt);
-      const int64_t k0 = c.keys[mi];
-      const double2 cf = c.coef[k0];
-      const int64_t q = c.qs[int64_t(ip.it) * c.trials_cap + trial];
-      const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
-      int32_t bin;
-      const double2 base = FreqBase(c, ip, q, k0, k1, &bin);
-      c.term_bin[int64_t(trial) * c.cap + mi] = bin;
-      double2 *val = c.term_val + int64_t(trial) * c.rows_cap * c.cap + mi;
-      val[0] = Cmul(cf, base);
-      for (int32_t bit = 0; bit < bits; ++bit) {
-        const double2 g = BitTwiddle(c, B, bit, k1);
-        val[int64_t(2 * bit + 1) * c.cap] = Cmul(cf, Cmul(base, g));
-        val[int64_t(2 * bit + 2) * c.cap] = Cmul(cf, Cmul(base, Conj(g)));
-      }
-    }
+    FreqTerms(c, ip, trials, blk, kPrepBlocks);
     return;
   }
 
@@ -881,6 +892,96 @@ __device__ __forceinline__ void ComputeStoreR(double2 *a, double2 (&v)[16], int 
 }
 
 // ---------------------------------------------------------------------------
+// BinInFreq's subtraction (src/binner.cc:565-591) in an order-independent
+// form.  The reference subtracts a bin's terms one by one in ModeMap order;
+// here every term v of a workgroup's bins is added, as two signed integer
+// limbs per component on the grid 2^(E-94) (E: the smallest power of two
+// above every term component of the workgroup), into per-bin accumulators:
+// x = v 2^(94-E) = h 2^47 + l with h = trunc(x / 2^47) exact and l = the
+// integer nearest x - h 2^47 (the only rounding: below 2^-94 of the largest
+// term).  Integer addition is associative, so each bin's sum, the one
+// rounding that turns it into a double and the bin value A = DFT - sum are
+// the same whatever order the terms (and the found-mode keys behind them)
+// arrive in, and the same in every form of an iteration that sees the same
+// terms.  Exact for up to 2^15 terms in one bin.
+struct TermGrid {
+  double scale, inv;  // 2^(94-E), 2^(E-94)
+  bool any;           // some term is nonzero
+};
+constexpr int kLimbs = 4;  // per bin: re h, re l, im h, im l
+
+// The workgroup's largest term component (v) -> the grid.  Every thread
+// calls it.
+__device__ __forceinline__ TermGrid MakeTermGrid(double v) {
+  __shared__ double wmax[kFftThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = 0.0;
+  for (int w = 0; w < int(blockDim.x >> 6); ++w) m = fmax(m, wmax[w]);
+  TermGrid g;
+  g.any = m > 0.0;
+  const int e = g.any ? ilogb(m) + 1 : 0;
+  g.scale = ldexp(1.0, 94 - e);
+  g.inv = ldexp(1.0, e - 94);
+  return g;
+}
+
+__device__ __forceinline__ double MaxComp(double2 v) { return fmax(fabs(v.x), fabs(v.y)); }
+
+__device__ __forceinline__ void LimbAdd(unsigned long long *acc, double2 v, const TermGrid &g) {
+  const double comp[2] = {v.x, v.y};
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const double x = comp[j] * g.scale;
+    const double h = trunc(x * 0x1p-47);
+    const double l = rint(x - h * 0x1p47);
+    atomicAdd(&acc[2 * j], static_cast<unsigned long long>(static_cast<long long>(h)));
+    atomicAdd(&acc[2 * j + 1], static_cast<unsigned long long>(static_cast<long long>(l)));
+  }
+}
+
+// The bin's accumulated terms as a double2 (one rounding per component).
+__device__ __forceinline__ double2 LimbValue(const unsigned long long *acc, const TermGrid &g) {
+  double out[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    long long h = static_cast<long long>(acc[2 * j]);
+    long long l = static_cast<long long>(acc[2 * j + 1]);
+    const long long carry = l >> 47;  // floor(l / 2^47): 0 <= l < 2^47 after it
+    l -= carry * (1ll << 47);
+    h += carry;
+    out[j] = (double(h) * 0x1p47 + double(l)) * g.inv;
+  }
+  return make_double2(out[0], out[1]);
+}
+
+// A bin FFT's outputs X_k, k = tid + r kFftThreads < B (at k in LDS, or at
+// Pad(k) on the large path), moved into registers so that the BinInFreq
+// limbs can take the FFT array's place; the caller has synchronised after
+// the FFT.
+constexpr int kOutPer = 2048 / kFftThreads;  // B <= 2048 on the LDS paths
+__device__ __forceinline__ void TakeOutputs(const double2 *a, bool small, int32_t B,
+                                            double2 (&xo)[kOutPer]) {
+#pragma unroll
+  for (int r = 0; r < kOutPer; ++r) {
+    const int k = threadIdx.x + r * kFftThreads;
+    xo[r] = k < B ? a[small ? k : Pad(k)] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+// A[k] = X_k - (the bin's term sum), for the outputs TakeOutputs took.
+__device__ __forceinline__ void StoreBins(double2 *A, int32_t B, const double2 (&xo)[kOutPer],
+                                          const unsigned long long *limbs, const TermGrid &g) {
+#pragma unroll
+  for (int r = 0; r < kOutPer; ++r) {
+    const int k = threadIdx.x + r * kFftThreads;
+    if (k < B) A[k] = g.any ? Sub(xo[r], LimbValue(limbs + kLimbs * k, g)) : xo[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_bin_fft: per (trial, row) the forward B-point DFT of the fold by
 // Bluestein (X_k = c_k sum_j (v_j c_j) conj(c_{k-j})) over two M-point
 // FFTs (M = 2^logM >= 2B-1), then the row phase e^{2 pi i b tau/n} and the
@@ -987,8 +1088,8 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
 
   // Prefetch, in the first pass's layout (element b R0 + r is index
   // j + r nb0, j = tid + b T): fold x chirp and the filter spectrum.  The
-  // chirp itself waits in LDS (after the padded array) for the last step.
-  double2 *lchirp = a + Pad(M);
+  // chirp of the last step is loaded again (into the filter's registers)
+  // once the filter product is formed.
   double2 v[16], hv[16], wv[2];
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
@@ -997,11 +1098,7 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
     const int m = j + r * nb0;
     v[e] = hv[e] = make_double2(0.0, 0.0);
     if (b < NB0 && j < nb0) {
-      if (m < B) {
-        const double2 cm = chirp[m];
-        lchirp[m] = cm;
-        v[e] = Cmul(CONJ_IN ? Conj(fold[m]) : fold[m], cm);
-      }
+      if (m < B) v[e] = Cmul(CONJ_IN ? Conj(fold[m]) : fold[m], chirp[m]);
       hv[e] = filt[m];
     }
   }
@@ -1031,7 +1128,13 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
   // Inverse transform of (spectrum x filter); the filter is pre-scaled by 1/M.
   LoadR(a, v, logM, R0);
 #pragma unroll
-  for (int e = 0; e < 16; ++e) v[e] = Cmul(v[e], hv[e]);
+  for (int e = 0; e < 16; ++e) {
+    v[e] = Cmul(v[e], hv[e]);
+    const int b = e >> lr0, r = e & (R0 - 1);
+    const int j = tid + b * T;
+    const int m = j + r * nb0;
+    hv[e] = (b < NB0 && j < nb0 && m < B) ? chirp[m] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
   ComputeStoreR<true>(a, v, logM, R0, 1, make_double2(1.0, 0.0));
   __syncthreads();
@@ -1050,7 +1153,7 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
     const int b = e >> lr0, r = e & (R0 - 1);
     const int j = tid + b * T;
     const int m = j + r * nb0;
-    if (b < NB0 && j < nb0 && m < B) a[Pad(m)] = Cmul(Cmul(a[Pad(m)], lchirp[m]), rp);
+    if (b < NB0 && j < nb0 && m < B) a[Pad(m)] = Cmul(Cmul(a[Pad(m)], hv[e]), rp);
   }
 }
 
@@ -1092,23 +1195,27 @@ __device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2
   else
     BinFftLarge<false>(c, ip, fold, rp, a);
   __syncthreads();
+  // BinInFreq: every bin's terms summed exactly (TermGrid), then A = X - sum.
+  // The limbs (2B double2 slots <= M) take the FFT array's place once the
+  // outputs are in registers.
+  double2 xo[kOutPer];
+  TakeOutputs(a, small, B, xo);
+  unsigned long long *limbs = reinterpret_cast<unsigned long long *>(a);
+  double mx = 0.0;
 #pragma unroll
-  for (int r = 0; r < kPerTerm; ++r) {
-    if (tid + r * T < cnt) {
-      const int i = small ? tbin[r] : Pad(tbin[r]);
-      atomicAdd(&a[i].x, -tval[r].x);
-      atomicAdd(&a[i].y, -tval[r].y);
-    }
-  }
-  for (int i = tid + kPerTerm * T; i < cnt; i += T) {
-    const double2 tvi = tv[i];
-    const int bin = small ? tb[i] : Pad(tb[i]);
-    atomicAdd(&a[bin].x, -tvi.x);
-    atomicAdd(&a[bin].y, -tvi.y);
+  for (int r = 0; r < kPerTerm; ++r)
+    if (tid + r * T < cnt) mx = fmax(mx, MaxComp(tval[r]));
+  for (int i = tid + kPerTerm * T; i < cnt; i += T) mx = fmax(mx, MaxComp(tv[i]));
+  for (int k = tid; k < kLimbs * B; k += T) limbs[k] = 0ull;
+  const TermGrid g = MakeTermGrid(mx);
+  if (g.any) {
+#pragma unroll
+    for (int r = 0; r < kPerTerm; ++r)
+      if (tid + r * T < cnt) LimbAdd(limbs + kLimbs * tbin[r], tval[r], g);
+    for (int i = tid + kPerTerm * T; i < cnt; i += T) LimbAdd(limbs + kLimbs * tb[i], tv[i], g);
   }
   __syncthreads();
-  double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  for (int k = tid; k < B; k += T) A[k] = a[small ? k : Pad(k)];
+  StoreBins(c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap, B, xo, limbs, g);
 }
 
 __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
@@ -1269,28 +1376,34 @@ __global__ void __launch_bounds__(kThreads) k_big_out(Ctx c, int p) {
       a[i] = Cmul(Cmul(a[i], Chirp(m, B)), rp);
     }
   }
-  __syncthreads();
   // BinInFreqV1 (src/binner.cc:565-591): the found modes whose bin lies in
-  // this workgroup's columns.
+  // this workgroup's columns, summed exactly per bin (TermGrid) and
+  // subtracted once.
   const int32_t cnt = min(st->count, c.cap);
   const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
   const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
+  __shared__ unsigned long long limbs[kLimbs * kBigTile];
+  double mx = 0.0;
   for (int i = threadIdx.x; i < cnt; i += kThreads) {
-    const int bin = tb[i];
-    const int col = (bin & (G.N2 - 1)) - n2_0;
-    if (col >= 0 && col < C) {
-      const double2 v = tv[i];
-      double *dst = &a[(col << G.logN1) + (bin >> G.logN2)].x;
-      atomicAdd(dst, -v.x);
-      atomicAdd(dst + 1, -v.y);
-    }
+    const int col = (tb[i] & (G.N2 - 1)) - n2_0;
+    if (col >= 0 && col < C) mx = fmax(mx, MaxComp(tv[i]));
   }
+  for (int k = threadIdx.x; k < kLimbs * kBigTile; k += kThreads) limbs[k] = 0ull;
+  const TermGrid tg = MakeTermGrid(mx);
+  if (tg.any)
+    for (int i = threadIdx.x; i < cnt; i += kThreads) {
+      const int bin = tb[i];
+      const int col = (bin & (G.N2 - 1)) - n2_0;
+      if (col >= 0 && col < C)
+        LimbAdd(limbs + kLimbs * ((col << G.logN1) + (bin >> G.logN2)), tv[i], tg);
+    }
   __syncthreads();
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
   for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
     const int col = e & (C - 1), n1 = e >> logC;
     const int m = (n1 << G.logN2) + n2_0 + col;
-    if (m < B) A[m] = a[(col << G.logN1) + n1];
+    const int i = (col << G.logN1) + n1;
+    if (m < B) A[m] = tg.any ? Sub(a[i], LimbValue(limbs + kLimbs * i, tg)) : a[i];
   }
 }
 
@@ -1316,9 +1429,9 @@ __device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v, 
   const uint32_t old = atomicOr(&c.present[k0 >> 5], bit);
   if (!(old & bit)) {
     const int32_t li = atomicAdd(ncnt, 1);  // LDS
-    if (li < kIdentWaves) {
+    if (li < kIdentKeys) {
       nkeys[li] = k0;
-    } else {  // more new keys than waves (a wave that handled several bins)
+    } else {  // more new keys than the list holds
       const int32_t idx = atomicAdd(&c.st->count, 1);
       if (idx < c.cap)
         c.keys[idx] = k0;
@@ -1334,7 +1447,7 @@ __device__ __forceinline__ void InsertMode(const Ctx &c, int64_t k0, double2 v, 
 __device__ __forceinline__ void AppendKeys(const Ctx &c, int32_t *ncnt, int32_t *nbase,
                                            const int64_t *nkeys) {
   __syncthreads();
-  const int32_t m = min(*ncnt, kIdentWaves);
+  const int32_t m = min(*ncnt, kIdentKeys);
   if (threadIdx.x == 0 && m > 0) *nbase = atomicAdd(&c.st->count, m);
   __syncthreads();
   if (int(threadIdx.x) < m) {
@@ -1387,7 +1500,7 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
   __shared__ int32_t ncnt, nbase, nany;
-  __shared__ int64_t nkeys[kIdentWaves];
+  __shared__ int64_t nkeys[kIdentKeys];
   if (threadIdx.x == 0) {
     ncnt = 0;
     nany = 0;
@@ -1722,71 +1835,133 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   else
     BinFftLarge<false>(c, ip, fold, rp, a);
   __syncthreads();
+  // BinInFreq, as k_bin_fft's epilogue (exact per-bin sums of the terms):
+  // the terms of the first modes are formed once, in registers, and the
+  // rest (more than kPerTerm x T modes) are formed again for the second pass.
+  double2 xo[kOutPer];
+  TakeOutputs(a, small, B, xo);
+  unsigned long long *limbs = reinterpret_cast<unsigned long long *>(a);
+  auto term = [&](int i, int64_t k0, const double2 &cf, int32_t *bin) {
+    if (i < cnt_pre) {
+      *bin = tb[i];
+      return Cmul(cf, tv[i]);
+    }
+    const int64_t k1 = ModN(ip.tf.a * k0 + ip.tf.b, n, mask);
+    return Cmul(cf, RowFactor(c, ip, q, row, k0, k1, bin));
+  };
+  double2 vr[kPerTerm];
+  double mx = 0.0;
 #pragma unroll
   for (int r = 0; r < kPerTerm; ++r) {
     const int i = tid + r * T;
+    vr[r] = make_double2(0.0, 0.0);
     if (i < cnt) {
-      double2 v;
-      int32_t bin = tbin[r];
       if (i < cnt_pre) {
-        v = Cmul(cfv[r], fac[r]);
+        vr[r] = Cmul(cfv[r], fac[r]);
       } else {
         const int64_t k1 = ModN(ip.tf.a * kk[r] + ip.tf.b, n, mask);
-        v = Cmul(cfv[r], RowFactor(c, ip, q, row, kk[r], k1, &bin));
+        vr[r] = Cmul(cfv[r], RowFactor(c, ip, q, row, kk[r], k1, &tbin[r]));
       }
-      const int j = small ? bin : Pad(bin);
-      atomicAdd(&a[j].x, -v.x);
-      atomicAdd(&a[j].y, -v.y);
+      mx = fmax(mx, MaxComp(vr[r]));
     }
   }
   for (int i = tid + kPerTerm * T; i < cnt; i += T) {
-    const int64_t k0 = c.keys[i];
-    double2 v;
     int32_t bin;
-    if (i < cnt_pre) {
-      v = Cmul(c.coef[k0], tv[i]);
-      bin = tb[i];
-    } else {
-      const int64_t k1 = ModN(ip.tf.a * k0 + ip.tf.b, n, mask);
-      v = Cmul(c.coef[k0], RowFactor(c, ip, q, row, k0, k1, &bin));
+    const int64_t k0 = c.keys[i];
+    mx = fmax(mx, MaxComp(term(i, k0, c.coef[k0], &bin)));
+  }
+  for (int k = tid; k < kLimbs * B; k += T) limbs[k] = 0ull;
+  const TermGrid g = MakeTermGrid(mx);
+  if (g.any) {
+#pragma unroll
+    for (int r = 0; r < kPerTerm; ++r)
+      if (tid + r * T < cnt) LimbAdd(limbs + kLimbs * tbin[r], vr[r], g);
+    for (int i = tid + kPerTerm * T; i < cnt; i += T) {
+      int32_t bin;
+      const int64_t k0 = c.keys[i];
+      const double2 v = term(i, k0, c.coef[k0], &bin);
+      LimbAdd(limbs + kLimbs * bin, v, g);
     }
-    const int j = small ? bin : Pad(bin);
-    atomicAdd(&a[j].x, -v.x);
-    atomicAdd(&a[j].y, -v.y);
   }
   __syncthreads();
-  double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  for (int k = tid; k < B; k += T) A[k] = a[small ? k : Pad(k)];
+  StoreBins(c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap, B, xo, limbs, g);
 }
 
-// BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix: one
-// thread per mode, subtracting from every row with device atomics.
-__global__ void __launch_bounds__(kThreads)
-    k_bin_in_freq_direct(int64_t n, int64_t mask, BParams bp, DevTf tf, int64_t q,
-                         const int64_t *k0s, const double2 *coefs, int64_t m,
-                         double2 *out) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
+// BinInFreqV1 (src/binner.cc:565-591) on a caller-owned bin matrix (rows x
+// B, row stride B), one thread per mode, with each bin's terms summed
+// exactly as in the solve (TermGrid) so the result does not depend on the
+// order of the modes or of the threads: a pass for the grid exponent (the
+// largest term component over all modes, by atomicMax), a pass adding the
+// limbs into limbs[(row B + bin) kLimbs ...], and one subtracting each
+// bin's sum.
+template <typename Fn>
+__device__ __forceinline__ void DirectTerms(int64_t n, int64_t mask, const BParams &bp,
+                                            const DevTf &tf, int64_t q, int64_t k0s,
+                                            double2 coef, Fn fn) {
   const int32_t B = bp.B;
-  const int64_t k0 = ModN(k0s[i], n, mask);
+  const int64_t k0 = ModN(k0s, n, mask);
   const int64_t k1 = ModN(tf.a * k0 + tf.b, n, mask);
   const int32_t bin = int32_t(k1 * B / n);
   const double xi = (double(bin) + 0.5) / double(B) - double(k1) / double(n);
   const double wf = SampleInFreq(xi, bp.width, bp.dfreq);
-  const double2 base = Cmul(Scale(coefs[i], wf), CisFrac(ModN(q * k1 + tf.c * k0, n, mask), n));
-  atomicAdd(&out[bin].x, -base.x);
-  atomicAdd(&out[bin].y, -base.y);
+  const double2 base = Cmul(Scale(coef, wf), CisFrac(ModN(q * k1 + tf.c * k0, n, mask), n));
+  fn(0, bin, base);
   for (int32_t bit = 0; bit < bp.bits; ++bit) {
     const double2 g = CisFrac(ModN((int64_t(B) << bit) * k1, n, mask), n);
-    const double2 c1 = Cmul(base, g);
-    const double2 c2 = Cmul(base, Conj(g));
-    double2 *r1 = out + int64_t(2 * bit + 1) * B + bin;
-    double2 *r2 = out + int64_t(2 * bit + 2) * B + bin;
-    atomicAdd(&r1->x, -c1.x);
-    atomicAdd(&r1->y, -c1.y);
-    atomicAdd(&r2->x, -c2.x);
-    atomicAdd(&r2->y, -c2.y);
+    fn(2 * bit + 1, bin, Cmul(base, g));
+    fn(2 * bit + 2, bin, Cmul(base, Conj(g)));
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_freq_direct_max(int64_t n, int64_t mask, BParams bp, DevTf tf, int64_t q,
+                      const int64_t *k0s, const double2 *coefs, int64_t m, int32_t *emax) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double mx = 0.0;
+  if (i < m)
+    DirectTerms(n, mask, bp, tf, q, k0s[i], coefs[i],
+                [&](int, int32_t, double2 v) { mx = fmax(mx, MaxComp(v)); });
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0 && mx > 0.0) atomicMax(emax, ilogb(mx) + 1);
+}
+
+// The grid of exponent e (every term component below 2^e); e = kNoTerms
+// when there is no nonzero term.
+constexpr int32_t kNoTerms = -(1 << 30);
+__device__ __forceinline__ TermGrid TermGridOf(int32_t e) {
+  TermGrid g;
+  g.any = e != kNoTerms;
+  g.scale = g.any ? ldexp(1.0, 94 - e) : 1.0;
+  g.inv = g.any ? ldexp(1.0, e - 94) : 1.0;
+  return g;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_freq_direct_add(int64_t n, int64_t mask, BParams bp, DevTf tf, int64_t q,
+                      const int64_t *k0s, const double2 *coefs, int64_t m, const int32_t *emax,
+                      unsigned long long *limbs) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const TermGrid g = TermGridOf(*emax);
+  if (i >= m || !g.any) return;
+  DirectTerms(n, mask, bp, tf, q, k0s[i], coefs[i], [&](int row, int32_t bin, double2 v) {
+    LimbAdd(limbs + kLimbs * (int64_t(row) * bp.B + bin), v, g);
+  });
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_freq_direct_sub(const unsigned long long *limbs, const int32_t *emax, int64_t count,
+                      double2 *out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const TermGrid g = TermGridOf(*emax);
+  if (i < count && g.any) out[i] = Sub(out[i], LimbValue(limbs + kLimbs * i, g));
+}
+
+// limbs[0..count) = 0 and *emax = kNoTerms (a kernel rather than a memset).
+__global__ void __launch_bounds__(kThreads)
+    k_freq_direct_clear(unsigned long long *limbs, int64_t count, int32_t *emax) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t j = i; j < count; j += int64_t(gridDim.x) * blockDim.x) limbs[j] = 0ull;
+  if (i == 0) *emax = kNoTerms;
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -1824,35 +1999,151 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-template <typename Tx>
-__global__ void __launch_bounds__(kThreads)
-    k_bin_time_batch(const Ctx *ctxs, const int2 *map, int p, int32_t *counters,
-                     int32_t *host) {
-  HIP_DYNAMIC_SHARED(double2, lds)
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+// ---- the batched iteration: plan, then three launches over packed units
+//
+// k_batch_plan starts every signal's next iteration (NextIteration and
+// PublishIteration, as k_bin_time's first workgroup does for one signal)
+// and packs the work of the three launches that follow: the units of
+// signal i occupy [off[i], off[i] + cnt[i]) of each launch's map, so a
+// launch's workgroups visit only units that exist this iteration (signals
+// that stopped, and the tiles of a B smaller than the largest, cost
+// nothing).  Units: k_bin_time: PrepUnits term blocks then trials x
+// ceil(B/R) tiles; k_bin_fft: trials x rows (trial, row) DFTs; k_identify:
+// ceil(B / ident_bins) blocks of ident_bins bins (ident_bins / kIdentWaves
+// per wavefront).
+constexpr int kPlanThreads = 1024;
+
+// Workgroups computing BinInFreq terms for `terms` (modes x trials).
+__device__ __forceinline__ int32_t PrepUnits(int64_t terms) {
+  return int32_t(min<int64_t>(kPrepBlocks, (terms + kThreads - 1) / kThreads));
+}
+
+__global__ void __launch_bounds__(kPlanThreads)
+    k_batch_plan(const Ctx *ctxs, int32_t count, int p, BatchWork w, int32_t ident_bins,
+                 int32_t *counters, int32_t *host) {
+  __shared__ int32_t part[3][kPlanThreads];
+  const int tid = threadIdx.x;
+  // signals [i0, i1) belong to this thread
+  const int32_t per = (count + kPlanThreads - 1) / kPlanThreads;
+  const int32_t i0 = min(count, tid * per), i1 = min(count, i0 + per);
+  int32_t sum[3] = {0, 0, 0};
+  for (int32_t i = i0; i < i1; ++i) {
+    const Ctx &c = ctxs[i];
+    // the iteration's parameters are built in place, in slot p
+    IterParams &ip = c.st->P[p];
+    LoopCounters lc;
+    NextIteration(c, p, &ip, &lc);
+    PublishLoop(c, p, ip, lc);
+    int32_t u[3] = {0, 0, 0};
+    if (!ip.done) {
+      const int32_t trials = c.st->trials;
+      u[0] = PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
+             trials * ((ip.B + ip.R - 1) / ip.R);
+      u[1] = trials * ip.rows;
+      u[2] = (ip.B + ident_bins - 1) / ident_bins;
+    }
+    for (int k = 0; k < 3; ++k) {
+      w.cnt[k * count + i] = u[k];
+      sum[k] += u[k];
+    }
+    w.rows[i] = ip.rows;
+  }
+  for (int k = 0; k < 3; ++k) part[k][tid] = sum[k];
+  __syncthreads();
+  for (int d = 1; d < kPlanThreads; d <<= 1) {  // inclusive scan of the sums
+    int32_t v[3];
+    for (int k = 0; k < 3; ++k) v[k] = tid >= d ? part[k][tid - d] : 0;
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) part[k][tid] += v[k];
+    __syncthreads();
+  }
+  int32_t base[3];
+  for (int k = 0; k < 3; ++k) base[k] = part[k][tid] - sum[k];
+  for (int32_t i = i0; i < i1; ++i)
+    for (int k = 0; k < 3; ++k) {
+      w.off[k * count + i] = base[k];
+      base[k] += w.cnt[k * count + i];
+    }
+  if (tid == kPlanThreads - 1)
+    for (int k = 0; k < 3; ++k) w.total[k] = part[k][tid];
+  // the counts and offsets above, and the stop counts of PublishIteration,
+  // are read below by other threads of this workgroup (device-scope loads)
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
     // The single writer of the host's copy: batched iterations started and
-    // signals stopped by the earlier launches (counters[1] only grows, and
-    // each launch publishes after the previous one has finished).
+    // signals stopped so far (counters[1] only grows, and each launch
+    // publishes after the previous one has finished).
     const int32_t v = atomicAdd(&counters[0], 1) + 1;
     const int32_t d = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&host[1], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[0], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  const int2 m = map[blockIdx.x];
-  BinTimeBody<Tx>(ctxs[m.x], p, m.y, lds);
+  // the maps: one wavefront per signal
+  const int lane = tid & 63;
+  for (int32_t i = tid >> 6; i < count; i += kPlanThreads / 64) {
+    int32_t off[3], cnt[3];
+    for (int k = 0; k < 3; ++k) {
+      off[k] = __hip_atomic_load(&w.off[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      cnt[k] = __hip_atomic_load(&w.cnt[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int32_t rows = __hip_atomic_load(&w.rows[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t rows_cap = ctxs[i].rows_cap;
+    for (int32_t u = lane; u < cnt[0]; u += 64) w.map[0][off[0] + u] = make_int2(i, u);
+    for (int32_t u = lane; u < cnt[1]; u += 64) {
+      const int32_t trial = u / rows;
+      w.map[1][off[1] + u] = make_int2(i, trial * rows_cap + (u - trial * rows));
+    }
+    for (int32_t u = lane; u < cnt[2]; u += 64) w.map[2][off[2] + u] = make_int2(i, u);
+  }
 }
 
-__global__ void __launch_bounds__(kFftThreads) k_bin_fft_batch(const Ctx *ctxs, const int2 *map,
-                                                               int p) {
+// One k_bin_time unit of a published iteration: BinInFreq terms (the first
+// PrepUnits units) or a tile of the gather-fold.
+template <typename Tx>
+__device__ __forceinline__ void BinTimeUnit(const Ctx &c, int p, int32_t unit, double2 *lds) {
+  const DevState *st = c.st;
+  const IterParams &ip = st->P[p];
+  const int32_t trials = st->trials;
+  const int32_t nprep = PrepUnits(int64_t(min(st->count, c.cap)) * trials);
+  if (unit < nprep)
+    FreqTerms(c, ip, trials, unit, nprep);
+  else
+    TileFold<Tx>(c, ip, trials, unit - nprep, lds);
+}
+
+template <typename Tx>
+__global__ void __launch_bounds__(kThreads)
+    k_bin_time_batch(const Ctx *ctxs, const int2 *map, const int32_t *total, int p) {
   HIP_DYNAMIC_SHARED(double2, lds)
-  const int2 m = map[blockIdx.x];
-  BinFftBody(ctxs[m.x], p, m.y, lds);
+  const int32_t tot = total[0];
+  for (int32_t u = blockIdx.x; u < tot; u += gridDim.x) {
+    if (u != int32_t(blockIdx.x)) __syncthreads();  // the previous unit's LDS reads are done
+    const int2 m = map[u];
+    BinTimeUnit<Tx>(ctxs[m.x], p, m.y, lds);
+  }
+}
+
+__global__ void __launch_bounds__(kFftThreads)
+    k_bin_fft_batch(const Ctx *ctxs, const int2 *map, const int32_t *total, int p) {
+  HIP_DYNAMIC_SHARED(double2, lds)
+  const int32_t tot = total[1];
+  for (int32_t u = blockIdx.x; u < tot; u += gridDim.x) {
+    if (u != int32_t(blockIdx.x)) __syncthreads();
+    const int2 m = map[u];
+    BinFftBody(ctxs[m.x], p, m.y, lds);
+  }
 }
 
 __global__ void __launch_bounds__(kThreads)
-    k_identify_batch(const Ctx *ctxs, const int2 *map, const int32_t *nblk, int p) {
-  const int2 m = map[blockIdx.x];
-  IdentifyBody(ctxs[m.x], p, m.y, nblk[m.x]);
+    k_identify_batch(const Ctx *ctxs, const int2 *map, const int32_t *total, const int32_t *cnt,
+                     int p) {
+  const int32_t tot = total[2];
+  for (int32_t u = blockIdx.x; u < tot; u += gridDim.x) {
+    if (u != int32_t(blockIdx.x)) __syncthreads();
+    const int2 m = map[u];
+    IdentifyBody(ctxs[m.x], p, m.y, cnt[m.x]);
+  }
 }
 
 // Per signal: what the host needs to report the batched solve.
@@ -2419,9 +2710,17 @@ hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s) 
 hipError_t LaunchBinInFreqDirect(int64_t n, int64_t mask, const BParams &bp,
                                  const DevTf &tf, int64_t q, const int64_t *k0,
                                  const double2 *coef, int64_t m, double2 *out,
-                                 hipStream_t s) {
+                                 unsigned long long *limbs, int32_t *emax, hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_in_freq_direct, dim3(Blocks(m)), dim3(kThreads), 0, s, n, mask, bp, tf, q, k0, coef, m, out);
+  const int64_t bins = int64_t(1 + 2 * bp.bits) * bp.B;
+  hipLaunchKernelGGL(k_freq_direct_clear, dim3(Blocks(kLimbs * bins)), dim3(kThreads), 0, s, limbs,
+                     kLimbs * bins, emax);
+  hipLaunchKernelGGL(k_freq_direct_max, dim3(Blocks(m)), dim3(kThreads), 0, s, n, mask, bp, tf, q,
+                     k0, coef, m, emax);
+  hipLaunchKernelGGL(k_freq_direct_add, dim3(Blocks(m)), dim3(kThreads), 0, s, n, mask, bp, tf, q,
+                     k0, coef, m, emax, limbs);
+  hipLaunchKernelGGL(k_freq_direct_sub, dim3(Blocks(bins)), dim3(kThreads), 0, s, limbs, emax, bins,
+                     out);
   return hipGetLastError();
 }
 
@@ -2439,16 +2738,19 @@ hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count
 }
 
 hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s) {
+  const BatchWork &w = b.work;
+  hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, s, b.ctxs, b.count, parity, w,
+                     b.ident_bins, b.counters, b.host);
   if (b.mixed)
-    hipLaunchKernelGGL(k_bin_time_batch<float2>, dim3(b.nb_time), dim3(kThreads), b.lds_time, s,
-                       b.ctxs, b.map_time, parity, b.counters, b.host);
+    hipLaunchKernelGGL(k_bin_time_batch<float2>, dim3(b.grid_time), dim3(kThreads), b.lds_time, s,
+                       b.ctxs, w.map[0], w.total, parity);
   else
-    hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.nb_time), dim3(kThreads), b.lds_time, s,
-                       b.ctxs, b.map_time, parity, b.counters, b.host);
-  hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
-                     b.map_fft, parity);
-  hipLaunchKernelGGL(k_identify_batch, dim3(b.nb_ident), dim3(kThreads), 0, s, b.ctxs,
-                     b.map_ident, b.nblk_ident, parity);
+    hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.grid_time), dim3(kThreads), b.lds_time,
+                       s, b.ctxs, w.map[0], w.total, parity);
+  hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.grid_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
+                     w.map[1], w.total, parity);
+  hipLaunchKernelGGL(k_identify_batch, dim3(b.grid_ident), dim3(kThreads), 0, s, b.ctxs, w.map[2],
+                     w.total, w.cnt + 2 * b.count, parity);
   return hipGetLastError();
 }
 

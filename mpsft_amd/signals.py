@@ -105,6 +105,18 @@ def quantization_sigma(x128, x64):
     return float(np.sqrt(np.mean(np.abs(d) ** 2)))
 
 
+def quantization_sigma_c64(x64):
+    """quantization_sigma from the complex64 samples alone, for a caller
+    that never held the complex128 signal: rounding to nearest leaves each
+    component an error uniform over one float32 spacing of the stored value
+    (variance spacing^2 / 12), so sigma_q^2 = mean(spacing(re)^2 +
+    spacing(im)^2) / 12."""
+    x64 = np.asarray(x64, np.complex64)
+    sr = np.spacing(np.abs(x64.real)).astype(np.float64)
+    si = np.spacing(np.abs(x64.imag)).astype(np.float64)
+    return float(np.sqrt(np.mean(sr * sr + si * si) / 12.0))
+
+
 def loss_targets(truth, seed, rel=0.01):
     """Targets of the differentiable-recovery loss: each true coefficient
     times (1 + rel * z), z complex Gaussian with E|z|^2 = 1."""

@@ -517,11 +517,15 @@ std::pair<double, bool> IdentifyFreq(const IterOpts &opt, int32_t b,
   return std::make_pair(double(2 * xi_sum + 1) / double(1 << (bits + 1)), true);
 }
 
+int IdentifyBins(int32_t n, const IterOpts &opt, const Window &win, int32_t bits,
+                 const Transform &tf, const std::vector<int64_t> &list_q,
+                 const std::vector<std::vector<Cplex>> &A, ModeMap *mm,
+                 orc_iterate_trace *trace, bool *found_out);
+
 // Iterate, src/iterate.cc:80-177.
 int IterateImpl(const Cplex *x, int32_t n, const IterOpts &opt,
                 const Transform *tf_in, const int32_t *qs_in, ModeMap *mm,
                 orc_iterate_trace *trace, bool *found_out) {
-  bool found = false;
   const int32_t trials = opt.trials;
   const int32_t bins = opt.bins;
   if (!(opt.window_threshold > 0)) return ORC_ERR_INVALID_ARG;
@@ -566,6 +570,20 @@ int IterateImpl(const Cplex *x, int32_t n, const IterOpts &opt,
                     sizeof(Cplex) * size_t(rows) * bins);
   }
 
+  const int rc = IdentifyBins(n, opt, win, bits, tf, list_q, A, mm, trace, found_out);
+  return rc;
+}
+
+// Iterate's per-bin body (src/iterate.cc:113-175) on bin matrices A[trial]
+// (rows x bins each): energy test, IdentifyFreq, k1 rounding, window test,
+// coefficient estimate, un-permutation, mm[k0] += coef.
+int IdentifyBins(int32_t n, const IterOpts &opt, const Window &win, int32_t bits,
+                 const Transform &tf, const std::vector<int64_t> &list_q,
+                 const std::vector<std::vector<Cplex>> &A, ModeMap *mm,
+                 orc_iterate_trace *trace, bool *found_out) {
+  bool found = false;
+  const int32_t trials = opt.trials;
+  const int32_t bins = opt.bins;
   int32_t nfound = 0;
   for (int32_t b = 0; b < bins; ++b) {
     double margin = 1e300;
@@ -636,6 +654,7 @@ int IterateImpl(const Cplex *x, int32_t n, const IterOpts &opt,
   *found_out = found;
   return ORC_OK;
 }
+
 
 IterOpts FromC(const orc_iterate_options *o) {
   IterOpts r;
@@ -891,6 +910,34 @@ int32_t Demo1Loop(int32_t k, double sigma, const orc_demo1_options *opt, int32_t
 }  // namespace
 
 extern "C" {
+
+int orc_identify(const double *a, int32_t trials, int32_t n, const orc_iterate_options *opt,
+                 const int32_t tf[4], const int32_t *qs, orc_modemap *mm,
+                 orc_iterate_trace *trace) {
+  IterOpts io = FromC(opt);
+  if (trials != io.trials || trials <= 0 || trials % 2 != 1 || !(io.window_threshold > 0))
+    return -ORC_ERR_INVALID_ARG;
+  Window win;
+  if (!win.Init(n, io.bins, io.window_delta)) return -ORC_ERR_INVALID_ARG;
+  const int32_t bits = NumBits(n, io.bins);
+  const int64_t per = int64_t(1 + 2 * bits) * io.bins;
+  std::vector<std::vector<Cplex>> A(trials);
+  std::vector<int64_t> list_q(trials);
+  for (int32_t t = 0; t < trials; ++t) {
+    const Cplex *src = reinterpret_cast<const Cplex *>(a) + t * per;
+    A[t].assign(src, src + per);
+    list_q[t] = qs[t];
+  }
+  Transform t4{tf[0], tf[1], tf[2], tf[3]};
+  if (trace) {
+    for (int i = 0; i < 4; ++i) trace->tf[i] = tf[i];
+    trace->bits = bits;
+  }
+  bool found = false;
+  const int rc = IdentifyBins(n, io, win, bits, t4, list_q, A, &mm->m, trace, &found);
+  if (rc != ORC_OK) return -rc;
+  return found ? 1 : 0;
+}
 
 int32_t orc_demo1_run_log(const double *x, int32_t n, int32_t k, double sigma,
                           const orc_demo1_options *opt, orc_modemap *found,

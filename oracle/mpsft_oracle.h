@@ -152,6 +152,15 @@ int orc_iterate_fixed(const double *x, int32_t n,
                       const int32_t *qs, orc_modemap *mm,
                       orc_iterate_trace *trace);
 
+/* Iterate's per-bin body alone (src/iterate.cc:113-175) on given bin
+ * matrices a (trials x rows x bins complex, after BinInFreq) with the
+ * iteration's transform and q's: the decisions, k1 and margins into trace
+ * (its a_out and q_out are not used) and the found modes added to mm.
+ * Returns 1 if a mode was found, 0 if not, or a negative status. */
+int orc_identify(const double *a, int32_t trials, int32_t n,
+                 const orc_iterate_options *opt, const int32_t tf[4],
+                 const int32_t *qs, orc_modemap *mm, orc_iterate_trace *trace);
+
 /* ---- Demo1::Run, src/demo1.h:25-33, src/demo1.cc:36-59 ---- */
 typedef struct orc_demo1_options {
   int32_t trials;

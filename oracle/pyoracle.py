@@ -163,6 +163,8 @@ def lib():
                               C.POINTER(IterateTrace)]
     L.orc_iterate_fixed.argtypes = [_f64p, C.c_int32, C.POINTER(IterateOptions), _i32p,
                                     _i32p, C.c_void_p, C.POINTER(IterateTrace)]
+    L.orc_identify.argtypes = [_f64p, C.c_int32, C.c_int32, C.POINTER(IterateOptions), _i32p,
+                               _i32p, C.c_void_p, C.POINTER(IterateTrace)]
     L.orc_demo1_run.argtypes = [_f64p, C.c_int32, C.c_int32, C.c_double,
                                 C.POINTER(Demo1Options), C.c_void_p, _i32p, _i32p, C.c_int32]
     L.orc_demo1_run.restype = C.c_int32
@@ -376,6 +378,25 @@ def iterate(x, opt, mm, tf=None, qs=None, want_a=False):
     if want_a:
         trace["a"] = a_out.view(np.complex128).reshape(opt.trials, rows, opt.bins)
     return bool(rc), trace
+
+
+def identify(a, n, opt, tf, qs, mm):
+    """Iterate's per-bin body on given bin matrices a (trials, rows, bins)
+    complex128 (src/iterate.cc:113-175).  Returns (found, trace dict)."""
+    a = np.ascontiguousarray(a, np.complex128)
+    decide = np.zeros(opt.bins, np.int32)
+    k1 = np.zeros(opt.bins, np.int32)
+    margin = np.zeros(opt.bins, np.float64)
+    tr = IterateTrace()
+    tr.decide_out = decide.ctypes.data
+    tr.k1_out = k1.ctypes.data
+    tr.margin_out = margin.ctypes.data
+    rc = lib().orc_identify(a.view(np.float64).reshape(-1), a.shape[0], n, C.byref(opt),
+                            np.ascontiguousarray(tf, np.int32), np.ascontiguousarray(qs, np.int32),
+                            mm.h, C.byref(tr))
+    if rc < 0:
+        raise RuntimeError(f"identify failed with status {-rc}")
+    return bool(rc), dict(decide=decide, k1=k1, margin=margin, found=tr.found, bits=tr.bits)
 
 
 def demo1_run(x, k, sigma, opt=None, log_cap=4096):

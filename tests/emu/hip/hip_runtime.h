@@ -70,6 +70,7 @@ void Launch(void *stream, dim3 grid, dim3 block, size_t lds, std::function<void(
 #define gridDim emu::gdim
 
 inline void __syncthreads() { emu::SyncThreads(); }
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 inline double __shfl(double v, int src, int = 64) { return emu::ShflD(v, src & 63); }
 inline int __shfl(int v, int src, int = 64) { return emu::ShflI(v, src & 63); }
 inline double __shfl_xor(double v, int mask, int = 64) {
@@ -107,8 +108,18 @@ inline double atomicAdd(double *p, double v) {
   }
 }
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v) {
+  return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
 inline unsigned atomicAdd(unsigned *p, unsigned v) {
   return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+inline int atomicMax(int *p, int v) {
+  int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (old < v &&
+         !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return old;
 }
 inline int atomicOr(int *p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomicOr(unsigned *p, unsigned v) {

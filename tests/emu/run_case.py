@@ -90,6 +90,27 @@ def plan(n, bins, bits, seed):
     return dict(rel_err=float(np.abs(got - ref).max() / np.abs(ref).max()))
 
 
+def plan_freq(n, bins, bits, m, seed):
+    """BinInFreq on a plan: against the oracle, and bit for bit the same
+    after the modes are permuted (the per-bin sums are exact)."""
+    rng = np.random.default_rng(seed)
+    keys = rng.choice(n, m, replace=False).astype(np.int64)
+    coefs = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    coefs *= np.exp(rng.uniform(-8, 0, m))
+    base = rng.standard_normal((1 + 2 * bits, bins)) + 1j * rng.standard_normal((1 + 2 * bits, bins))
+    orc.seed(seed)
+    tf = orc.transform_draw(n)
+    q = int(rng.integers(0, n))
+    ref = orc.bin_in_freq(keys, coefs, n, bins, 1e-6, bits, tf, q, base.copy(), version=0)
+    p = gpu.Plan(n, bins, 1e-6, bits)
+    gtf = gpu.transform_make(n, *map(int, tf[:3]))
+    got = p.bin_in_freq(keys, coefs, gtf, q, base.copy())
+    perm = rng.permutation(m)
+    got2 = p.bin_in_freq(keys[perm], coefs[perm], gtf, q, base.copy())
+    return dict(abs_err=float(np.abs(got - ref).max()), scale=float(np.abs(ref).max()),
+                identical=bool(np.array_equal(got.view(np.float64), got2.view(np.float64))))
+
+
 def batch(ns, ks, sigma, seeds, precision=0, autotune=False):
     xs = [signal(n, k, sigma, sd) for n, k, sd in zip(ns, ks, seeds)]
     xg = [x.astype(np.complex64) if precision else x for x in xs]
@@ -254,5 +275,5 @@ if __name__ == "__main__":
     case = json.loads(sys.argv[1])
     fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch,
           "adjoint": adjoint, "batch_adjoint": batch_adjoint, "gen": gen, "analyze": analyze,
-          "identify": identify}[case.pop("kind")]
+          "identify": identify, "plan_freq": plan_freq}[case.pop("kind")]
     print("RESULT " + json.dumps(fn(**case)), flush=True)

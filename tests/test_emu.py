@@ -49,6 +49,7 @@ def run_case(env, timeout=600, **case):
 CASES = {
     "plan": dict(kind="plan", n=4096, bins=101, bits=5, seed=1),
     "plan_four_step": dict(kind="plan", n=1 << 16, bins=4097, bits=4, seed=2),
+    "plan_freq": dict(kind="plan_freq", n=1 << 14, bins=301, bits=6, m=900, seed=4),
     "iterate_four_step": dict(kind="iterate", n=1 << 16, k=2100, sigma=1e-6, seed=3, bins=4201),
     "solve_0": dict(kind="solve", n=1 << 14, k=20, sigma=1e-6, seed=7, precision=0, repeat=2),
     "solve_1": dict(kind="solve", n=1 << 14, k=20, sigma=3e-6, seed=7, precision=1, repeat=2),
@@ -98,6 +99,14 @@ def test_emu_plan_bin_in_time(emu_runs):
 def test_emu_plan_four_step(emu_runs):
     """B = 4097: M = 16384, the four-step bin FFT (k_big_cols/rows/out)."""
     assert result(emu_runs, "plan_four_step")["rel_err"] < 1e-12
+
+
+def test_emu_plan_bin_in_freq_exact_sums(emu_runs):
+    """A plan's BinInFreq sums each bin's terms exactly: it agrees with the
+    oracle to rounding and is bit for bit the same when the modes come in
+    another order."""
+    r = result(emu_runs, "plan_freq")
+    assert r["abs_err"] < 1e-12 * r["scale"] and r["identical"], r
 
 
 def test_emu_iterate_four_step_with_found_modes(emu_runs):

@@ -1,0 +1,201 @@
+"""Which arithmetic the binning may use: FP32 window products, gathers and
+bin FFTs (with plain, Kahan-compensated or double-float accumulation) against
+the FP64 form, measured on one real first iteration (n = 2^20, S = 1000
+log-uniform modes, sigma = 1e-7, B = 2001, complex64-rounded samples) and
+judged by the oracle's own decisions (Iterate's per-bin body, orc_identify).
+
+The fold is restated here in numpy from the device kernel's definition
+(fold[row][m] = sum_k x[a u + c] tap(t), u = q - p2 + (off(row) + k) B + rho,
+A[row] = e^{2 pi i b tau/n} DFT_B(fold[row])) and pinned against the oracle's
+BinInTime (1e-13) before it is trusted.
+
+Findings (the numbers are printed): an FP32 bin FFT, or an FP32 fold of FP32
+taps (plain or Kahan-summed), leaves bin errors of the order of the noise per
+bin sigma/sqrt(2B), and the matrix-pencil's sigma_2 test (a threshold at the
+noise level) then flips on hundreds of bins, so the support differs from the
+FP64 form's.  Only a double-float fold (taps as hi + lo floats, error-free
+products, two-sum accumulation) matches FP64's decisions; it costs about 40
+FP32 operations per complex multiply-add against 4 FP64 FMAs, i.e. 5x the
+arithmetic time at MI355X's 2:1 FP32:FP64 vector rate.  The MIXED form
+therefore keeps FP32 only where it is exact and free: the complex64 storage of
+x (half the bytes; the widening to FP64 is exact), with FP64 arithmetic."""
+import numpy as np
+import pytest
+import scipy.fft as sfft
+
+from mpsft_amd.signals import make_signal, quantization_sigma
+
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def iteration(orc):
+    n, K, sigma, delta = 1 << 20, 1000, 1e-7, 1e-6
+    x, _ = make_signal(n, K, sigma, seed=2, magnitude="loguniform")
+    x64 = x.astype(np.complex64)
+    xq = x64.astype(np.complex128)
+    sig = float(np.hypot(sigma, quantization_sigma(x, x64)))
+    B = 2 * K + 1
+    bits = orc.num_bits(n, B)
+    opt = orc.IterateOptions(bins=B, window_delta=delta, trials=1,
+                             bin_threshold=3 * sig / np.sqrt(B), window_threshold=0.1,
+                             sv_threshold=sig / np.sqrt(B))
+    orc.seed(11)
+    tf = orc.transform_draw(n)
+    q = int(orc.rand_int32() % n)
+    mm = orc.ModeMap()
+    _, tr = orc.iterate(xq, opt, mm, tf=tf, qs=np.array([q], np.int32), want_a=True)
+    wt, _ = orc.window_taps(n, B, delta)
+    return dict(n=n, B=B, bits=bits, sig=sig, opt=opt, tf=tf, q=q, tr=tr, ref=mm.to_dict(),
+                x=x64, wt=wt)
+
+
+def _geometry(it):
+    n, B, bits, q = it["n"], it["B"], it["bits"], it["q"]
+    a, b, c = int(it["tf"][0]), int(it["tf"][1]), int(it["tf"][2])
+    wt = it["wt"]
+    P = wt.size
+    p2, f = (P - 1) // 2, P // B
+    offs = [0] + [s * (1 << k) for k in range(bits) for s in (1, -1)]
+    kk = np.arange(f)[:, None]
+    rho = np.arange(B)[None, :]
+    t = kk * B + rho - p2
+    w = wt[np.where(t >= 0, t, t + P)]
+    turns = ((b * t) % n) / n + ((-t / (2 * B)) - np.floor(-t / (2 * B)))
+    tap = w * np.exp(2j * np.pi * turns)                     # f x B, FP64
+    m_idx = (np.arange(B) - (B - 1) // 2) % B
+    xs = []
+    for off in offs:
+        u = q - p2 + (off + kk) * B + rho
+        xs.append(it["x"][(a * (u % n) + c) % n])            # f x B complex64 samples
+    return dict(offs=offs, tap=tap, m_idx=m_idx, xs=xs, b=b)
+
+
+def _two_sum(s, v):
+    """Knuth's error-free sum in float32: s + v = t + e exactly."""
+    t = (s + v).astype(F32)
+    z = (t - s).astype(F32)
+    e = ((s - (t - z).astype(F32)).astype(F32) + (v - z).astype(F32)).astype(F32)
+    return t, e
+
+
+def _fold(g, mode):
+    """rows x B fold in the given arithmetic: 'f64'; 'f32' (FP32 taps and
+    products, plain FP32 sum); 'kahan' (the same, Kahan-summed); 'dd' (taps
+    as hi + lo floats, error-free FP32 products, two-sum accumulation)."""
+    tap = g["tap"]
+    out = []
+    for xv in g["xs"]:
+        if mode == "f64":
+            s = (xv.astype(np.complex128) * tap).sum(axis=0)
+        elif mode in ("f32", "kahan"):
+            prod = (xv * tap.astype(np.complex64)).astype(np.complex64)
+            if mode == "f32":
+                s = np.zeros(prod.shape[1], np.complex64)
+                for p in prod:
+                    s = (s + p).astype(np.complex64)
+            else:
+                parts = []
+                for comp in (prod.real, prod.imag):
+                    acc = np.zeros(prod.shape[1], F32)
+                    cmp = np.zeros(prod.shape[1], F32)
+                    for v in comp.astype(F32):
+                        y = (v - cmp).astype(F32)
+                        t = (acc + y).astype(F32)
+                        cmp = ((t - acc).astype(F32) - y).astype(F32)
+                        acc = t
+                    parts.append(acc.astype(np.float64))
+                s = parts[0] + 1j * parts[1]
+        else:  # double-float
+            th = tap.astype(np.complex64)
+            tl = (tap - th.astype(np.complex128)).astype(np.complex64)
+            xr, xi = xv.real.astype(np.float64), xv.imag.astype(np.float64)
+            res = []
+            for terms in (((xr, th.real), (-xi, th.imag), (xr, tl.real), (-xi, tl.imag)),
+                          ((xr, th.imag), (xi, th.real), (xr, tl.imag), (xi, tl.real))):
+                acc = np.zeros(tap.shape[1], F32)
+                cmp = np.zeros(tap.shape[1], F32)
+                for k in range(tap.shape[0]):
+                    for u, v in terms:
+                        exact = u[k] * v[k].astype(np.float64)  # a float x float product is exact in FP64
+                        p = exact.astype(F32)
+                        e = (exact - p.astype(np.float64)).astype(F32)
+                        acc, err = _two_sum(acc, p)
+                        cmp = (cmp + (err + e).astype(F32)).astype(F32)
+                res.append(acc.astype(np.float64) + cmp.astype(np.float64))
+            s = res[0] + 1j * res[1]
+        row = np.zeros(tap.shape[1], np.complex128)
+        row[g["m_idx"]] = s
+        out.append(row)
+    return np.array(out)
+
+
+def _bins(it, g, fold, fft32=False):
+    n, B, q = it["n"], it["B"], it["q"]
+    A = np.zeros_like(fold)
+    for r, off in enumerate(g["offs"]):
+        rp = np.exp(2j * np.pi * ((g["b"] * ((q + off * B) % n)) % n) / n)
+        F = sfft.fft(fold[r].astype(np.complex64)).astype(np.complex128) if fft32 else np.fft.fft(fold[r])
+        A[r] = rp * F
+    return A
+
+
+def _judge(orc, it, A):
+    """Decisions of Iterate's per-bin body on A against the FP64 form's."""
+    mm = orc.ModeMap()
+    _, tr = orc.identify(A[None], it["n"], it["opt"], it["tf"], np.array([it["q"]], np.int32), mm)
+    ref = it["tr"]
+    flips = int(np.sum((tr["decide"] != ref["decide"]) | (tr["k1"] != ref["k1"])))
+    got = mm.to_dict()
+    common = set(got) & set(it["ref"])
+    # relative to each coefficient, floored at 1e-2 of the largest (a tiny
+    # leakage mode's rounding is set by the large bins around it)
+    floor = 1e-2 * max(abs(v) for v in it["ref"].values())
+    coef = max((abs(got[k] - it["ref"][k]) / max(abs(it["ref"][k]), floor) for k in common),
+               default=0.0)
+    err = A - it["tr"]["a"][0]
+    return dict(flips=flips, same_support=set(got) == set(it["ref"]), coef=coef,
+                rms=float(np.sqrt(np.mean(np.abs(err) ** 2))))
+
+
+def test_numpy_fold_matches_oracle(iteration):
+    g = _geometry(iteration)
+    A = _bins(iteration, g, _fold(g, "f64"))
+    ref = iteration["tr"]["a"][0]
+    assert np.abs(A - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("form", ["fft32", "f32", "kahan", "dd"])
+def test_precision_form_decisions(orc, iteration, form):
+    g = _geometry(iteration)
+    if form == "fft32":
+        A = _bins(iteration, g, _fold(g, "f64"), fft32=True)
+    else:
+        A = _bins(iteration, g, _fold(g, form))
+    r = _judge(orc, iteration, A)
+    noise = iteration["sig"] / np.sqrt(2 * iteration["B"])
+    print(f"[precision] {form}: bin error rms {r['rms']:.2e} (noise per bin {noise:.2e}), "
+          f"decisions changed on {r['flips']} of {iteration['B']} bins, "
+          f"same support {r['same_support']}, max rel coef diff {r['coef']:.2e}", flush=True)
+    if form == "dd":
+        # double-float: FP64-grade bins, the same decisions and support
+        assert r["rms"] < 1e-4 * noise and r["flips"] == 0 and r["same_support"]
+        assert r["coef"] < 1e-12
+    else:
+        # FP32 products or FFT: errors at the noise level flip decisions
+        assert r["rms"] > 0.5 * noise and r["flips"] > 100 and not r["same_support"]
+
+
+@pytest.mark.parametrize("n,k,mag,sigma", [(1 << 20, 1000, "loguniform", 1e-7),
+                                           (1 << 18, 100, "unit", 1e-6),
+                                           (1 << 16, 20, "unit", 1e-3)])
+def test_quantization_sigma_from_complex64_alone(n, k, mag, sigma):
+    """sigma_q estimated from the complex64 samples (float32 spacing of each
+    stored component) agrees with the rms error against the complex128
+    signal to 3%."""
+    from mpsft_amd.signals import quantization_sigma_c64
+    x, _ = make_signal(n, k, sigma, seed=5, magnitude=mag)
+    x64 = x.astype(np.complex64)
+    exact = quantization_sigma(x, x64)
+    est = quantization_sigma_c64(x64)
+    assert abs(est / exact - 1) < 0.03, (est, exact)
