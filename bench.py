@@ -44,14 +44,21 @@ def time_solves(M, solver, xd, steps, warmup):
     return (time.perf_counter() - t0) / steps * 1e6, iters
 
 
+# Bytes an HBM read request of a random gather moves on MI355X: every 8-byte
+# gather is served as one 128-byte line (rocprofv3 TCC_EA0_RDREQ_128B, with
+# plain, non-temporal and sc0/sc1 loads alike).
+LINE = 128
+
+
 def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
-    """Algorithmic bytes of every k_bin_time launch of the last solve: 64 B
-    per distinct 64-B line of x its gathers touch, plus the fold it writes
-    (rows x B double2).  Also the reference's logical gather bytes."""
+    """Algorithmic bytes of every k_bin_time launch of the last solve: LINE
+    bytes per distinct LINE-byte line of x its gathers touch, plus the fold
+    it writes (rows x B double2).  Also the reference's logical gather
+    bytes."""
     bins_log, _ = solver.log()
     tfs, qs = solver.transforms()
     line_bytes = logical = 0
-    per_line = 64 // elem_bytes
+    per_line = LINE // elem_bytes
     seen = np.zeros(n // per_line, dtype=bool)
     for it, B in enumerate(bins_log):
         B = int(B)
@@ -66,7 +73,7 @@ def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
             for o in offs:
                 u = np.arange(o - p2, o + p2 + 1, dtype=np.int64) + int(q)
                 seen[((a * (u % n) + c) % n) // per_line] = True
-            line_bytes += 64 * int(seen.sum()) + rows * B * 16
+            line_bytes += LINE * int(seen.sum()) + rows * B * 16
             logical += rows * P * elem_bytes
     return line_bytes, logical, len(bins_log)
 
@@ -401,10 +408,11 @@ def copy_calibration():
 
 
 def traffic_from_profile():
-    """HBM bytes per k_bin_time<float2> launch measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), as committed
-    in profiles/; None when absent."""
-    path = os.path.join(ROOT, "profiles", "k_bin_time_traffic_mpsft_solve_n22_s1000_mixed.txt")
+    """HBM bytes per k_bin_time_batch<float2> launch measured by rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), as
+    committed in profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles",
+                        "k_bin_time_batch_traffic_mpsft_batched_64x_n22_s1000_mixed.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -432,7 +440,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch-count", type=int, default=16,
+    ap.add_argument("--batch-count", type=int, default=64,
                     help="signals per batched solve (the timed form)")
     ap.add_argument("--cpu-solves", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -525,6 +533,9 @@ def main():
     # complex64-rounded input, for every signal of the last step's seeds
     last = seeds_for(args.steps - 1)
     bm_modes = [bm.modes_dict(g) for g in range(G)]
+    if not args.no_roofline:
+        log("instrumented batched solve (event-timed stages, the last step's seeds)")
+        _, _, stage_ms, n_batched = bm.solve_profiled(xdb, last)
     del xdb
     bf = M.Batch([N] * G, opts, M.MPSFT_FP64)
     xdf = bf.upload(xs128q)
@@ -532,15 +543,24 @@ def main():
     t_bf = time_batched(M, bf, xdf, seeds_for, args.steps, args.warmup)
     del xdf, bf
     same_support, max_rel = 0, 0.0
-    s_chk = M.Solver(N, opts[0], M.MPSFT_FP64)
+    line_total = logical_total = 0
     for g in range(G):
         s_chk = M.Solver(N, opts[g], M.MPSFT_FP64)
         s_chk.solve(s_chk.upload(xs128q[g]), last[g])
         ref = s_chk.modes_dict()
+        del s_chk
         if set(ref) == set(bm_modes[g]):
             same_support += 1
             max_rel = max([max_rel] + [abs(bm_modes[g][q] - v) / abs(v) for q, v in ref.items()])
-    del s_chk
+        if not args.no_roofline:
+            # the signal's gathers in the batched solve are its single MIXED
+            # solve's (same seed, same iterations): their line bytes
+            s_mx = M.Solver(N, opts[g], M.MPSFT_MIXED)
+            s_mx.solve(s_mx.upload(xs64[g]), last[g])
+            lb, lg, _ = gather_bytes(M, s_mx, 8)
+            line_total += lb
+            logical_total += lg
+            del s_mx
 
     # ---- single-signal forms on the headline signal
     fp64q = M.Solver(N, M.solve_options(K, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
@@ -625,21 +645,29 @@ def main():
     }
 
     if not args.no_roofline:
-        log("profiled solve for the roofline")
-        it_p, ms = mixed.solve_profiled(xd64, SOLVE_SEED)
-        line_bytes, logical, n_launch = gather_bytes(M, mixed, 8)
-        t_bt = ms[0] / 1000.0
-        achieved = line_bytes / t_bt / 1e9
+        # the timed step's dominant kernel: the batched gather-fold
+        t_bt = stage_ms[1] / 1000.0
+        achieved = line_total / t_bt / 1e9
         line["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-            "kernel": "k_bin_time<float2>", "launches": n_launch,
-            "avg_launch_us": round(ms[0] * 1000.0 / n_launch, 2),
-            "algorithmic_bytes_per_launch": int(line_bytes / n_launch),
-            "logical_gather_GBs": round(logical / t_bt / 1e9, 1),
+            "kernel": "k_bin_time_batch<float2>", "launches": n_batched,
+            "avg_launch_us": round(stage_ms[1] * 1000.0 / n_batched, 2),
+            "algorithmic_bytes_per_launch": int(line_total / n_batched),
+            "line_bytes": LINE,
+            "logical_gather_GBs": round(logical_total / t_bt / 1e9, 1),
         }
-        line["kernel_ms_per_solve"] = {"k_bin_time": round(ms[0], 4), "k_bin_fft": round(ms[1], 4),
-                                       "k_identify": round(ms[2], 4)}
+        line["batched_stage_ms_per_step"] = {
+            "plan": round(stage_ms[0], 4), "k_bin_time_batch": round(stage_ms[1], 4),
+            "k_bin_fft_batch": round(stage_ms[2], 4), "k_identify_batch": round(stage_ms[3], 4),
+            "launches": n_batched}
+        log("profiled single solve")
+        it_p, ms = mixed.solve_profiled(xd64, SOLVE_SEED)
+        line_bytes, logical, n_launch = gather_bytes(M, mixed, 8)
+        line["single_solve"]["kernel_ms"] = {
+            "k_bin_time": round(ms[0], 4), "k_bin_fft": round(ms[1], 4),
+            "k_identify": round(ms[2], 4)}
+        line["single_solve"]["k_bin_time_GBs"] = round(line_bytes / (ms[0] / 1000.0) / 1e9, 1)
         line["bin_fft"] = fft_throughput(M, mixed, ms[1])
 
     if not args.no_cpu_baseline:

@@ -321,6 +321,15 @@ int mpsft_batch_destroy(mpsft_batch *batch);
  * status, else MPSFT_OK. */
 int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
                       int32_t *iters, int32_t *status, void *stream);
+/* Instrumentation, no reference counterpart: the same batched solve run
+ * one iteration at a time without a graph, each stage's kernel bracketed
+ * by HIP events on `stream`.  kernel_ms[0..3]: summed milliseconds of the
+ * plan, gather-fold, bin-FFT and identify kernels over the iterations;
+ * *launches: the batched iterations launched.  Results are identical to
+ * mpsft_batch_solve. */
+int mpsft_batch_solve_profiled(mpsft_batch *batch, const void *const *x_dev,
+                               const uint32_t *seeds, int32_t *iters, int32_t *status,
+                               double *kernel_ms, int32_t *launches, void *stream);
 /* Times short batched solves of these signals at tile widths 1, 2, 4, 8
  * and keeps the fastest for later solves; *residues gets it. */
 int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,

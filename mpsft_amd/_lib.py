@@ -35,6 +35,7 @@ EXPORTS = [
     "mpsft_solver_set_record", "mpsft_solver_info", "mpsft_solver_replay",
     "mpsft_solver_adjoint",
     "mpsft_batch_create", "mpsft_batch_destroy", "mpsft_batch_solve", "mpsft_batch_autotune",
+    "mpsft_batch_solve_profiled",
     "mpsft_batch_modes_count", "mpsft_batch_modes_get", "mpsft_batch_log_get",
     "mpsft_batch_set_record", "mpsft_batch_info", "mpsft_batch_replay", "mpsft_batch_adjoint",
     "mpsft_batch_modes_counts", "mpsft_batch_modes_device",
@@ -165,6 +166,8 @@ def declare(L):
         "mpsft_batch_solve": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_uint32), pi32, pi32,
                                         vp]),
         "mpsft_batch_autotune": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_uint32), pi32, vp]),
+        "mpsft_batch_solve_profiled": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_uint32), pi32,
+                                                 pi32, C.POINTER(C.c_double), pi32, vp]),
         "mpsft_batch_modes_count": (C.c_int, [vp, i32, pi64]),
         "mpsft_batch_modes_get": (C.c_int, [vp, i32, pi64, pf64, i64, pi64]),
         "mpsft_batch_log_get": (C.c_int, [vp, i32, pi32, pi32, i32, pi32]),

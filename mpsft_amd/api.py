@@ -488,6 +488,21 @@ class Batch:
             check(rc, "mpsft_batch_solve")
         return list(it), list(stv)
 
+    def solve_profiled(self, xds, seeds, stream=None):
+        """The batched solve one iteration at a time with event-timed
+        stages: returns (iterations, statuses, kernel_ms, launches) with
+        kernel_ms the summed ms of the plan, gather-fold, bin-FFT and
+        identify kernels."""
+        ptrs, sd = self._ptrs(xds, seeds)
+        it = (C.c_int32 * self.count)()
+        stv = (C.c_int32 * self.count)()
+        ms = (C.c_double * 4)()
+        nl = C.c_int32(0)
+        rc = lib().mpsft_batch_solve_profiled(self.h, ptrs, sd, it, stv, ms, C.byref(nl), stream)
+        if rc not in (0, 6, 7):
+            check(rc, "mpsft_batch_solve_profiled")
+        return list(it), list(stv), list(ms), nl.value
+
     def autotune(self, xds, seeds, stream=None):
         ptrs, sd = self._ptrs(xds, seeds)
         r = C.c_int32(0)

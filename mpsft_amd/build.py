@@ -116,7 +116,25 @@ def build_capi_client(verbose=False):
     return CLIENT
 
 
+def build_gather_probe(verbose=False):
+    """Compile tests/gather_probe.hip (random-gather load forms, a hardware
+    measurement behind the gather-fold's line-size figures) with hipcc for
+    gfx950.  Returns its path."""
+    tests = os.path.join(os.path.dirname(HERE), "tests")
+    src = os.path.join(tests, "gather_probe.hip")
+    out = os.path.join(tests, "gather_probe")
+    cmd = [hipcc()] + HIP_FLAGS + [src, "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on gather_probe.hip:\n{r.stderr[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_torch(verbose=True))
     print(build_capi_client(verbose=True))
+    print(build_gather_probe(verbose=True))

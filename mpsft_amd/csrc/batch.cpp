@@ -134,7 +134,8 @@ int Batch::Build(hipStream_t s) {
   for (size_t i = 0; i < eng_.size(); ++i) {
     Engine &e = *eng_[i];
     const int32_t si = int32_t(i);
-    const int32_t nt = kPrepBlocks + e.TilesCap();
+    // units of k_bin_time: rounded up to 8 per signal (XCD alignment)
+    const int32_t nt = (kPrepBlocks + e.TilesCap() + 7) & ~7;
     for (int32_t b = 0; b < nt; ++b) map_time.push_back(make_int2(si, b));
     const int32_t nf = e.trials_cap_ * e.rows_cap_;
     for (int32_t b = 0; b < nf; ++b) map_fft.push_back(make_int2(si, b));
@@ -179,7 +180,7 @@ int Batch::Build(hipStream_t s) {
   // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
   const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 2048, 64, 1 << 20);
   bl_.ident_bins = EnvInt("MPSFT_IDENT_BINS", 16, 4, kIdentKeys) / 4 * 4;
-  bl_.grid_time = std::max(1, std::min(bl_.nb_time, grid));
+  bl_.grid_time = std::max(8, std::min(bl_.nb_time, grid) & ~7);  // a multiple of 8
   bl_.grid_fft = std::max(1, std::min(bl_.nb_fft, grid));
   bl_.grid_ident = std::max(1, std::min(bl_.nb_ident, grid));
   bl_.count = int32_t(K);
@@ -222,6 +223,30 @@ int Batch::Build(hipStream_t s) {
   return MPSFT_OK;
 }
 
+// Every signal's transforms and q's drawn for iterations [0, need); when an
+// engine's arrays must grow, the stream is drained first and the Ctx array
+// re-uploaded (the kernels read it).
+int Batch::EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s) {
+  int32_t have = max_iter;
+  for (auto &e : eng_) have = std::min(have, e->tf_count_);
+  if (need <= have) return MPSFT_OK;
+  bool grow = false;
+  const int32_t want = std::min(max_iter, std::max(need, 2 * have));
+  for (auto &e : eng_) grow = grow || want > e->tf_cap_;
+  if (grow && hipStreamSynchronize(s) != hipSuccess) return MPSFT_ERR_HIP;
+  std::vector<Ctx> ctxs(eng_.size());
+  for (size_t i = 0; i < eng_.size(); ++i) {
+    const int rc = eng_[i]->EnsureTransforms(seeds_[i], want, opts_[i].trials, s);
+    if (rc != MPSFT_OK) return rc;
+    ctxs[i] = eng_[i]->MakeCtx();
+    ctxs[i].batch_done = d_counters_ + 1;
+  }
+  if (grow && hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
+                             hipMemcpyHostToDevice, s) != hipSuccess)
+    return MPSFT_ERR_HIP;
+  return MPSFT_OK;
+}
+
 // Launches graphs of two batched iterations, at most `lookahead` ahead of
 // the device, until every signal has stopped (or max_iter).  Transforms are
 // drawn ahead for all signals; when an engine's arrays must grow, the stream
@@ -241,26 +266,8 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
     const int32_t started = h_host_[0];
     if (h_host_[1] >= K || launched > max_iter) break;
     if (launched - started < lookahead) {
-      const int32_t need = std::min(max_iter, launched + 3);
-      int32_t have = max_iter;
-      for (auto &e : eng_) have = std::min(have, e->tf_count_);
-      if (need > have) {
-        bool grow = false;
-        const int32_t want = std::min(max_iter, std::max(need, 2 * have));
-        for (auto &e : eng_) grow = grow || want > e->tf_cap_;
-        if (grow && hipStreamSynchronize(s) != hipSuccess) return fail(MPSFT_ERR_HIP);
-        std::vector<Ctx> ctxs(eng_.size());
-        for (size_t i = 0; i < eng_.size(); ++i) {
-          const int rc = eng_[i]->EnsureTransforms(seeds_[i], want, opts_[i].trials, s);
-          if (rc != MPSFT_OK) return fail(rc);
-          ctxs[i] = eng_[i]->MakeCtx();
-          ctxs[i].batch_done = d_counters_ + 1;
-        }
-        if (grow &&
-            hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
-                           hipMemcpyHostToDevice, s) != hipSuccess)
-          return fail(MPSFT_ERR_HIP);
-      }
+      const int rc = EnsureAhead(std::min(max_iter, launched + 3), max_iter, s);
+      if (rc != MPSFT_OK) return fail(rc);
       if (hipGraphLaunch(graph_, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
       launched += 2;
       continue;
@@ -290,6 +297,58 @@ int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, in
   HIP_OK(LaunchBatchReset(d_ctxs_, d_args_, count(), d_counters_, s));
   rc = Run(s, max_iter);
   if (rc != MPSFT_OK) return rc;
+  return Finish(iters, status, s);
+}
+
+int Batch::SolveProfiled(const void *const *x, const uint32_t *seeds, int32_t *iters,
+                         int32_t *status, double *kernel_ms, int32_t *launches, hipStream_t s) {
+  if (!kernel_ms || !launches) return MPSFT_ERR_INVALID_ARG;
+  solved_ = false;
+  int rc = Prepare(x, seeds, s);
+  if (rc != MPSFT_OK) return rc;
+  int32_t max_iter = 0;
+  for (auto &a : args_) max_iter = std::max(max_iter, a.max_iter);
+  h_host_[0] = 0;
+  h_host_[1] = 0;
+  HIP_OK(LaunchBatchReset(d_ctxs_, d_args_, count(), d_counters_, s));
+  rc = RunProfiled(s, max_iter, kernel_ms, launches);
+  if (rc != MPSFT_OK) return rc;
+  return Finish(iters, status, s);
+}
+
+// The batched iterations one at a time, each stage once between events on
+// the stream, until every signal has stopped.
+int Batch::RunProfiled(hipStream_t s, int32_t max_iter, double *kernel_ms, int32_t *launches) {
+  hipEvent_t ev[kBatchStages + 1];
+  for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+  for (int i = 0; i < kBatchStages; ++i) kernel_ms[i] = 0.0;
+  *launches = 0;
+  int out = MPSFT_OK;
+  for (int32_t it = 0; it <= max_iter && out == MPSFT_OK; ++it) {
+    if ((out = EnsureAhead(std::min(max_iter, it + 2), max_iter, s)) != MPSFT_OK) break;
+    bool ok = hipEventRecord(ev[0], s) == hipSuccess;
+    for (int st = 0; st < kBatchStages && ok; ++st)
+      ok = LaunchBatchStage(bl_, it & 1, st, s) == hipSuccess &&
+           hipEventRecord(ev[st + 1], s) == hipSuccess;
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+      out = MPSFT_ERR_HIP;
+      break;
+    }
+    for (int st = 0; st < kBatchStages; ++st) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[st], ev[st + 1]) != hipSuccess) out = MPSFT_ERR_HIP;
+      kernel_ms[st] += ms;
+    }
+    ++*launches;
+    if (h_host_[1] >= count()) break;  // this iteration's plan found every signal stopped
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (out != MPSFT_OK) (void)hipStreamSynchronize(s);
+  return out;
+}
+
+// Per-signal results of the batched solve just run.
+int Batch::Finish(int32_t *iters, int32_t *status, hipStream_t s) {
   HIP_OK(LaunchBatchResults(d_ctxs_, count(), d_results_, s));
   HIP_OK(hipMemcpyAsync(h_results_, d_results_, count() * sizeof(BatchResultDev),
                         hipMemcpyDeviceToHost, s));

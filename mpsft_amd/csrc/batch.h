@@ -30,6 +30,12 @@ class Batch {
   // Tries k_bin_time tile widths 1, 2, 4, 8 on a short batched solve of
   // these signals and keeps the fastest; *chosen (may be null) gets it.
   int Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen, hipStream_t s);
+  // Solve with each stage's launches bracketed by events (no graph):
+  // kernel_ms[stage] is the summed milliseconds of that stage's kernel over
+  // the iterations (0 plan, 1 gather-fold, 2 bin FFT, 3 identify) and
+  // *launches the iterations launched.
+  int SolveProfiled(const void *const *x, const uint32_t *seeds, int32_t *iters,
+                    int32_t *status, double *kernel_ms, int32_t *launches, hipStream_t s);
 
   // The differentiable form over the batch: with recording on, a batched
   // solve keeps every signal's decisions (Engine::SetRecord); Adjoint then
@@ -49,6 +55,9 @@ class Batch {
   int Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s);
   int Build(hipStream_t s);
   int Run(hipStream_t s, int32_t max_iter);
+  int RunProfiled(hipStream_t s, int32_t max_iter, double *kernel_ms, int32_t *launches);
+  int Finish(int32_t *iters, int32_t *status, hipStream_t s);
+  int EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s);
 
   int status_ = MPSFT_OK;
   int precision_;

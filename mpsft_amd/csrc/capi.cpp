@@ -398,6 +398,13 @@ int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32
   return batch->b->Solve(x_dev, seeds, iters, status, S(stream));
 }
 
+int mpsft_batch_solve_profiled(mpsft_batch *batch, const void *const *x_dev,
+                               const uint32_t *seeds, int32_t *iters, int32_t *status,
+                               double *kernel_ms, int32_t *launches, void *stream) {
+  if (!batch || !x_dev || !seeds || !kernel_ms || !launches) return MPSFT_ERR_INVALID_ARG;
+  return batch->b->SolveProfiled(x_dev, seeds, iters, status, kernel_ms, launches, S(stream));
+}
+
 int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
                          int32_t *residues, void *stream) {
   if (!batch || !x_dev || !seeds) return MPSFT_ERR_INVALID_ARG;

@@ -78,6 +78,33 @@ int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f, int32_t forced) {
   return R;
 }
 
+// Split of an iteration's gather-fold over the XCDs (1 or kSlices), opt-in
+// with MPSFT_SLICES=8 where n is a power of two and the bin FFT is the LDS
+// one.  Each XCD then gathers only its eighth of x, so its L2 (4 MiB) serves
+// every gather into that eighth; at N = 2^22 complex64 (32 MiB) a gather-
+// only test runs 1.56x faster that way.  But every slice repeats the
+// tile's taps and its dense fold sums, and measured on MI355X the whole
+// solve is 1.25x (one signal) to 2x (16 signals batched) slower, so the
+// unsplit form is the default.  The split is kept for measurement and is
+// checked against the oracle like the default.
+bool SlicingOn(int64_t n, int64_t mask) {
+  static const int mode = [] {
+    const char *v = std::getenv("MPSFT_SLICES");
+    return v ? std::atoi(v) : 0;
+  }();
+  return mode == kSlices && mask >= 0 && n >= 64;
+}
+
+int32_t SlicesFor(int64_t n, int64_t mask, const BParams &bp) {
+  return SlicingOn(n, mask) && bp.logM <= kMaxLogM ? kSlices : 1;
+}
+
+int32_t Log2(int64_t n) {
+  int32_t l = 0;
+  while ((int64_t(1) << (l + 1)) <= n) ++l;
+  return l;
+}
+
 // Transforms drawn before a solve's first launch.
 constexpr int32_t kFirstDraws = 32;
 
@@ -159,6 +186,8 @@ bool WindowParams(int64_t n, int32_t bins, double delta, BParams *bp) {
   bp->M = 1 << logM;
   bp->off = 0;
   bp->R = 1;
+  bp->slices = 1;
+  bp->slice_shift = 0;
   bp->width = width;
   bp->sigma_f = sigma_f;
   bp->dfreq = kSqrt2 * kPi * sigma_t;
@@ -354,8 +383,10 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     BParams bp;
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
-    bp.R = ResiduesPerTile(B, NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f), bp.f,
-                           forced_R_);
+    const int32_t nlam = NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f);
+    bp.R = ResiduesPerTile(B, nlam, bp.f, forced_R_);
+    bp.slices = SlicesFor(n_, mask_, bp);
+    bp.slice_shift = bp.slices > 1 ? Log2(n_) - 3 : 0;
     bp.off = 0;
     if (bp.logM <= kMaxLogM) {
       bp.off = table_used_ + need;
@@ -458,7 +489,7 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
     Bc = std::max(Bc, B);
     rc = std::max(rc, 1 + 2 * bits);
-    tc = std::max(tc, (B + bp.R - 1) / bp.R);
+    tc = std::max(tc, (B + bp.R - 1) / bp.R * bp.slices);
     Mc = std::max(Mc, bp.M);
     const size_t staged = size_t(NumLags(bits, bp.f)) * bp.R;
     const size_t rest = (size_t(bp.f) * bp.R + kThreads) * sizeof(double2);
@@ -485,11 +516,13 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     rows_cap_ = std::max(rows_cap_, rc);
     B_cap_ = std::max(B_cap_, Bc);
     const size_t sz = size_t(trials_cap_) * rows_cap_ * B_cap_;
-    if (Alloc(&d_fold_, sz) != MPSFT_OK || Alloc(&d_A_, sz) != MPSFT_OK ||
+    // room for kSlices partial folds where iterations may split
+    const size_t fold_parts = SlicingOn(n_, mask_) ? kSlices : 1;
+    if (Alloc(&d_fold_, sz * fold_parts) != MPSFT_OK || Alloc(&d_A_, sz) != MPSFT_OK ||
         Alloc(&d_term_bin_, size_t(trials_cap_) * cap_) != MPSFT_OK ||
         Alloc(&d_term_val_, size_t(trials_cap_) * rows_cap_ * cap_) != MPSFT_OK)
       return MPSFT_ERR_NOMEM;
-    HIP_OK(hipMemset(d_fold_, 0, sz * sizeof(double2)));
+    HIP_OK(hipMemset(d_fold_, 0, sz * fold_parts * sizeof(double2)));
     HIP_OK(hipMemset(d_A_, 0, sz * sizeof(double2)));
     tf_valid_ = false;  // q stride follows trials_cap_
     ++shape_gen_;
@@ -617,6 +650,7 @@ Ctx Engine::MakeCtx() const {
   c.n = n_;
   c.mask = mask_;
   c.fold = d_fold_;
+  c.fold_stride = int64_t(trials_cap_) * rows_cap_ * B_cap_;
   c.A = A_override_ ? A_override_ : d_A_;
   c.term_bin = d_term_bin_;
   c.term_val = d_term_val_;
@@ -712,7 +746,7 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
     const BParams &bp = hbtab_[late_B_];
     const int32_t waves = kThreads / 64;
     const int32_t nb_ident = std::max<int32_t>(1, (late_B_ + waves - 1) / waves);
-    const int32_t tiles = trials_cap_ * ((late_B_ + bp.R - 1) / bp.R);
+    const int32_t tiles = trials_cap_ * ((late_B_ + bp.R - 1) / bp.R) * bp.slices;
     for (int k = 2; k < 4; ++k) {
       hipStream_t cs;
       HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

@@ -42,6 +42,7 @@ constexpr int kFftSmallLogM = 10;  // k_bin_fft's in-place radix-4 path up to M 
 constexpr int kMaxLogMBig = 16;    // four-step Bluestein FFT up to M = 65536
 constexpr int kTwLogM = kMaxLogMBig;  // twiddle table e^{-2 pi i j / 2^kTwLogM}
 constexpr int kBigTile = 1024;     // elements of one four-step workgroup
+constexpr int kSlices = 8;         // XCDs of MI355X: partial gather-folds of an eighth of x each
 constexpr int kIdentKeys = 64;     // new keys a k_identify workgroup lists before appending
 constexpr int kPrepBlocks = 16;    // k_bin_time workgroups computing mode terms
 constexpr int kMaxBins = 32767;    // largest B (2B-1 <= 2^kMaxLogMBig)
@@ -54,6 +55,8 @@ constexpr int kMaxTrials = 15;     // trials per iteration (odd)
 struct BParams {
   int32_t B, P, f, bits, M, logM;
   int32_t R;        // residues per k_bin_time tile
+  int32_t slices;   // 1, or kSlices: the gather-fold split by eighths of x (one per XCD)
+  int32_t slice_shift;  // log2(n) - 3 when slices > 1
   int64_t off;      // offset (in double2) of [chirp(B) | filter(M)] in table
                     // (B <= kMaxLdsBins only; larger B compute both on the fly)
   double width;     // 1/(2B)
@@ -106,6 +109,7 @@ struct IterParams {
   double width, sigma_f, dfreq;  // window (src/window.cc:28-50)
   double bin_thr, sv_thr;
   int32_t B, bits, rows, f, P, p2, M, logM, R;
+  int32_t slices, slice_shift;  // as BParams
   int32_t it;    // iteration index
   int32_t done;  // the loop stopped before this iteration
 };
@@ -145,7 +149,8 @@ struct Ctx {
   const int64_t *qs;     // per iteration x trials_cap
   int64_t n;
   int64_t mask;          // n-1 if n is a power of two, else -1
-  double2 *fold;         // trials_cap x rows_cap x B_cap
+  double2 *fold;         // kSlices partial folds, each trials_cap x rows_cap x B_cap
+  int64_t fold_stride;   // entries between partial folds
   double2 *A;            // trials_cap x rows_cap x B_cap (bin coefficients)
   double2 *big;          // (trials_cap x rows_cap + 1) x M_cap: four-step FFT scratch
   int32_t *term_bin;     // trials_cap x cap: bin of each found mode

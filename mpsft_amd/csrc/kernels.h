@@ -70,6 +70,10 @@ struct BatchLaunch {
 hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count,
                             int32_t *counters, hipStream_t s);
 hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s);
+// One stage of a batched iteration: 0 the plan, 1 the gather-fold, 2 the
+// bin FFT, 3 identify.
+constexpr int kBatchStages = 4;
+hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStream_t s);
 struct BatchResultDev {
   int32_t iters, stop_stale, flags, stopped, last_it, count;
 };

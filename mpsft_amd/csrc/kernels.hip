@@ -70,10 +70,13 @@ __device__ __forceinline__ double2 CisFrac(int64_t r, int64_t n) {
 // A sample of x (double2 or float2) in double precision.
 __device__ __forceinline__ double2 Widen(double2 v) { return v; }
 
-// A sample of x read by the gather-fold with the nontemporal hint: the
-// gathers are random 8- or 16-byte reads with little reuse within a launch,
-// and loading them as streaming data measured faster (N=2^22 S=1000 solve
-// 521 -> 507 us, N=2^26 S=4000 1227 -> 1186 us).
+// A sample of x read by the gather-fold.  Every random 8- or 16-byte
+// gather is served as a 128-byte line.  The single-signal kernels load with
+// the nontemporal hint (no L2 allocation: measured faster, N=2^22 S=1000
+// solve 546 -> 525 us, N=2^26 S=4000 1377 -> 1340 us); the batched
+// gather-fold, whose launch gathers from 64 signals' lines, loads them
+// plainly so that a line gathered again by the same XCD hits its L2
+// (64 x N=2^22 S=1000: 170 -> 154 us per solve).
 // x comes to the kernels as a pointer stored in device memory (the loop
 // state), which the compiler cannot place in an address space: its loads
 // would be flat loads, which also count against the LDS wait counter, so
@@ -83,15 +86,24 @@ __device__ __forceinline__ double2 Widen(double2 v) { return v; }
 typedef const unsigned long long __attribute__((address_space(1))) *GlobalU64;
 typedef const double __attribute__((address_space(1))) *GlobalF64;
 
-__device__ __forceinline__ float2 LoadStream(const float2 *p) {
-  const unsigned long long v = __builtin_nontemporal_load((GlobalU64)p);
+typedef const unsigned __int128 __attribute__((address_space(1))) *GlobalU128;
+template <bool kNt>
+__device__ __forceinline__ float2 LoadX(const float2 *p) {
+  const unsigned long long v = kNt ? __builtin_nontemporal_load((GlobalU64)p) : *(GlobalU64)p;
   float2 r;
   __builtin_memcpy(&r, &v, sizeof(r));
   return r;
 }
-__device__ __forceinline__ double2 LoadStream(const double2 *p) {
-  const GlobalF64 q = (GlobalF64)p;
-  return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+template <bool kNt>
+__device__ __forceinline__ double2 LoadX(const double2 *p) {
+  if (kNt) {
+    const GlobalF64 q = (GlobalF64)p;
+    return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+  }
+  const unsigned __int128 v = *(GlobalU128)p;
+  double2 r;
+  __builtin_memcpy(&r, &v, sizeof(r));
+  return r;
 }
 __device__ __forceinline__ double2 Widen(float2 v) { return make_double2(double(v.x), double(v.y)); }
 
@@ -316,6 +328,8 @@ __device__ __forceinline__ void FillParams(IterParams *ip, const BParams &bp, in
   ip->M = bp.M;
   ip->logM = bp.logM;
   ip->R = bp.R;
+  ip->slices = bp.slices;
+  ip->slice_shift = bp.slice_shift;
   ip->tab_off = bp.off;
   ip->width = bp.width;
   ip->sigma_f = bp.sigma_f;
@@ -536,10 +550,17 @@ struct Lags {
 //   phase A gathers the nlam x R distinct samples into LDS (each once),
 //   phase T computes the f x R taps into LDS (each once per iteration),
 //   phase B forms every (row, rho) sum, split over `parts` threads.
+// With ip.slices = kSlices the tile is split over the 8 XCDs: its unit s
+// gathers only the samples in the s-th eighth of x (index >> slice_shift ==
+// s; the others read as 0) and writes partial fold s.  Units go to XCDs
+// round-robin (unit s of every tile on XCD s), so each XCD's L2 serves one
+// eighth of x, and a line of x is fetched by one XCD only, however many of
+// the tile's gathers it serves.  The bin FFT adds the partials in the fixed
+// order s = 0..7.
 // Workgroups below kPrepBlocks instead compute the BinInFreq terms of every
 // found mode for this iteration.
 // blk: this workgroup's block index within the signal's launch.
-template <typename Tx>
+template <typename Tx, bool kNt = true>
 __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int32_t trials,
                                          int32_t tile, double2 *lds);
 
@@ -615,7 +636,7 @@ __device__ __forceinline__ void BinTimeBody(const Ctx &c, int p, int blk, double
 
 // The gather-fold of one tile (trial, block of R residues) of an iteration
 // with parameters ip (see BinTimeBody).
-template <typename Tx>
+template <typename Tx, bool kNt>
 __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int32_t trials,
                                          int32_t tile, double2 *lds) {
   const DevState *st = c.st;
@@ -624,6 +645,9 @@ __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int
   const int64_t n = c.n, mask = c.mask;
   const int32_t R = ip.R;
   const int32_t nblk = (B + R - 1) / R;
+  const int32_t S = ip.slices;
+  const int32_t slice = tile % S;  // this unit's eighth of x (0 unsliced)
+  tile /= S;
   if (tile >= trials * nblk) return;
   const int32_t trial = tile / nblk;
   const int32_t rho0 = (tile - trial * nblk) * R;
@@ -659,7 +683,8 @@ __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int
       g[k] = Tx{};
       if (idx < total && ri < nrho) {
         const int64_t u = ubase + L.Lambda(li) * B + ri;
-        g[k] = LoadStream(x + ModN(tf.a * u + tf.c, n, mask));
+        const int64_t j = ModN(tf.a * u + tf.c, n, mask);
+        if (S == 1 || int32_t(j >> ip.slice_shift) == slice) g[k] = LoadX<kNt>(x + j);
       }
     }
     if (base == 0) {
@@ -693,7 +718,7 @@ __device__ __forceinline__ void TileFold(const Ctx &c, const IterParams &ip, int
   int32_t parts = 1;
   while (parts * 2 * pairs <= kThreads && parts * 2 <= f) parts *= 2;
   const int32_t h = (B - 1) / 2;
-  double2 *fold_t = c.fold + int64_t(trial) * c.rows_cap * c.B_cap;
+  double2 *fold_t = c.fold + slice * c.fold_stride + int64_t(trial) * c.rows_cap * c.B_cap;
   for (int base = 0; base < pairs * parts; base += blockDim.x) {
     const int p = base + threadIdx.x;
     const int pair = p / parts;
@@ -997,7 +1022,20 @@ __device__ __forceinline__ void StoreBins(double2 *A, int32_t B, const double2 (
 // Either path leaves X_k (chirp and row phase applied) in LDS, at k on the
 // small path and at Pad(k) on the large one.
 
-// CONJ_IN: transform conj(fold) (the adjoint pass feeds conjugated bins).
+// Fold entry k: with a gather-fold split over XCD slices (parts =
+// kSlices), the sum of the partial folds in the fixed order 0..7.
+__device__ __forceinline__ double2 FoldAt(const double2 *f, int k, int32_t parts,
+                                          int64_t stride) {
+  double2 v = f[k];
+  if (parts > 1) {
+#pragma unroll
+    for (int s = 1; s < kSlices; ++s) v = Add(v, f[s * stride + k]);
+  }
+  return v;
+}
+
+// CONJ_IN: transform conj(fold) (the adjoint pass feeds conjugated bins,
+// never split); otherwise fold is partial fold 0 of ip.slices.
 template <bool CONJ_IN>
 __device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, const double2 *fold,
                                             double2 rp, double2 *a) {
@@ -1016,7 +1054,7 @@ __device__ __forceinline__ void BinFftSmall(const Ctx &c, const IterParams &ip, 
     const int k = tid + r * T;
     fv[r] = cv[r] = make_double2(0.0, 0.0);
     if (k < B) {
-      fv[r] = CONJ_IN ? Conj(fold[k]) : fold[k];
+      fv[r] = CONJ_IN ? Conj(fold[k]) : FoldAt(fold, k, ip.slices, c.fold_stride);
       cv[r] = chirp[k];
     }
   }
@@ -1098,7 +1136,8 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
     const int m = j + r * nb0;
     v[e] = hv[e] = make_double2(0.0, 0.0);
     if (b < NB0 && j < nb0) {
-      if (m < B) v[e] = Cmul(CONJ_IN ? Conj(fold[m]) : fold[m], chirp[m]);
+      if (m < B)
+        v[e] = Cmul(CONJ_IN ? Conj(fold[m]) : FoldAt(fold, m, ip.slices, c.fold_stride), chirp[m]);
       hv[e] = filt[m];
     }
   }
@@ -2013,15 +2052,21 @@ __global__ void __launch_bounds__(kThreads)
 // per wavefront).
 constexpr int kPlanThreads = 1024;
 
-// Workgroups computing BinInFreq terms for `terms` (modes x trials).
+// Workgroups computing BinInFreq terms for `terms` (modes x trials), a
+// multiple of 8 so that tile units keep their XCD (unit index mod 8).
 __device__ __forceinline__ int32_t PrepUnits(int64_t terms) {
-  return int32_t(min<int64_t>(kPrepBlocks, (terms + kThreads - 1) / kThreads));
+  const int32_t u = int32_t(min<int64_t>(kPrepBlocks, (terms + kThreads - 1) / kThreads));
+  return (u + 7) & ~7;
 }
 
 __global__ void __launch_bounds__(kPlanThreads)
     k_batch_plan(const Ctx *ctxs, int32_t count, int p, BatchWork w, int32_t ident_bins,
                  int32_t *counters, int32_t *host) {
   __shared__ int32_t part[3][kPlanThreads];
+  // with at most kPlanThreads signals (one per thread): each signal's unit
+  // counts, rows and rows_cap, for the map fill below
+  __shared__ int32_t s_cnt[3][kPlanThreads];
+  __shared__ int32_t s_rows[2][kPlanThreads];
   const int tid = threadIdx.x;
   // signals [i0, i1) belong to this thread
   const int32_t per = (count + kPlanThreads - 1) / kPlanThreads;
@@ -2037,16 +2082,22 @@ __global__ void __launch_bounds__(kPlanThreads)
     int32_t u[3] = {0, 0, 0};
     if (!ip.done) {
       const int32_t trials = c.st->trials;
-      u[0] = PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
-             trials * ((ip.B + ip.R - 1) / ip.R);
+      // a multiple of 8 units per signal: every signal's units start on XCD 0
+      u[0] = (PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
+              trials * ((ip.B + ip.R - 1) / ip.R) * ip.slices + 7) & ~7;
       u[1] = trials * ip.rows;
       u[2] = (ip.B + ident_bins - 1) / ident_bins;
     }
     for (int k = 0; k < 3; ++k) {
       w.cnt[k * count + i] = u[k];
       sum[k] += u[k];
+      if (per == 1) s_cnt[k][tid] = u[k];
     }
     w.rows[i] = ip.rows;
+    if (per == 1) {
+      s_rows[0][tid] = ip.rows;
+      s_rows[1][tid] = c.rows_cap;
+    }
   }
   for (int k = 0; k < 3; ++k) part[k][tid] = sum[k];
   __syncthreads();
@@ -2066,9 +2117,9 @@ __global__ void __launch_bounds__(kPlanThreads)
     }
   if (tid == kPlanThreads - 1)
     for (int k = 0; k < 3; ++k) w.total[k] = part[k][tid];
-  // the counts and offsets above, and the stop counts of PublishIteration,
-  // are read below by other threads of this workgroup (device-scope loads)
-  __threadfence();
+  // with more signals than threads, the counts and offsets above are read
+  // below by other threads of this workgroup (device-scope loads)
+  if (per > 1) __threadfence();
   __syncthreads();
   if (tid == 0) {
     // The single writer of the host's copy: batched iterations started and
@@ -2082,13 +2133,22 @@ __global__ void __launch_bounds__(kPlanThreads)
   // the maps: one wavefront per signal
   const int lane = tid & 63;
   for (int32_t i = tid >> 6; i < count; i += kPlanThreads / 64) {
-    int32_t off[3], cnt[3];
-    for (int k = 0; k < 3; ++k) {
-      off[k] = __hip_atomic_load(&w.off[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      cnt[k] = __hip_atomic_load(&w.cnt[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int32_t off[3], cnt[3], rows, rows_cap;
+    if (per == 1) {  // signal i belonged to thread i: its counts and inclusive sums in LDS
+      for (int k = 0; k < 3; ++k) {
+        cnt[k] = s_cnt[k][i];
+        off[k] = part[k][i] - cnt[k];
+      }
+      rows = s_rows[0][i];
+      rows_cap = s_rows[1][i];
+    } else {
+      for (int k = 0; k < 3; ++k) {
+        off[k] = __hip_atomic_load(&w.off[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cnt[k] = __hip_atomic_load(&w.cnt[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      rows = __hip_atomic_load(&w.rows[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      rows_cap = ctxs[i].rows_cap;
     }
-    const int32_t rows = __hip_atomic_load(&w.rows[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int32_t rows_cap = ctxs[i].rows_cap;
     for (int32_t u = lane; u < cnt[0]; u += 64) w.map[0][off[0] + u] = make_int2(i, u);
     for (int32_t u = lane; u < cnt[1]; u += 64) {
       const int32_t trial = u / rows;
@@ -2109,7 +2169,7 @@ __device__ __forceinline__ void BinTimeUnit(const Ctx &c, int p, int32_t unit, d
   if (unit < nprep)
     FreqTerms(c, ip, trials, unit, nprep);
   else
-    TileFold<Tx>(c, ip, trials, unit - nprep, lds);
+    TileFold<Tx, false>(c, ip, trials, unit - nprep, lds);
 }
 
 template <typename Tx>
@@ -2737,21 +2797,39 @@ hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count
   return hipGetLastError();
 }
 
-hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s) {
+hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStream_t s) {
   const BatchWork &w = b.work;
-  hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, s, b.ctxs, b.count, parity, w,
-                     b.ident_bins, b.counters, b.host);
-  if (b.mixed)
-    hipLaunchKernelGGL(k_bin_time_batch<float2>, dim3(b.grid_time), dim3(kThreads), b.lds_time, s,
-                       b.ctxs, w.map[0], w.total, parity);
-  else
-    hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.grid_time), dim3(kThreads), b.lds_time,
-                       s, b.ctxs, w.map[0], w.total, parity);
-  hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.grid_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
-                     w.map[1], w.total, parity);
-  hipLaunchKernelGGL(k_identify_batch, dim3(b.grid_ident), dim3(kThreads), 0, s, b.ctxs, w.map[2],
-                     w.total, w.cnt + 2 * b.count, parity);
+  switch (stage) {
+    case 0:
+      hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, s, b.ctxs, b.count, parity,
+                         w, b.ident_bins, b.counters, b.host);
+      break;
+    case 1:
+      if (b.mixed)
+        hipLaunchKernelGGL(k_bin_time_batch<float2>, dim3(b.grid_time), dim3(kThreads),
+                           b.lds_time, s, b.ctxs, w.map[0], w.total, parity);
+      else
+        hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.grid_time), dim3(kThreads),
+                           b.lds_time, s, b.ctxs, w.map[0], w.total, parity);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.grid_fft), dim3(kFftThreads), b.lds_fft, s,
+                         b.ctxs, w.map[1], w.total, parity);
+      break;
+    default:
+      hipLaunchKernelGGL(k_identify_batch, dim3(b.grid_ident), dim3(kThreads), 0, s, b.ctxs,
+                         w.map[2], w.total, w.cnt + 2 * b.count, parity);
+      break;
+  }
   return hipGetLastError();
+}
+
+hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s) {
+  for (int stage = 0; stage < kBatchStages; ++stage) {
+    const hipError_t e = LaunchBatchStage(b, parity, stage, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Blocks per signal of k_batch_modes_out.

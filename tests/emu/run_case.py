@@ -273,6 +273,7 @@ def identify(n, k, sigma, seed, trials=1):
 
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
+    os.environ.update(case.pop("setenv", {}))  # read by the library at its first table build
     fn = {"solve": solve, "iterate": iterate, "plan": plan, "batch": batch,
           "adjoint": adjoint, "batch_adjoint": batch_adjoint, "gen": gen, "analyze": analyze,
           "identify": identify, "plan_freq": plan_freq}[case.pop("kind")]

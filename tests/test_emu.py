@@ -56,6 +56,13 @@ CASES = {
     "solve_2": dict(kind="solve", n=1 << 16, k=300, sigma=1e-6, seed=5, precision=0, repeat=1),
     "batch": dict(kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8], sigma=1e-6,
                   seeds=[3, 4, 5, 6]),
+    # the gather-fold split over the 8 XCD slices of x, forced at small n
+    "solve_sliced": dict(kind="solve", n=1 << 14, k=40, sigma=3e-6, seed=7, precision=1,
+                         repeat=1, setenv={"MPSFT_SLICES": "8"}),
+    "batch_sliced": dict(kind="batch", ns=[4096, 8192, 16384, 4096], ks=[5, 12, 20, 8],
+                         sigma=1e-6, seeds=[3, 4, 5, 6], setenv={"MPSFT_SLICES": "8"}),
+    "plan_sliced": dict(kind="plan", n=4096, bins=101, bits=5, seed=1,
+                        setenv={"MPSFT_SLICES": "8"}),
     "adjoint_16384_20_1": dict(kind="adjoint", n=1 << 14, k=20, sigma=1e-6, seed=7, trials=1),
     "adjoint_16384_20_3": dict(kind="adjoint", n=1 << 14, k=20, sigma=1e-6, seed=7, trials=3),
     "adjoint_65536_300_1": dict(kind="adjoint", n=1 << 16, k=300, sigma=1e-6, seed=7, trials=1),
@@ -130,6 +137,18 @@ def test_emu_solve_graph(emu_runs, case):
     assert r["back_to_back_iters"] == [r["iters0"]] * 3 and r["back_to_back_same"]
     assert r["bins"] == r["ref_bins"] and r["same_support"]
     assert r["max_rel_err"] < 1e-10
+
+
+def test_emu_sliced_gather_fold(emu_runs):
+    """The gather-fold split over the 8 XCD slices of x (forced): a plan's
+    BinInTime, a single solve through the graphs and the late form, and the
+    batched form all agree with the oracle as the unsplit form does."""
+    assert result(emu_runs, "plan_sliced")["rel_err"] < 1e-12
+    r = result(emu_runs, "solve_sliced")
+    assert r["bins"] == r["ref_bins"] and r["same_support"] and r["max_rel_err"] < 1e-10
+    r = result(emu_runs, "batch_sliced")
+    assert r["status"] == [0, 0, 0, 0]
+    assert all(r["same_as_single"]) and all(r["same_as_oracle"])
 
 
 def test_emu_batch_matches_single_and_oracle(emu_runs):

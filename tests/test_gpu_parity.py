@@ -548,6 +548,36 @@ def test_batch_config4_full_workload(gpu, orc):
         assert_coefs_close(got[i], ref.to_dict(), MIXED_TOL, f"oracle {i}")
 
 
+def test_batch_more_signals_than_plan_threads(gpu):
+    """1100 small complex64 signals in one batched solve: more signals than
+    the batched plan kernel's threads (1024), so each plan thread starts the
+    iterations of two signals and the work maps are filled from device
+    memory.  Every signal stops normally, and a spread of 24 signals equal
+    their single-signal solves."""
+    import torch
+    from mpsft_amd.signals import make_signal_device
+    count, sigma = 1100, 1e-6
+    rng = np.random.default_rng(11)
+    ns = [1 << int(v) for v in rng.integers(12, 15, count)]
+    ks = [int(v) for v in rng.integers(1, 21, count)]
+    seeds = [int(v) for v in rng.integers(1, 1 << 30, count)]
+    xs, opts = [], []
+    for i, (n, k) in enumerate(zip(ns, ks)):
+        x, sq, _ = make_signal_device(n, k, sigma, seed=9000 + i)
+        xs.append(x)
+        opts.append(gpu.solve_options(k, float(np.hypot(sigma, sq))))
+    torch.cuda.synchronize()
+    b = gpu.Batch(ns, opts, gpu.MPSFT_MIXED)
+    iters, stats = b.solve(xs, seeds)
+    assert stats == [0] * count
+    for i in np.linspace(0, count - 1, 24).astype(int):
+        s = gpu.Solver(ns[i], opts[i], gpu.MPSFT_MIXED)
+        assert s.solve(xs[i], seeds[i]) == iters[i], i
+        assert np.array_equal(s.log()[0], b.log(i)[0]), i
+        assert_coefs_close(b.modes_dict(i), s.modes_dict(), 1e-12, f"single {i}")
+        del s
+
+
 # ---------------------------------------------------------------- edge cases
 def test_solve_zero_modes_and_errors(gpu):
     n = 1 << 12
