@@ -16,9 +16,11 @@ namespace {
     if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
   } while (0)
 
-// Workgroups of each packed batched launch (default: 8 per CU of MI355X's
-// 256), and bins per k_identify unit (default 16: four per wavefront), each
-// overridable for measurement by MPSFT_PACKED_GRID / MPSFT_IDENT_BINS.
+// Workgroups of each packed batched launch (default 8192: 32 per CU of
+// MI355X's 256; measured on 64 config-2 signals, 2048: 152.2, 4096: 148.9,
+// 8192: 145.9 us per solve), and bins per k_identify unit (default 16: four
+// per wavefront), each overridable for measurement by MPSFT_PACKED_GRID /
+// MPSFT_IDENT_BINS.
 int32_t EnvInt(const char *name, int32_t def, int32_t lo, int32_t hi) {
   const char *e = std::getenv(name);
   if (!e) return def;
@@ -58,6 +60,10 @@ Batch::Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, i
       status_ = eng_.back()->status();
       return;
     }
+    // wider gather-fold tiles when many signals fill the launches: about
+    // 2048 tiles per trial over the batch (measured on 64 config-2 signals:
+    // 512 tiles per signal 152.2, 64 per signal 144.7 us per solve)
+    eng_.back()->tile_target_ = std::max(16, std::min(kTargetTiles, 2048 / count));
     opts_.push_back(opts[i]);
   }
   args_.resize(count);
@@ -178,7 +184,7 @@ int Batch::Build(hipStream_t s) {
   bl_.nb_ident = int32_t(map_ident.size());
   // The packed launches: at most kPackedGrid workgroups each (8 per CU),
   // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
-  const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 2048, 64, 1 << 20);
+  const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 8192, 64, 1 << 20);
   bl_.ident_bins = EnvInt("MPSFT_IDENT_BINS", 16, 4, kIdentKeys) / 4 * 4;
   bl_.grid_time = std::max(8, std::min(bl_.nb_time, grid) & ~7);  // a multiple of 8
   bl_.grid_fft = std::max(1, std::min(bl_.nb_fft, grid));

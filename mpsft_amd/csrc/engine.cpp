@@ -62,17 +62,17 @@ int32_t NumLags(int32_t bits, int32_t f) {
 }
 
 // Residues per k_bin_time tile: the largest power of two <= 8 that still
-// gives kTargetTiles tiles, and keeps the tile's LDS under 64 KiB.  A
-// forced width (forced > 0, from the autotuner) is kept within the LDS of
-// one workgroup.
-int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f, int32_t forced) {
+// gives `target` tiles, and keeps the tile's LDS under 64 KiB.  A forced
+// width (forced > 0, from the autotuner) is kept within the LDS of one
+// workgroup.
+int32_t ResiduesPerTile(int32_t B, int32_t nlam, int32_t f, int32_t forced, int32_t target) {
   if (forced > 0) {
     int32_t R = forced;
     while (R > 1 && (size_t(nlam + f) * R + kThreads) * sizeof(double2) > 160 * 1024) R /= 2;
     return R;
   }
   int32_t R = 8;
-  while (R > 1 && ((B + R - 1) / R < kTargetTiles ||
+  while (R > 1 && ((B + R - 1) / R < target ||
                    size_t(nlam + f) * R * sizeof(double2) > 65536))
     R /= 2;
   return R;
@@ -384,7 +384,7 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
     const int32_t nlam = NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f);
-    bp.R = ResiduesPerTile(B, nlam, bp.f, forced_R_);
+    bp.R = ResiduesPerTile(B, nlam, bp.f, forced_R_, tile_target_);
     bp.slices = SlicesFor(n_, mask_, bp);
     bp.slice_shift = bp.slices > 1 ? Log2(n_) - 3 : 0;
     bp.off = 0;
@@ -470,7 +470,7 @@ int Engine::SetTileResidues(int32_t R) {
     if (!built_[B]) continue;
     BParams &bp = hbtab_[B];
     const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
-    bp.R = ResiduesPerTile(int32_t(B), NumLags(bits, bp.f), bp.f, forced_R_);
+    bp.R = ResiduesPerTile(int32_t(B), NumLags(bits, bp.f), bp.f, forced_R_, tile_target_);
   }
   if (d_btab_)
     HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),

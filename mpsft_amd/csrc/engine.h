@@ -355,6 +355,9 @@ class Engine {
   int32_t graph_late_iters_ = 2;            // iterations per late graph
   int64_t shape_gen_ = 0;
   int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
+  // k_bin_time tiles aimed for per trial by the default tile width: fewer
+  // (wider tiles) when many signals share the launches of a batch
+  int32_t tile_target_ = kTargetTiles;
 
   // recorded solve
   bool record_ = false;
