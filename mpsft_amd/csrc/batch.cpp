@@ -136,7 +136,7 @@ int Batch::Build(hipStream_t s) {
   if (same) return MPSFT_OK;
   std::vector<int2> map_time, map_fft, map_ident;
   std::vector<int32_t> nblk_ident(eng_.size());
-  size_t lds_time = 0, lds_fft = 0;
+  size_t lds_time = 0, lds_fft = 0, lds_fft_small = 0;
   for (size_t i = 0; i < eng_.size(); ++i) {
     Engine &e = *eng_[i];
     const int32_t si = int32_t(i);
@@ -149,6 +149,7 @@ int Batch::Build(hipStream_t s) {
     for (int32_t b = 0; b < nblk_ident[i]; ++b) map_ident.push_back(make_int2(si, b));
     lds_time = std::max(lds_time, e.LdsBinTime(precision_ == MPSFT_MIXED));
     lds_fft = std::max(lds_fft, e.LdsFft());
+    lds_fft_small = std::max(lds_fft_small, e.lds_fft_small_);
     built_gen_[i] = e.shape_gen_;
   }
   // the adjoint's DFT-and-terms launch: kPrepBlocks terms blocks, then the
@@ -164,7 +165,8 @@ int Batch::Build(hipStream_t s) {
   const size_t total = map_time.size() + map_fft.size() + map_ident.size();
   const size_t K = eng_.size();
   if (AllocDev(&d_map_, total) != MPSFT_OK || AllocDev(&d_nblk_ident_, K) != MPSFT_OK ||
-      AllocDev(&d_work_map_, total) != MPSFT_OK || AllocDev(&d_work_i32_, 7 * K + 3) != MPSFT_OK)
+      AllocDev(&d_work_map_, total + map_fft.size()) != MPSFT_OK ||
+      AllocDev(&d_work_i32_, (2 * kUnitKinds + 1) * K + kUnitKinds) != MPSFT_OK)
     return MPSFT_ERR_NOMEM;
   HIP_OK(hipMemcpy(d_map_, map_time.data(), map_time.size() * sizeof(int2),
                    hipMemcpyHostToDevice));
@@ -191,14 +193,16 @@ int Batch::Build(hipStream_t s) {
   bl_.grid_ident = std::max(1, std::min(bl_.nb_ident, grid));
   bl_.count = int32_t(K);
   bl_.work.cnt = d_work_i32_;
-  bl_.work.off = d_work_i32_ + 3 * K;
-  bl_.work.rows = d_work_i32_ + 6 * K;
-  bl_.work.total = d_work_i32_ + 7 * K;
+  bl_.work.off = d_work_i32_ + kUnitKinds * K;
+  bl_.work.rows = d_work_i32_ + 2 * kUnitKinds * K;
+  bl_.work.total = d_work_i32_ + (2 * kUnitKinds + 1) * K;
   bl_.work.map[0] = d_work_map_;
   bl_.work.map[1] = d_work_map_ + map_time.size();
   bl_.work.map[2] = d_work_map_ + map_time.size() + map_fft.size();
+  bl_.work.map[3] = d_work_map_ + total;  // the large-path FFT units, as many as map_fft
   bl_.lds_time = lds_time;
   bl_.lds_fft = lds_fft;
+  bl_.lds_fft_small = lds_fft_small;
   bl_.counters = d_counters_;
   bl_.host = d_host_;
   bl_.mixed = precision_ == MPSFT_MIXED ? 1 : 0;

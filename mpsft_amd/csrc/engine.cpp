@@ -482,7 +482,7 @@ int Engine::SetTileResidues(int32_t R) {
 // Sizes the per-iteration buffers for trials and every B in bs.
 int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
   int32_t Bc = 0, rc = 0, tc = 0, Mc = 0, Mbig = 0;
-  size_t lbt = 0, lbt32 = 0, lfft = 0;
+  size_t lbt = 0, lbt32 = 0, lfft = 0, lfft_small = 0;
   bool small = false, big = false;
   for (int32_t B : bs) {
     const BParams &bp = hbtab_[B];
@@ -503,6 +503,7 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
       const size_t el = bp.logM <= kFftSmallLogM ? size_t(bp.M + bp.M / 2)
                                                  : size_t(bp.M + bp.M / 16);
       lfft = std::max(lfft, el * sizeof(double2));
+      if (bp.logM <= kFftSmallLogM) lfft_small = std::max(lfft_small, el * sizeof(double2));
     } else {
       big = true;
       Mbig = std::max(Mbig, bp.M);
@@ -544,7 +545,8 @@ int Engine::Reserve(int32_t trials, const std::vector<int32_t> &bs) {
     ++shape_gen_;
   }
   if (tc > tiles_cap_ || Mc > M_cap_ || lbt > lds_bin_time_ || lbt32 > lds_bin_time32_ ||
-      lfft > lds_fft_) {
+      lfft > lds_fft_ || lfft_small > lds_fft_small_) {
+    lds_fft_small_ = std::max(lds_fft_small_, lfft_small);
     tiles_cap_ = std::max(tiles_cap_, tc);
     M_cap_ = std::max(M_cap_, Mc);
     lds_bin_time_ = std::max(lds_bin_time_, lbt);
@@ -747,6 +749,10 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
     const int32_t waves = kThreads / 64;
     const int32_t nb_ident = std::max<int32_t>(1, (late_B_ + waves - 1) / waves);
     const int32_t tiles = trials_cap_ * ((late_B_ + bp.R - 1) / bp.R) * bp.slices;
+    // the late B's bin FFT on its own path: the small one's kernel and LDS
+    // (array M and M/2 twiddles) where it applies
+    const bool small_fft = bp.logM <= kFftSmallLogM;
+    const size_t lds_fft = small_fft ? size_t(bp.M + bp.M / 2) * sizeof(double2) : LdsFft();
     for (int k = 2; k < 4; ++k) {
       hipStream_t cs;
       HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -757,8 +763,8 @@ int Engine::EnsureGraph(int32_t iters, hipStream_t s) {
       }
       hipError_t el = hipSuccess;
       for (int32_t i = 0; i < late_iters && el == hipSuccess; ++i)
-        el = LaunchLateIteration(c, i & 1, nb_ident, tiles, LdsBinTime(c.mixed != 0), LdsFft(),
-                                 (k == 3 || i > 0) ? 1 : 0, cs);
+        el = LaunchLateIteration(c, i & 1, nb_ident, tiles, LdsBinTime(c.mixed != 0), lds_fft,
+                                 (k == 3 || i > 0) ? 1 : 0, small_fft ? 1 : 0, cs);
       const hipError_t ec = hipStreamEndCapture(cs, &g);
       (void)hipStreamDestroy(cs);
       if (el != hipSuccess || ec != hipSuccess) return MPSFT_ERR_HIP;

@@ -354,6 +354,7 @@ class Engine {
   int32_t late_B_ = 0, graph_late_B_ = 0;  // min_bins of the solve (0: no late graphs)
   int32_t graph_late_iters_ = 2;            // iterations per late graph
   int64_t shape_gen_ = 0;
+  size_t lds_fft_small_ = 0;  // k_bin_fft's LDS on its small path (largest reserved B)
   int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
   // k_bin_time tiles aimed for per trial by the default tile width: fewer
   // (wider tiles) when many signals share the launches of a batch

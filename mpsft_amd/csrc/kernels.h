@@ -28,9 +28,10 @@ hipError_t LaunchReset(const Ctx &c, const InitArgs &a, hipStream_t s);
 // One late iteration of a solve (every iteration from here on has
 // B = min_bins <= kMaxLdsBins): k_late_fold (the previous iteration's
 // identify unless with_identify is 0, beside this one's BinInFreq factors
-// and gather-fold over `tiles` tiles) then k_late_fft.
+// and gather-fold over `tiles` tiles) then k_late_fft (the kernel of the
+// small bin-FFT path alone when small_fft, with lds_fft sized for it).
 hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32_t tiles,
-                               size_t lds_time, size_t lds_fft, int with_identify,
+                               size_t lds_time, size_t lds_fft, int with_identify, int small_fft,
                                hipStream_t s);
 // parity = iteration & 1: the IterParams slot the kernels use.
 hipError_t LaunchBinTime(const Ctx &c, int parity, int32_t tiles, size_t lds, hipStream_t s);
@@ -44,14 +45,16 @@ hipError_t LaunchPublish(const Ctx &c, int parity, hipStream_t s);
 
 // The batched form: per-stage launches over every signal of a batch.
 // BatchWork (device) is rewritten by each batched iteration's plan: per
-// launch k (0 k_bin_time, 1 k_bin_fft, 2 k_identify) the units of signal i
+// kind k (0 k_bin_time, 1 k_bin_fft on the small path, 2 k_identify, 3
+// k_bin_fft on the large path) the units of signal i
 // (cnt[k count + i] of them from off[k count + i]) and the packed map of
 // {signal, unit} entries, total[k] in all.
+constexpr int kUnitKinds = 4;  // gather-fold, small-path FFT, identify, large-path FFT
 struct BatchWork {
-  int32_t *cnt, *off;  // 3 x count each
+  int32_t *cnt, *off;  // kUnitKinds x count each
   int32_t *rows;       // count: the iteration's rows per signal
-  int32_t *total;      // 3
-  int2 *map[3];        // capacities: the units of every signal at its largest B
+  int32_t *total;      // kUnitKinds
+  int2 *map[kUnitKinds];  // capacities: the units of every signal at its largest B
 };
 struct BatchLaunch {
   const Ctx *ctxs;                      // device, one per signal
@@ -63,6 +66,7 @@ struct BatchLaunch {
   int32_t count;                        // signals
   BatchWork work;
   size_t lds_time, lds_fft;             // dynamic LDS (max over signals)
+  size_t lds_fft_small;                 // the small bin-FFT path's (max over signals)
   int32_t *counters;                    // device {iterations started, signals stopped}
   int32_t *host;                        // host-mapped copy of counters
   int mixed;

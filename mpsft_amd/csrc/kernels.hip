@@ -985,11 +985,26 @@ __device__ __forceinline__ double2 LimbValue(const unsigned long long *acc, cons
 // Pad(k) on the large path), moved into registers so that the BinInFreq
 // limbs can take the FFT array's place; the caller has synchronised after
 // the FFT.
-constexpr int kOutPer = 2048 / kFftThreads;  // B <= 2048 on the LDS paths
+// Which bin-FFT paths a kernel carries: both (the path picked per launch
+// from M), or one of them alone, which then runs with the registers and LDS
+// of that path only (the small path's kernels reach several wavefronts per
+// SIMD; with both paths the large one's 256 VGPRs set one).
+enum : int { kFftBoth = 0, kFftSmallOnly = 1, kFftLargeOnly = 2 };
+template <int kPaths>
+constexpr int OutPer() {  // B <= 512 on the small path, <= 2048 on the large one
+  return kPaths == kFftSmallOnly ? 512 / kFftThreads : 2048 / kFftThreads;
+}
+template <int kPaths>
+__device__ __forceinline__ bool FftPathMatches(const IterParams &ip) {
+  const bool small = ip.logM <= kFftSmallLogM;
+  return kPaths == kFftBoth || (kPaths == kFftSmallOnly) == small;
+}
+
+template <int N>
 __device__ __forceinline__ void TakeOutputs(const double2 *a, bool small, int32_t B,
-                                            double2 (&xo)[kOutPer]) {
+                                            double2 (&xo)[N]) {
 #pragma unroll
-  for (int r = 0; r < kOutPer; ++r) {
+  for (int r = 0; r < N; ++r) {
     const int k = threadIdx.x + r * kFftThreads;
     xo[r] = k < B ? a[small ? k : Pad(k)] : make_double2(0.0, 0.0);
   }
@@ -997,10 +1012,11 @@ __device__ __forceinline__ void TakeOutputs(const double2 *a, bool small, int32_
 }
 
 // A[k] = X_k - (the bin's term sum), for the outputs TakeOutputs took.
-__device__ __forceinline__ void StoreBins(double2 *A, int32_t B, const double2 (&xo)[kOutPer],
+template <int N>
+__device__ __forceinline__ void StoreBins(double2 *A, int32_t B, const double2 (&xo)[N],
                                           const unsigned long long *limbs, const TermGrid &g) {
 #pragma unroll
-  for (int r = 0; r < kOutPer; ++r) {
+  for (int r = 0; r < N; ++r) {
     const int k = threadIdx.x + r * kFftThreads;
     if (k < B) A[k] = g.any ? Sub(xo[r], LimbValue(limbs + kLimbs * k, g)) : xo[r];
   }
@@ -1196,12 +1212,13 @@ __device__ __forceinline__ void BinFftLarge(const Ctx &c, const IterParams &ip, 
   }
 }
 
+template <int kPaths>
 __device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2 *lds) {
   constexpr int T = kFftThreads;
   constexpr int kPerTerm = 2;  // BinInFreq terms prefetched per thread
   const DevState *st = c.st;
   const IterParams &ip = st->P[p];
-  if (ip.done || ip.logM > kMaxLogM) return;
+  if (ip.done || ip.logM > kMaxLogM || !FftPathMatches<kPaths>(ip)) return;
   const int32_t trials = st->trials, rows = ip.rows;
   const int32_t trial = blk / c.rows_cap;
   const int32_t row = blk - trial * c.rows_cap;
@@ -1229,15 +1246,21 @@ __device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2
   const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
   double2 *a = lds;
   const bool small = ip.logM <= kFftSmallLogM;
-  if (small)
+  if constexpr (kPaths == kFftSmallOnly) {
     BinFftSmall<false>(c, ip, fold, rp, a);
-  else
+  } else if constexpr (kPaths == kFftLargeOnly) {
     BinFftLarge<false>(c, ip, fold, rp, a);
+  } else {
+    if (small)
+      BinFftSmall<false>(c, ip, fold, rp, a);
+    else
+      BinFftLarge<false>(c, ip, fold, rp, a);
+  }
   __syncthreads();
   // BinInFreq: every bin's terms summed exactly (TermGrid), then A = X - sum.
   // The limbs (2B double2 slots <= M) take the FFT array's place once the
   // outputs are in registers.
-  double2 xo[kOutPer];
+  double2 xo[OutPer<kPaths>()];
   TakeOutputs(a, small, B, xo);
   unsigned long long *limbs = reinterpret_cast<unsigned long long *>(a);
   double mx = 0.0;
@@ -1259,7 +1282,7 @@ __device__ __forceinline__ void BinFftBody(const Ctx &c, int p, int blk, double2
 
 __global__ void __launch_bounds__(kFftThreads) k_bin_fft(Ctx c, int p) {
   HIP_DYNAMIC_SHARED(double2, lds)
-  BinFftBody(c, p, blockIdx.x, lds);
+  BinFftBody<kFftBoth>(c, p, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,32 +1440,52 @@ __global__ void __launch_bounds__(kThreads) k_big_out(Ctx c, int p) {
   }
   // BinInFreqV1 (src/binner.cc:565-591): the found modes whose bin lies in
   // this workgroup's columns, summed exactly per bin (TermGrid) and
-  // subtracted once.
+  // subtracted once.  The outputs move to registers and the limbs take the
+  // tile's place in LDS, half of the tile's bins at a time (4 limbs per bin:
+  // half a tile of limbs fills the array).
+  constexpr int kPer = kBigTile / kThreads;
+  constexpr int kHalf = kBigTile / 2;
   const int32_t cnt = min(st->count, c.cap);
   const int32_t *tb = c.term_bin + int64_t(trial) * c.cap;
   const double2 *tv = c.term_val + (int64_t(trial) * c.rows_cap + row) * c.cap;
-  __shared__ unsigned long long limbs[kLimbs * kBigTile];
+  __syncthreads();
+  double2 xo[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int e = threadIdx.x + r * kThreads;
+    const int col = e & (C - 1), n1 = e >> logC;
+    xo[r] = a[(col << G.logN1) + n1];
+  }
   double mx = 0.0;
   for (int i = threadIdx.x; i < cnt; i += kThreads) {
     const int col = (tb[i] & (G.N2 - 1)) - n2_0;
     if (col >= 0 && col < C) mx = fmax(mx, MaxComp(tv[i]));
   }
-  for (int k = threadIdx.x; k < kLimbs * kBigTile; k += kThreads) limbs[k] = 0ull;
-  const TermGrid tg = MakeTermGrid(mx);
-  if (tg.any)
-    for (int i = threadIdx.x; i < cnt; i += kThreads) {
-      const int bin = tb[i];
-      const int col = (bin & (G.N2 - 1)) - n2_0;
-      if (col >= 0 && col < C)
-        LimbAdd(limbs + kLimbs * ((col << G.logN1) + (bin >> G.logN2)), tv[i], tg);
-    }
-  __syncthreads();
+  const TermGrid tg = MakeTermGrid(mx);  // its barrier: every output is out of LDS
+  unsigned long long *limbs = reinterpret_cast<unsigned long long *>(a);
   double2 *A = c.A + (int64_t(trial) * c.rows_cap + row) * c.B_cap;
-  for (int e = threadIdx.x; e < kBigTile; e += kThreads) {
-    const int col = e & (C - 1), n1 = e >> logC;
-    const int m = (n1 << G.logN2) + n2_0 + col;
-    const int i = (col << G.logN1) + n1;
-    if (m < B) A[m] = tg.any ? Sub(a[i], LimbValue(limbs + kLimbs * i, tg)) : a[i];
+  for (int h = 0; h < 2; ++h) {
+    if (tg.any) {
+      for (int k = threadIdx.x; k < kLimbs * kHalf; k += kThreads) limbs[k] = 0ull;
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += kThreads) {
+        const int bin = tb[i];
+        const int col = (bin & (G.N2 - 1)) - n2_0;
+        const int li = (col << G.logN1) + (bin >> G.logN2) - h * kHalf;
+        if (col >= 0 && col < C && li >= 0 && li < kHalf) LimbAdd(limbs + kLimbs * li, tv[i], tg);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int e = threadIdx.x + r * kThreads;
+      const int col = e & (C - 1), n1 = e >> logC;
+      const int m = (n1 << G.logN2) + n2_0 + col;
+      const int li = (col << G.logN1) + n1 - h * kHalf;
+      if (m < B && li >= 0 && li < kHalf)
+        A[m] = tg.any ? Sub(xo[r], LimbValue(limbs + kLimbs * li, tg)) : xo[r];
+    }
+    if (tg.any) __syncthreads();  // the limbs are read before the next half clears them
   }
 }
 
@@ -1787,6 +1830,7 @@ __global__ void __launch_bounds__(kThreads)
   TileFold<Tx>(c, ip, trials, pb - kPrepBlocks, lds);
 }
 
+template <int kPaths>
 __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   HIP_DYNAMIC_SHARED(double2, lds)
   constexpr int T = kFftThreads;
@@ -1869,15 +1913,19 @@ __global__ void __launch_bounds__(kFftThreads) k_late_fft(Ctx c, int p) {
   for (int r = 0; r < kPerTerm; ++r) cfv[r] = kk[r] >= 0 ? c.coef[kk[r]] : make_double2(0.0, 0.0);
   double2 *a = lds;
   const bool small = ip.logM <= kFftSmallLogM;
-  if (small)
+  if constexpr (kPaths == kFftSmallOnly) {
     BinFftSmall<false>(c, ip, fold, rp, a);
-  else
-    BinFftLarge<false>(c, ip, fold, rp, a);
+  } else {
+    if (small)
+      BinFftSmall<false>(c, ip, fold, rp, a);
+    else
+      BinFftLarge<false>(c, ip, fold, rp, a);
+  }
   __syncthreads();
   // BinInFreq, as k_bin_fft's epilogue (exact per-bin sums of the terms):
   // the terms of the first modes are formed once, in registers, and the
   // rest (more than kPerTerm x T modes) are formed again for the second pass.
-  double2 xo[kOutPer];
+  double2 xo[OutPer<kPaths>()];
   TakeOutputs(a, small, B, xo);
   unsigned long long *limbs = reinterpret_cast<unsigned long long *>(a);
   auto term = [&](int i, int64_t k0, const double2 &cf, int32_t *bin) {
@@ -2046,10 +2094,11 @@ __global__ void __launch_bounds__(kThreads)
 // signal i occupy [off[i], off[i] + cnt[i]) of each launch's map, so a
 // launch's workgroups visit only units that exist this iteration (signals
 // that stopped, and the tiles of a B smaller than the largest, cost
-// nothing).  Units: k_bin_time: PrepUnits term blocks then trials x
-// ceil(B/R) tiles; k_bin_fft: trials x rows (trial, row) DFTs; k_identify:
-// ceil(B / ident_bins) blocks of ident_bins bins (ident_bins / kIdentWaves
-// per wavefront).
+// nothing).  Units (kinds 0..3): k_bin_time: PrepUnits term blocks then
+// trials x ceil(B/R) tiles; k_bin_fft: trials x rows (trial, row) DFTs, as
+// kind 1 on the small FFT path (M <= 1024) or kind 3 on the large one, each
+// path a kernel of its own; k_identify (kind 2): ceil(B / ident_bins) blocks
+// of ident_bins bins (ident_bins / kIdentWaves per wavefront).
 constexpr int kPlanThreads = 1024;
 
 // Workgroups computing BinInFreq terms for `terms` (modes x trials), a
@@ -2062,16 +2111,16 @@ __device__ __forceinline__ int32_t PrepUnits(int64_t terms) {
 __global__ void __launch_bounds__(kPlanThreads)
     k_batch_plan(const Ctx *ctxs, int32_t count, int p, BatchWork w, int32_t ident_bins,
                  int32_t *counters, int32_t *host) {
-  __shared__ int32_t part[3][kPlanThreads];
+  __shared__ int32_t part[kUnitKinds][kPlanThreads];
   // with at most kPlanThreads signals (one per thread): each signal's unit
   // counts, rows and rows_cap, for the map fill below
-  __shared__ int32_t s_cnt[3][kPlanThreads];
+  __shared__ int32_t s_cnt[kUnitKinds][kPlanThreads];
   __shared__ int32_t s_rows[2][kPlanThreads];
   const int tid = threadIdx.x;
   // signals [i0, i1) belong to this thread
   const int32_t per = (count + kPlanThreads - 1) / kPlanThreads;
   const int32_t i0 = min(count, tid * per), i1 = min(count, i0 + per);
-  int32_t sum[3] = {0, 0, 0};
+  int32_t sum[kUnitKinds] = {0, 0, 0, 0};
   for (int32_t i = i0; i < i1; ++i) {
     const Ctx &c = ctxs[i];
     // the iteration's parameters are built in place, in slot p
@@ -2079,16 +2128,16 @@ __global__ void __launch_bounds__(kPlanThreads)
     LoopCounters lc;
     NextIteration(c, p, &ip, &lc);
     PublishLoop(c, p, ip, lc);
-    int32_t u[3] = {0, 0, 0};
+    int32_t u[kUnitKinds] = {0, 0, 0, 0};
     if (!ip.done) {
       const int32_t trials = c.st->trials;
       // a multiple of 8 units per signal: every signal's units start on XCD 0
       u[0] = (PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
               trials * ((ip.B + ip.R - 1) / ip.R) * ip.slices + 7) & ~7;
-      u[1] = trials * ip.rows;
+      u[ip.logM <= kFftSmallLogM ? 1 : 3] = trials * ip.rows;
       u[2] = (ip.B + ident_bins - 1) / ident_bins;
     }
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kUnitKinds; ++k) {
       w.cnt[k * count + i] = u[k];
       sum[k] += u[k];
       if (per == 1) s_cnt[k][tid] = u[k];
@@ -2099,24 +2148,24 @@ __global__ void __launch_bounds__(kPlanThreads)
       s_rows[1][tid] = c.rows_cap;
     }
   }
-  for (int k = 0; k < 3; ++k) part[k][tid] = sum[k];
+  for (int k = 0; k < kUnitKinds; ++k) part[k][tid] = sum[k];
   __syncthreads();
   for (int d = 1; d < kPlanThreads; d <<= 1) {  // inclusive scan of the sums
-    int32_t v[3];
-    for (int k = 0; k < 3; ++k) v[k] = tid >= d ? part[k][tid - d] : 0;
+    int32_t v[kUnitKinds];
+    for (int k = 0; k < kUnitKinds; ++k) v[k] = tid >= d ? part[k][tid - d] : 0;
     __syncthreads();
-    for (int k = 0; k < 3; ++k) part[k][tid] += v[k];
+    for (int k = 0; k < kUnitKinds; ++k) part[k][tid] += v[k];
     __syncthreads();
   }
-  int32_t base[3];
-  for (int k = 0; k < 3; ++k) base[k] = part[k][tid] - sum[k];
+  int32_t base[kUnitKinds];
+  for (int k = 0; k < kUnitKinds; ++k) base[k] = part[k][tid] - sum[k];
   for (int32_t i = i0; i < i1; ++i)
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kUnitKinds; ++k) {
       w.off[k * count + i] = base[k];
       base[k] += w.cnt[k * count + i];
     }
   if (tid == kPlanThreads - 1)
-    for (int k = 0; k < 3; ++k) w.total[k] = part[k][tid];
+    for (int k = 0; k < kUnitKinds; ++k) w.total[k] = part[k][tid];
   // with more signals than threads, the counts and offsets above are read
   // below by other threads of this workgroup (device-scope loads)
   if (per > 1) __threadfence();
@@ -2133,16 +2182,16 @@ __global__ void __launch_bounds__(kPlanThreads)
   // the maps: one wavefront per signal
   const int lane = tid & 63;
   for (int32_t i = tid >> 6; i < count; i += kPlanThreads / 64) {
-    int32_t off[3], cnt[3], rows, rows_cap;
+    int32_t off[kUnitKinds], cnt[kUnitKinds], rows, rows_cap;
     if (per == 1) {  // signal i belonged to thread i: its counts and inclusive sums in LDS
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 0; k < kUnitKinds; ++k) {
         cnt[k] = s_cnt[k][i];
         off[k] = part[k][i] - cnt[k];
       }
       rows = s_rows[0][i];
       rows_cap = s_rows[1][i];
     } else {
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 0; k < kUnitKinds; ++k) {
         off[k] = __hip_atomic_load(&w.off[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         cnt[k] = __hip_atomic_load(&w.cnt[k * count + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -2150,10 +2199,11 @@ __global__ void __launch_bounds__(kPlanThreads)
       rows_cap = ctxs[i].rows_cap;
     }
     for (int32_t u = lane; u < cnt[0]; u += 64) w.map[0][off[0] + u] = make_int2(i, u);
-    for (int32_t u = lane; u < cnt[1]; u += 64) {
-      const int32_t trial = u / rows;
-      w.map[1][off[1] + u] = make_int2(i, trial * rows_cap + (u - trial * rows));
-    }
+    for (int k = 1; k < kUnitKinds; k += 2)  // the bin-FFT units, small or large path
+      for (int32_t u = lane; u < cnt[k]; u += 64) {
+        const int32_t trial = u / rows;
+        w.map[k][off[k] + u] = make_int2(i, trial * rows_cap + (u - trial * rows));
+      }
     for (int32_t u = lane; u < cnt[2]; u += 64) w.map[2][off[2] + u] = make_int2(i, u);
   }
 }
@@ -2184,14 +2234,15 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+template <int kPaths>
 __global__ void __launch_bounds__(kFftThreads)
     k_bin_fft_batch(const Ctx *ctxs, const int2 *map, const int32_t *total, int p) {
   HIP_DYNAMIC_SHARED(double2, lds)
-  const int32_t tot = total[1];
+  const int32_t tot = total[kPaths == kFftSmallOnly ? 1 : 3];
   for (int32_t u = blockIdx.x; u < tot; u += gridDim.x) {
     if (u != int32_t(blockIdx.x)) __syncthreads();
     const int2 m = map[u];
-    BinFftBody(ctxs[m.x], p, m.y, lds);
+    BinFftBody<kPaths>(ctxs[m.x], p, m.y, lds);
   }
 }
 
@@ -2732,7 +2783,7 @@ hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s) {
 }
 
 hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32_t tiles,
-                               size_t lds_time, size_t lds_fft, int with_identify,
+                               size_t lds_time, size_t lds_fft, int with_identify, int small_fft,
                                hipStream_t s) {
   if (c.mixed)
     hipLaunchKernelGGL(k_late_fold<float2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
@@ -2740,8 +2791,13 @@ hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32
   else
     hipLaunchKernelGGL(k_late_fold<double2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
                        lds_time, s, c, parity, nb_ident, with_identify);
-  hipLaunchKernelGGL(k_late_fft, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads), lds_fft, s, c,
-                     parity);
+  // the late B's own bin-FFT path: the small one alone where it applies
+  if (small_fft)
+    hipLaunchKernelGGL(k_late_fft<kFftSmallOnly>, dim3(c.trials_cap * c.rows_cap),
+                       dim3(kFftThreads), lds_fft, s, c, parity);
+  else
+    hipLaunchKernelGGL(k_late_fft<kFftBoth>, dim3(c.trials_cap * c.rows_cap), dim3(kFftThreads),
+                       lds_fft, s, c, parity);
   return hipGetLastError();
 }
 
@@ -2812,9 +2868,11 @@ hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStre
         hipLaunchKernelGGL(k_bin_time_batch<double2>, dim3(b.grid_time), dim3(kThreads),
                            b.lds_time, s, b.ctxs, w.map[0], w.total, parity);
       break;
-    case 2:
-      hipLaunchKernelGGL(k_bin_fft_batch, dim3(b.grid_fft), dim3(kFftThreads), b.lds_fft, s,
-                         b.ctxs, w.map[1], w.total, parity);
+    case 2:  // one kernel per bin-FFT path, each over its own units
+      hipLaunchKernelGGL(k_bin_fft_batch<kFftSmallOnly>, dim3(b.grid_fft), dim3(kFftThreads),
+                         b.lds_fft_small, s, b.ctxs, w.map[1], w.total, parity);
+      hipLaunchKernelGGL(k_bin_fft_batch<kFftLargeOnly>, dim3(b.grid_fft), dim3(kFftThreads),
+                         b.lds_fft, s, b.ctxs, w.map[3], w.total, parity);
       break;
     default:
       hipLaunchKernelGGL(k_identify_batch, dim3(b.grid_ident), dim3(kThreads), 0, s, b.ctxs,
