@@ -179,7 +179,8 @@ def batch_workload(M, steps, warmup, count=256):
     uniform in [16, 22], S uniform in [10, 500], unit coefficients with
     uniform phases, sigma = 1e-6, synthesised on the GPU.  Solves/s of one
     batched solve (tile width from the autotuner) against a serial loop of
-    single-signal solves of the same signals."""
+    single-signal solves of the same signals; default_ms is the same batched
+    solve before autotuning (default tile widths, 16 bins per identify unit)."""
     import torch
     from mpsft_amd.signals import make_signal_device
     rng = np.random.default_rng(4)
@@ -196,16 +197,22 @@ def batch_workload(M, steps, warmup, count=256):
         opts.append(M.solve_options(k, float(np.hypot(sigma, sq)), window_delta=DELTA))
     torch.cuda.synchronize()
     b = M.Batch(ns, opts, M.MPSFT_MIXED)
+
+    def timed():
+        for _ in range(warmup):
+            b.solve(xs, seeds)
+        M.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = b.solve(xs, seeds)
+        M.synchronize()
+        return (time.perf_counter() - t0) / steps, r
+
+    log("batched solves at the default tile widths and identify bins")
+    t_default, _ = timed()
     log("autotuning the batch")
     tile = b.autotune(xs, seeds)
-    for _ in range(warmup):
-        b.solve(xs, seeds)
-    M.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        iters, stats = b.solve(xs, seeds)
-    M.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
+    t_batch, (iters, stats) = timed()
     log("serial loop of single-signal solves")
     solvers = [M.Solver(n, o, M.MPSFT_MIXED) for n, o in zip(ns, opts)]
     for s_, x, sd in zip(solvers, xs, seeds):  # warm: tables, graphs
@@ -224,6 +231,7 @@ def batch_workload(M, steps, warmup, count=256):
                     "sigma=1e-6, MIXED, one batched solve",
         "signals": count, "total_samples": int(sum(ns)), "total_modes": int(sum(ks)),
         "batched_ms": round(t_batch * 1e3, 3), "batched_solves_per_sec": round(count / t_batch, 1),
+        "default_ms": round(t_default * 1e3, 3),
         "serial_ms": round(t_serial * 1e3, 3), "serial_solves_per_sec": round(count / t_serial, 1),
         "speedup_vs_serial": round(t_serial / t_batch, 2), "tile_residues": tile,
         "max_iterations": int(max(iters)), "status_ok": int(sum(1 for v in stats if v == 0)),

@@ -330,8 +330,10 @@ int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32
 int mpsft_batch_solve_profiled(mpsft_batch *batch, const void *const *x_dev,
                                const uint32_t *seeds, int32_t *iters, int32_t *status,
                                double *kernel_ms, int32_t *launches, void *stream);
-/* Times short batched solves of these signals at tile widths 1, 2, 4, 8
- * and keeps the fastest for later solves; *residues gets it. */
+/* Times short batched solves of these signals at every gather-fold tile
+ * width (0: the default width per B, or 1, 2, 4, 8 residues) and k_identify
+ * bins per unit (8, 16, 32), and keeps the fastest pair for later solves;
+ * *residues gets its tile width.  Results do not depend on either. */
 int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
                          int32_t *residues, void *stream);
 /* Results of signal i of the last batched solve, as the solver calls. */

@@ -68,6 +68,7 @@ Batch::Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, i
   }
   args_.resize(count);
   built_gen_.assign(count, -1);
+  ident_bins_ = EnvInt("MPSFT_IDENT_BINS", 16, 4, kIdentKeys) / 4 * 4;
   int32_t *hp = nullptr;
   if (AllocDev(&d_ctxs_, count) != MPSFT_OK || AllocDev(&d_args_, count) != MPSFT_OK ||
       AllocDev(&d_counters_, 2) != MPSFT_OK || AllocDev(&d_results_, count) != MPSFT_OK ||
@@ -131,9 +132,10 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
 // Block maps of the three batched launches and the graph of two batched
 // iterations; redone only when an engine's shapes changed.
 int Batch::Build(hipStream_t s) {
-  bool same = graph_ != nullptr;
+  bool same = graph_ != nullptr && !graph_stale_;
   for (size_t i = 0; i < eng_.size() && same; ++i) same = built_gen_[i] == eng_[i]->shape_gen_;
   if (same) return MPSFT_OK;
+  graph_stale_ = false;
   std::vector<int2> map_time, map_fft, map_ident;
   std::vector<int32_t> nblk_ident(eng_.size());
   size_t lds_time = 0, lds_fft = 0, lds_fft_small = 0;
@@ -187,7 +189,7 @@ int Batch::Build(hipStream_t s) {
   // The packed launches: at most kPackedGrid workgroups each (8 per CU),
   // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
   const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 8192, 64, 1 << 20);
-  bl_.ident_bins = EnvInt("MPSFT_IDENT_BINS", 16, 4, kIdentKeys) / 4 * 4;
+  bl_.ident_bins = ident_bins_;
   bl_.grid_time = std::max(8, std::min(bl_.nb_time, grid) & ~7);  // a multiple of 8
   bl_.grid_fft = std::max(1, std::min(bl_.nb_fft, grid));
   bl_.grid_ident = std::max(1, std::min(bl_.nb_ident, grid));
@@ -498,37 +500,51 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
 int Batch::Autotune(const void *const *x, const uint32_t *seeds, int32_t *chosen,
                     hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
-  // A short solve (3 iterations per signal) per candidate width, timed with
-  // events on the launch stream; the first run of each also builds tables.
+  // Short solves (3 iterations per signal: the large-B iterations, where the
+  // split of the work matters most), timed with events on the launch stream,
+  // for every gather-fold tile width (0: the default width per B, or 1, 2,
+  // 4, 8 residues for every B) and k_identify bins per unit (8, 16, 32); the
+  // first run of each also builds tables and graphs.  Neither changes a
+  // result, only how the work is split.
   std::vector<mpsft_solve_options> saved = opts_;
   for (auto &o : opts_) o.max_iter = 3;
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
-  int best_R = 0, rc = MPSFT_OK;
+  const int32_t kWidths[] = {0, 1, 2, 4, 8}, kBins[] = {8, 16, 32};
+  const int32_t saved_bins = ident_bins_;
+  int best_R = -1, best_bins = saved_bins, rc = MPSFT_OK;
   float best_ms = 0;
-  for (int R = 1; R <= 8 && rc == MPSFT_OK; R *= 2) {
+  for (const int32_t R : kWidths) {
     for (auto &e : eng_)
       if ((rc = e->SetTileResidues(R)) != MPSFT_OK) break;
+    for (const int32_t nb : kBins) {
+      if (rc != MPSFT_OK) break;
+      ident_bins_ = nb;
+      graph_stale_ = true;  // the launch parameters changed: capture again
+      float ms = 0;
+      for (int rep = 0; rep < 2 && rc == MPSFT_OK; ++rep) {  // first run warms up
+        if (hipEventRecord(e0, s) != hipSuccess) rc = MPSFT_ERR_HIP;
+        const int r = Solve(x, seeds, nullptr, nullptr, s);
+        if (r != MPSFT_OK && r != MPSFT_ERR_NOT_CONVERGED) rc = r;
+        if (rc == MPSFT_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                               hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = MPSFT_ERR_HIP;
+      }
+      if (rc == MPSFT_OK && (best_R < 0 || ms < best_ms)) {
+        best_R = R;
+        best_bins = nb;
+        best_ms = ms;
+      }
+    }
     if (rc != MPSFT_OK) break;
-    float ms = 0;
-    for (int rep = 0; rep < 2 && rc == MPSFT_OK; ++rep) {  // first run warms up
-      if (hipEventRecord(e0, s) != hipSuccess) rc = MPSFT_ERR_HIP;
-      const int r = Solve(x, seeds, nullptr, nullptr, s);
-      if (r != MPSFT_OK && r != MPSFT_ERR_NOT_CONVERGED) rc = r;
-      if (rc == MPSFT_OK && (hipEventRecord(e1, s) != hipSuccess ||
-                             hipEventSynchronize(e1) != hipSuccess ||
-                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        rc = MPSFT_ERR_HIP;
-    }
-    if (rc == MPSFT_OK && (best_R == 0 || ms < best_ms)) {
-      best_R = R;
-      best_ms = ms;
-    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   opts_ = saved;
+  ident_bins_ = rc == MPSFT_OK ? best_bins : saved_bins;
+  graph_stale_ = true;
   if (rc != MPSFT_OK) return rc;
   for (auto &e : eng_)
     if ((rc = e->SetTileResidues(best_R)) != MPSFT_OK) return rc;

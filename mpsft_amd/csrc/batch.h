@@ -80,6 +80,8 @@ class Batch {
   int32_t *d_host_ = nullptr;
   BatchLaunch bl_{};
   hipGraphExec_t graph_ = nullptr;
+  bool graph_stale_ = false;   // launch parameters changed since the graph's capture
+  int32_t ident_bins_ = 16;    // k_identify bins per unit (autotuned)
   AdjSig *d_adj_sig_ = nullptr;
   int2 *d_adj_map_fft_ = nullptr;
   int32_t adj_nb_fft_ = 0;
