@@ -332,6 +332,7 @@ Engine::~Engine() {
   (void)hipFree(d_lam_);
   (void)hipFree(d_fbar_);
   (void)hipFree(d_taps_);
+  (void)hipFree(d_lags_);
   (void)hipFree(d_ctx1_);
   (void)hipFree(d_sig1_);
   (void)hipFree(d_steps1_);
@@ -1110,6 +1111,19 @@ int Engine::EnsureAdjBuffers(int32_t iters) {
     }
     taps_len_ = twant;
   }
+  // The per-lag contributions k_adj_lags tabulates for the gather (n a
+  // power of two <= 2^30 only; the general gather computes them per sample).
+  if (mask_ >= 0 && n_ <= (int64_t(1) << 30)) {
+    lags_stride_ = int64_t(rows_cap_ + 2) * stride;
+    const size_t lwant = size_t(std::max(iters, 1)) * trials_cap_ * size_t(lags_stride_);
+    if (lwant > lags_len_) {
+      if (Alloc(&d_lags_, lwant) != MPSFT_OK) {
+        lags_len_ = 0;
+        return MPSFT_ERR_NOMEM;
+      }
+      lags_len_ = lwant;
+    }
+  }
   return MPSFT_OK;
 }
 
@@ -1136,6 +1150,8 @@ int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out,
   g.fbar = d_fbar_;
   g.taps = d_taps_;
   g.taps_stride = taps_stride_;
+  g.lags = lags_len_ ? d_lags_ : nullptr;
+  g.lags_stride = lags_stride_;
   g.rec = d_rec_;
   g.keys = keys;
   g.grad = static_cast<const double2 *>(grad);
@@ -1157,6 +1173,9 @@ int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out,
   a.lds_fft = LdsFft();
   a.per_small = 16;
   a.per_gather = int32_t(std::min<int64_t>(4096, (n_ + 1023) / 1024));
+  a.per_lags = LagBlocks(lags_stride_);
+  a.any_tab = g.lags ? 1 : 0;
+  a.any_gen = g.lags ? 0 : 1;
   a.mixed = precision == MPSFT_MIXED ? 1 : 0;
   HIP_OK(LaunchAdj(a, s));
   // c and g are read by the copies above; the output is the caller's next read

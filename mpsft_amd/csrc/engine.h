@@ -379,9 +379,10 @@ class Engine {
   int64_t prep_gen_ = -1, tab_epoch_ = 0, prep_epoch_ = -1;
   uint32_t rec_seed_ = 0;
   mpsft_solve_options rec_opt_{};
-  double2 *d_lam_ = nullptr, *d_fbar_ = nullptr, *d_taps_ = nullptr;
-  size_t fbar_len_ = 0, taps_len_ = 0;
+  double2 *d_lam_ = nullptr, *d_fbar_ = nullptr, *d_taps_ = nullptr, *d_lags_ = nullptr;
+  size_t fbar_len_ = 0, taps_len_ = 0, lags_len_ = 0;
   int64_t taps_stride_ = 0;  // max f B over the built tables
+  int64_t lags_stride_ = 0;  // (rows_cap + 2) taps_stride: one (iteration, trial)'s lag table
   // one-signal launch state of Adjoint (device): Ctx, AdjSig, steps
   Ctx *d_ctx1_ = nullptr;
   AdjSig *d_sig1_ = nullptr;

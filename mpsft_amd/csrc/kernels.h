@@ -98,6 +98,8 @@ struct AdjSig {            // one signal (device memory, per call)
   double2 *fbar;           // iters x trials_cap x rows_cap x B_cap: fold gradients
   double2 *taps;           // iters x taps_stride: each iteration's conjugated taps (f B)
   int64_t taps_stride;
+  double2 *lags;           // iters x trials_cap x lags_stride: per-lag contributions
+  int64_t lags_stride;     // (k_adj_lags); null: the gather computes them per sample
   const int64_t *rec;      // recorded k1 rows, Ctx.B_cap apart
   const int64_t *keys;     // m seed keys and their gradients
   const double2 *grad;
@@ -122,9 +124,17 @@ struct AdjLaunch {
                              // blocks, then trials_cap x rows_cap DFT blocks
   size_t lds_fft;
   int32_t per_small, per_gather;  // blocks per signal: clear / seed, gather
+  int32_t per_lags;               // blocks per (signal, step) of k_adj_lags
+  int any_tab, any_gen;           // some signal has lag tables / some has none
   int mixed;                      // output precision
 };
 hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s);
+// Blocks per (signal, step) of k_adj_lags (one lag slot of B residues per
+// block at a time) for lag tables of up to `stride` entries.
+inline int32_t LagBlocks(int64_t stride) {
+  const int64_t b = (stride + 4 * 256 - 1) / (4 * 256);
+  return int32_t(b < 1 ? 1 : (b > 64 ? 64 : b));
+}
 hipError_t LaunchSetModes(const Ctx &c, const int64_t *keys, const double2 *coefs,
                           int64_t m, hipStream_t s);
 hipError_t LaunchGetCoefs(const Ctx &c, double2 *out, int64_t m, hipStream_t s);

@@ -2482,18 +2482,102 @@ __device__ __forceinline__ double2 AdjLagTerms32(int32_t f, int32_t bits, int32_
   return acc;
 }
 
-// AdjGatherBody for n = 2^k <= 2^30: the products mod n are the low bits
-// of 32-bit products, and every lag range fits int32.
+// The lags of one recorded iteration, in sample offsets w = lam B + rho
+// from the trial's base (lam the lag, rho the residue), for n = 2^k <=
+// 2^30.  AdjLagTerms32 is non-zero only in segment 0 (lags [-s0, s0 + f),
+// s0 = 2^(K-1)) and in the windows [2^b, 2^b + f) and [-2^b, -2^b + f) of
+// the rows b in [K, bits) outside it.  k_adj_lags tabulates AdjLagTerms32
+// over exactly those offsets, per (iteration, trial):
+//   segment 0, w in [-s0 B, (s0 + f) B):  index w + s0 B
+//   row +2^b, w = 2^b B + off, off < f B: index pos0 + b f B + off
+//   row -2^b, w = -2^b B + off:          index neg0 + b f B + off
+// (E entries in all, E <= (rows_cap + 2) f B since 2 s0 < 2 f and
+// 2 bits + 1 <= rows_cap), and the gather looks a sample's offsets up
+// there instead of recomputing the lag sums.
+struct LagGeom {
+  int32_t B, f, bits, K, lb;        // lb = floor(log2 B)
+  uint32_t sB, seg_end, fB;         // s0 B, (s0 + f) B, f B
+  int32_t pos0, neg0, E;            // table layout
+  int32_t top;                      // hi - n: offsets v < top also stand for w = v + n
+};
+
+__device__ __forceinline__ LagGeom MakeLagGeom(const IterParams &ip, int64_t n) {
+  LagGeom L;
+  L.B = ip.B;
+  L.f = ip.f;
+  L.bits = ip.bits;
+  L.K = 0;  // the smallest power with 2^K >= f, at most bits
+  while (L.K < L.bits && (1 << L.K) < L.f) ++L.K;
+  L.lb = 31 - __clz(L.B);
+  const int32_t s0 = L.K ? (1 << (L.K - 1)) : 0;
+  L.sB = uint32_t(s0) * uint32_t(L.B);
+  L.fB = uint32_t(L.f) * uint32_t(L.B);
+  L.seg_end = L.sB + L.fB;
+  const int32_t L0 = int32_t(2 * L.sB + L.fB);
+  L.pos0 = L0 - L.K * int32_t(L.fB);
+  L.neg0 = L0 + (L.bits - 2 * L.K) * int32_t(L.fB);
+  L.E = L0 + 2 * (L.bits - L.K) * int32_t(L.fB);
+  L.top = int32_t(int64_t((1 << (L.bits - 1)) + L.f) * L.B - n);
+  return L;
+}
+
+// Table index of offset w = v >= 0 (v < n + f B < 2^31), or -1 outside
+// every window.  Outside segment 0 the window is row +2^b with
+// 2^b B <= v < 2^(b+1) B (b = floor(log2 lam)), found from the leading bit
+// of v.  Selects only, no branches: the lanes of a wavefront take
+// different cases.
+__device__ __forceinline__ int32_t PosRowIndex(const LagGeom &L, uint32_t v) {
+  int32_t b = max(31 - __clz(int32_t(v | 1u)) - L.lb, 0);
+  const uint32_t t0 = uint32_t(L.B) << b;
+  const int32_t below = v < t0 ? 1 : 0;
+  b -= below;  // >= K outside segment 0 (v >= (s0 + f) B > 2^K B)
+  const uint32_t off = v - (t0 >> below);
+  return ((b < L.bits) & (off < L.fB)) ? L.pos0 + b * int32_t(L.fB) + int32_t(off) : -1;
+}
+__device__ __forceinline__ int32_t PosLagIndex(const LagGeom &L, uint32_t v) {
+  return v < L.seg_end ? int32_t(v + L.sB) : PosRowIndex(L, v);
+}
+
+// Table index of offset w = -z (0 < z <= n), or -1.  Outside segment 0
+// the window is row -2^b with 2^(b-1) B < z <= 2^b B (b = ceil(log2 -lam)).
+__device__ __forceinline__ int32_t NegLagIndex(const LagGeom &L, uint32_t z) {
+  const uint32_t zp = z - 1;
+  int32_t d = max(31 - __clz(int32_t(zp | 1u)) - L.lb, 0);
+  d -= zp < (uint32_t(L.B) << d) ? 1 : 0;  // zp < B: d = -1
+  const int32_t b = d + 1;                  // B << b <= 2 zp < 2^31 when b >= 1
+  const uint32_t off = (uint32_t(L.B) << min(b, 30)) - z;
+  const int32_t row =
+      ((b < L.bits) & (off < L.fB)) ? L.neg0 + b * int32_t(L.fB) + int32_t(off) : -1;
+  return z <= L.sB ? int32_t(L.sB - z) : row;
+}
+
+// The table is E / B slots of B residues: slot sigma holds lag lam (offsets
+// w = lam B + rho, rho < B): lags -s0 .. s0 + f - 1 (segment 0), then
+// 2^b + k and -2^b + k (k < f) for the rows b = K .. bits - 1.
+__device__ __forceinline__ int32_t SlotLag(const LagGeom &L, int32_t sigma) {
+  const int32_t s0 = L.K ? (1 << (L.K - 1)) : 0;
+  const int32_t n0 = 2 * s0 + L.f;
+  if (sigma < n0) return sigma - s0;
+  const int32_t r = (sigma - n0) / L.f, k = (sigma - n0) - r * L.f;
+  const int32_t rows = L.bits - L.K;
+  return r < rows ? (1 << (L.K + r)) + k : k - (1 << (L.K + r - rows));
+}
+
+// AdjGatherBody for n = 2^k <= 2^30 with the lag tables of k_adj_lags: per
+// sample and recorded (iteration, trial), the offset v = u - base of its
+// permuted index u (the low bits of 32-bit products) is looked up as
+// w = v - n, w = v and, for v < top, w = v + n: every representative of v
+// in the lag range [lo, hi) = [-2^(bits-1) B, (2^(bits-1) + f) B), which
+// lies in [-n, n + f B) since 2^(bits-1) B <= n (bits = 1 + floor(log2(n /
+// B))), in increasing order.  top <= f B <= (s0 + f) B, so for v < top the
+// offset w = v is in segment 0 and only w = v + n needs the general lookup.
 template <typename Tout>
 __device__ __forceinline__ void AdjGatherBody32(const Ctx &c, const AdjSig &g, const AdjStep *steps,
                                                 int32_t count, int32_t si, Tout *out, int64_t tid,
                                                 int64_t gsize) {
-  const int32_t n = int32_t(c.n);
-  const uint32_t m32 = uint32_t(c.n - 1);
-  const int32_t rstride = c.B_cap;
-  const int64_t tstride = int64_t(c.rows_cap) * c.B_cap;
-  const int64_t istride = int64_t(c.trials_cap) * tstride;
-  for (int64_t j0 = tid; j0 < n; j0 += gsize * kGatherJ) {
+  const uint32_t n = uint32_t(c.n);
+  const uint32_t m32 = n - 1;
+  for (int64_t j0 = tid; j0 < int64_t(n); j0 += gsize * kGatherJ) {
     double2 acc[kGatherJ];
 #pragma unroll
     for (int q = 0; q < kGatherJ; ++q) acc[q] = make_double2(0.0, 0.0);
@@ -2501,46 +2585,43 @@ __device__ __forceinline__ void AdjGatherBody32(const Ctx &c, const AdjSig &g, c
       const AdjStep &st = steps[int64_t(s) * count + si];
       if (!st.active) continue;
       const IterParams &ip = st.ip;
-      const int32_t B = ip.B, f = ip.f, bits = ip.bits;
-      const int32_t h = (B - 1) / 2;
-      const double invB = 1.0 / double(B);
-      const int32_t half = 1 << (bits - 1);
-      const int32_t lo = -half * B, hi = (half + f) * B;  // lag range times B
-      int32_t K = 0;  // as Lags: the smallest power with 2^K >= f, at most bits
-      while (K < bits && (1 << K) < f) ++K;
-      const double2 *taps = g.taps + int64_t(st.it) * g.taps_stride;
+      const LagGeom L = MakeLagGeom(ip, n);
       const uint32_t a_inv = uint32_t(ip.tf.a_inv), cc = uint32_t(ip.tf.c);
       for (int32_t t = 0; t < st.trials; ++t) {
         const uint32_t base = uint32_t(ip.q[t] - ip.p2) & m32;
-        const double2 *fb = g.fbar + int64_t(st.it) * istride + t * tstride;
+        const double2 *tab = g.lags + (int64_t(st.it) * c.trials_cap + t) * g.lags_stride;
+        // every lookup of the kGatherJ samples is issued before any is
+        // added; only lanes with a window load (a miss adds +0, which
+        // leaves a sum that started at +0 unchanged)
+        int32_t idx[kGatherJ][3];
+        double2 val[kGatherJ][3];
 #pragma unroll
         for (int q = 0; q < kGatherJ; ++q) {
           const int64_t j = j0 + q * gsize;
-          if (j >= n) continue;
           const uint32_t u = (a_inv * ((uint32_t(j) - cc) & m32)) & m32;
-          const int32_t v = int32_t((u - base) & m32);
-          for (int32_t w = v >= hi ? v - n : (v - n < lo ? v : v - n); w < hi; w += n) {
-            // lam B + rho = w, w = v mod n, lo <= w < hi
-            int32_t lam = int32_t(floor(double(w) * invB));
-            int32_t rho = w - lam * B;
-            if (rho < 0) {
-              --lam;
-              rho += B;
-            } else if (rho >= B) {
-              ++lam;
-              rho -= B;
-            }
-            const int32_t m = rho >= h ? rho - h : rho - h + B;
-            acc[q] = Add(acc[q], AdjLagTerms32(f, bits, K, B, lam, taps + rho, fb + m, rstride));
-            if (w > hi - n) break;  // w + n >= hi
+          const uint32_t v = (u - base) & m32;
+          const bool in = j < int64_t(n), wrap = int32_t(v) < L.top;
+          const uint32_t w = wrap ? v + n : v;
+          const int32_t p = w < L.seg_end ? int32_t(w + L.sB) : PosRowIndex(L, w);
+          idx[q][0] = in ? NegLagIndex(L, n - v) : -1;
+          idx[q][1] = !in ? -1 : (wrap ? int32_t(v + L.sB) : p);
+          idx[q][2] = (in & wrap) ? p : -1;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            val[q][r] = make_double2(0.0, 0.0);
+            if (idx[q][r] >= 0) val[q][r] = LoadX<false>(tab + idx[q][r]);
           }
         }
+#pragma unroll
+        for (int q = 0; q < kGatherJ; ++q)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) acc[q] = Add(acc[q], val[q][r]);
       }
     }
 #pragma unroll
     for (int q = 0; q < kGatherJ; ++q) {
       const int64_t j = j0 + q * gsize;
-      if (j < n) {
+      if (j < int64_t(n)) {
         out[j].x = acc[q].x;
         out[j].y = acc[q].y;
       }
@@ -2709,20 +2790,66 @@ __global__ void __launch_bounds__(kFftThreads)
   AdjFftBody(c, st.ip, st.trials, g.fbar + int64_t(st.it) * istride, m.y - kPrepBlocks, lds);
 }
 
+// The lag tables of every (signal, step, trial) with a table (AdjSig.lags):
+// `per` blocks per (step, signal), in the steps array's order.
+__global__ void __launch_bounds__(kThreads)
+    k_adj_lags(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, int32_t count,
+               int32_t max_steps, int32_t per) {
+  const int64_t unit = blockIdx.x / per;
+  if (unit >= int64_t(count) * max_steps) return;
+  const AdjStep &st = steps[unit];
+  const int32_t si = int32_t(unit % count);
+  const AdjSig &g = sig[si];
+  if (!st.active || !g.lags) return;
+  const Ctx &c = ctxs[si];
+  const LagGeom L = MakeLagGeom(st.ip, c.n);
+  if (L.E > g.lags_stride) return;  // cannot happen: E <= (rows_cap + 2) f B
+  const int64_t tstride = int64_t(c.rows_cap) * c.B_cap;
+  const int64_t istride = int64_t(c.trials_cap) * tstride;
+  const double2 *taps = g.taps + int64_t(st.it) * g.taps_stride;
+  const int32_t B = L.B, h = (B - 1) / 2, slots = L.E / B;
+  for (int32_t t = 0; t < st.trials; ++t) {
+    const double2 *fb = g.fbar + int64_t(st.it) * istride + t * tstride;
+    double2 *tab = g.lags + (int64_t(st.it) * c.trials_cap + t) * g.lags_stride;
+    // one slot per wavefront at a time, its residues over the lanes
+    const int32_t waves = int32_t(blockDim.x >> 6), lane = int32_t(threadIdx.x & 63);
+    for (int32_t sigma = int32_t(blockIdx.x % per) * waves + int32_t(threadIdx.x >> 6);
+         sigma < slots; sigma += per * waves) {
+      const int32_t lam = SlotLag(L, sigma);  // the same for the whole wavefront
+      for (int32_t rho = lane; rho < B; rho += 64) {
+        const int32_t m = rho >= h ? rho - h : rho - h + B;
+        tab[int64_t(sigma) * B + rho] =
+            AdjLagTerms32(L.f, L.bits, L.K, B, lam, taps + rho, fb + m, c.B_cap);
+      }
+    }
+  }
+}
+
+// The gather of every signal with lag tables (k_adj_gather_tab) and of
+// every other signal (k_adj_gather): two kernels, so that the table path's
+// register use is its own.
+template <typename Tout>
+__global__ void __launch_bounds__(kThreads)
+    k_adj_gather_tab(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, int32_t count,
+                     int32_t per) {
+  const int32_t si = blockIdx.x / per;
+  const AdjSig &g = sig[si];
+  if (!g.lags) return;
+  const int64_t tid = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x;
+  AdjGatherBody32(ctxs[si], g, steps, count, si, static_cast<Tout *>(g.out), tid,
+                  int64_t(per) * blockDim.x);
+}
+
 template <typename Tout>
 __global__ void __launch_bounds__(kThreads)
     k_adj_gather(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, int32_t count,
                  int32_t per) {
   const int32_t si = blockIdx.x / per;
   const AdjSig &g = sig[si];
-  const Ctx &c = ctxs[si];
+  if (g.lags) return;
   const int64_t tid = int64_t(blockIdx.x % per) * blockDim.x + threadIdx.x;
-  if (c.mask >= 0 && c.n <= (int64_t(1) << 30))
-    AdjGatherBody32(c, g, steps, count, si, static_cast<Tout *>(g.out), tid,
-                    int64_t(per) * blockDim.x);
-  else
-    AdjGatherBody(c, g, steps, count, si, static_cast<Tout *>(g.out), tid,
-                  int64_t(per) * blockDim.x);
+  AdjGatherBody(ctxs[si], g, steps, count, si, static_cast<Tout *>(g.out), tid,
+                int64_t(per) * blockDim.x);
 }
 
 int32_t Blocks(int64_t work) {
@@ -2935,12 +3062,22 @@ hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_fft_terms, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
                        b.sig, st, b.map_fft);
   }
-  if (b.mixed)
-    hipLaunchKernelGGL(k_adj_gather<float2>, dim3(b.count * b.per_gather), dim3(kThreads), 0, s,
-                       b.ctxs, b.sig, b.steps, b.count, b.per_gather);
-  else
-    hipLaunchKernelGGL(k_adj_gather<double2>, dim3(b.count * b.per_gather), dim3(kThreads), 0, s,
-                       b.ctxs, b.sig, b.steps, b.count, b.per_gather);
+  if (b.any_tab)
+    hipLaunchKernelGGL(k_adj_lags, dim3(b.count * b.max_steps * b.per_lags), dim3(kThreads), 0,
+                       s, b.ctxs, b.sig, b.steps, b.count, b.max_steps, b.per_lags);
+  const dim3 grid(b.count * b.per_gather);
+  if (b.any_tab && b.mixed)
+    hipLaunchKernelGGL(k_adj_gather_tab<float2>, grid, dim3(kThreads), 0, s, b.ctxs, b.sig,
+                       b.steps, b.count, b.per_gather);
+  if (b.any_tab && !b.mixed)
+    hipLaunchKernelGGL(k_adj_gather_tab<double2>, grid, dim3(kThreads), 0, s, b.ctxs, b.sig,
+                       b.steps, b.count, b.per_gather);
+  if (b.any_gen && b.mixed)
+    hipLaunchKernelGGL(k_adj_gather<float2>, grid, dim3(kThreads), 0, s, b.ctxs, b.sig, b.steps,
+                       b.count, b.per_gather);
+  if (b.any_gen && !b.mixed)
+    hipLaunchKernelGGL(k_adj_gather<double2>, grid, dim3(kThreads), 0, s, b.ctxs, b.sig, b.steps,
+                       b.count, b.per_gather);
   return hipGetLastError();
 }
 
