@@ -190,6 +190,9 @@ int Batch::Build(hipStream_t s) {
   // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
   const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 8192, 64, 1 << 20);
   bl_.ident_bins = ident_bins_;
+  int64_t trials_rows = 1;
+  for (auto &e : eng_) trials_rows = std::max<int64_t>(trials_rows, int64_t(e->trials_cap_) * e->rows_cap_);
+  bl_.col_slices = trials_rows <= 64 ? 1 : kColSlices;
   bl_.grid_time = std::max(8, std::min(bl_.nb_time, grid) & ~7);  // a multiple of 8
   bl_.grid_fft = std::max(1, std::min(bl_.nb_fft, grid));
   bl_.grid_ident = std::max(1, std::min(bl_.nb_ident, grid));

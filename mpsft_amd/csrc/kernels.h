@@ -63,6 +63,7 @@ struct BatchLaunch {
   int32_t nb_time, nb_fft, nb_ident;    // blocks of the largest iteration of every signal
   int32_t grid_time, grid_fft, grid_ident;  // workgroups of the packed launches
   int32_t ident_bins;                   // bins per k_identify unit (a multiple of 4)
+  int32_t col_slices;                   // k_identify column slices (1: trials x rows <= 64)
   int32_t count;                        // signals
   BatchWork work;
   size_t lds_time, lds_fft;             // dynamic LDS (max over signals)

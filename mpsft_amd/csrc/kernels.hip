@@ -1553,10 +1553,11 @@ __device__ __forceinline__ double WaveMin(double v) {
 
 // Entry `src` of a column held as col[s] = entry (lane + 64 s).  Every lane
 // of the wave must call it (the shuffles read other lanes' registers).
-__device__ __forceinline__ double2 ColFetch(const double2 (&col)[kColSlices], int src) {
+template <int NS>
+__device__ __forceinline__ double2 ColFetch(const double2 (&col)[NS], int src) {
   double2 r = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int s = 0; s < kColSlices; ++s) {
+  for (int s = 0; s < NS; ++s) {
     const double x = __shfl(col[s].x, src & 63);
     const double y = __shfl(col[s].y, src & 63);
     if ((src >> 6) == s) r = make_double2(x, y);
@@ -1566,6 +1567,12 @@ __device__ __forceinline__ double2 ColFetch(const double2 (&col)[kColSlices], in
 
 // blk, nblk: this workgroup's block index and the block count of the
 // signal's launch (bins are distributed over nblk x waves wavefronts).
+// NS: column slices of 64 lanes (1 when trials x rows <= 64, else
+// kColSlices); kIo: the per-bin outputs of the identify-and-estimate entry
+// point are compiled in (c.idout is used only then).  A solve's kernels run
+// with one slice and no outputs, which leaves them registers for more
+// wavefronts per SIMD.
+template <int NS, bool kIo>
 __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int nblk) {
 #pragma clang fp contract(off)
   DevState *st = c.st;
@@ -1594,12 +1601,12 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
   // With per-bin outputs (the identify-and-estimate entry point) every bin
   // reports its decision, k1, k0, coefficient and decision margin, and the
   // found-mode map is left alone.
-  const IdOut *const io = c.idout;
+  const IdOut *const io = kIo ? c.idout : nullptr;
   for (int32_t b = blk * waves + (threadIdx.x >> 6); b < B; b += nblk * waves) {
     const double2 *Ab = c.A + b;
-    double2 col[kColSlices];
+    double2 col[NS];
 #pragma unroll
-    for (int s = 0; s < kColSlices; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int p = lane + 64 * s;
       col[s] = make_double2(0.0, 0.0);
       if (p < npairs) {
@@ -1694,7 +1701,7 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
     // of A * e^{-2 pi i (q + off*B) k1 / n}.
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int s = 0; s < kColSlices; ++s) {
+    for (int s = 0; s < NS; ++s) {
       const int p = lane + 64 * s;
       if (p < npairs) {
         const int t = p / rows;
@@ -1732,9 +1739,13 @@ __device__ __forceinline__ void IdentifyBody(const Ctx &c, int p, int blk, int n
   if (threadIdx.x == 0 && nany) atomicOr(&st->found_iter[p], 1);
 }
 
+template <int NS, bool kIo>
 __global__ void __launch_bounds__(kThreads) k_identify(Ctx c, int p) {
-  IdentifyBody(c, p, blockIdx.x, gridDim.x);
+  IdentifyBody<NS, kIo>(c, p, blockIdx.x, gridDim.x);
 }
+
+// Column slices a launch needs for trials x rows entries per bin.
+inline int ColSlicesFor(int64_t trials_rows) { return trials_rows <= 64 ? 1 : kColSlices; }
 
 // ---------------------------------------------------------------------------
 // The late iterations of a solve.  Once an iteration starts with
@@ -1780,13 +1791,13 @@ __device__ __forceinline__ double2 RowFactor(const Ctx &c, const IterParams &ip,
   return Cmul(base, (row & 1) ? g : Conj(g));
 }
 
-template <typename Tx>
+template <typename Tx, int NS>
 __global__ void __launch_bounds__(kThreads)
     k_late_fold(Ctx c, int p, int32_t nb_ident, int32_t with_identify) {
   HIP_DYNAMIC_SHARED(double2, lds)
   const int32_t blk = blockIdx.x;
   if (blk < nb_ident) {
-    if (with_identify) IdentifyBody(c, 1 - p, blk, nb_ident);
+    if (with_identify) IdentifyBody<NS, false>(c, 1 - p, blk, nb_ident);
     return;
   }
   IterParams ip;
@@ -2246,14 +2257,15 @@ __global__ void __launch_bounds__(kFftThreads)
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
+template <int NS>
+__global__ void __launch_bounds__(kThreads, NS == 1 ? 3 : 1)
     k_identify_batch(const Ctx *ctxs, const int2 *map, const int32_t *total, const int32_t *cnt,
                      int p) {
   const int32_t tot = total[2];
   for (int32_t u = blockIdx.x; u < tot; u += gridDim.x) {
     if (u != int32_t(blockIdx.x)) __syncthreads();
     const int2 m = map[u];
-    IdentifyBody(ctxs[m.x], p, m.y, cnt[m.x]);
+    IdentifyBody<NS, false>(ctxs[m.x], p, m.y, cnt[m.x]);
   }
 }
 
@@ -2912,12 +2924,20 @@ hipError_t LaunchBigFft(const Ctx &c, int parity, hipStream_t s) {
 hipError_t LaunchLateIteration(const Ctx &c, int parity, int32_t nb_ident, int32_t tiles,
                                size_t lds_time, size_t lds_fft, int with_identify, int small_fft,
                                hipStream_t s) {
-  if (c.mixed)
-    hipLaunchKernelGGL(k_late_fold<float2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
-                       lds_time, s, c, parity, nb_ident, with_identify);
+  const dim3 grid(nb_ident + kPrepBlocks + tiles);
+  const bool one = ColSlicesFor(int64_t(c.trials_cap) * c.rows_cap) == 1;
+  if (c.mixed && one)
+    hipLaunchKernelGGL((k_late_fold<float2, 1>), grid, dim3(kThreads), lds_time, s, c, parity,
+                       nb_ident, with_identify);
+  else if (c.mixed)
+    hipLaunchKernelGGL((k_late_fold<float2, kColSlices>), grid, dim3(kThreads), lds_time, s, c,
+                       parity, nb_ident, with_identify);
+  else if (one)
+    hipLaunchKernelGGL((k_late_fold<double2, 1>), grid, dim3(kThreads), lds_time, s, c, parity,
+                       nb_ident, with_identify);
   else
-    hipLaunchKernelGGL(k_late_fold<double2>, dim3(nb_ident + kPrepBlocks + tiles), dim3(kThreads),
-                       lds_time, s, c, parity, nb_ident, with_identify);
+    hipLaunchKernelGGL((k_late_fold<double2, kColSlices>), grid, dim3(kThreads), lds_time, s, c,
+                       parity, nb_ident, with_identify);
   // the late B's own bin-FFT path: the small one alone where it applies
   if (small_fft)
     hipLaunchKernelGGL(k_late_fft<kFftSmallOnly>, dim3(c.trials_cap * c.rows_cap),
@@ -2934,7 +2954,20 @@ hipError_t LaunchPublish(const Ctx &c, int parity, hipStream_t s) {
 }
 
 hipError_t LaunchIdentify(const Ctx &c, int parity, int32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_identify, dim3(blocks), dim3(kThreads), 0, s, c, parity);
+  const bool one = ColSlicesFor(int64_t(c.trials_cap) * c.rows_cap) == 1;
+  if (c.idout) {
+    if (one)
+      hipLaunchKernelGGL((k_identify<1, true>), dim3(blocks), dim3(kThreads), 0, s, c, parity);
+    else
+      hipLaunchKernelGGL((k_identify<kColSlices, true>), dim3(blocks), dim3(kThreads), 0, s, c,
+                         parity);
+  } else {
+    if (one)
+      hipLaunchKernelGGL((k_identify<1, false>), dim3(blocks), dim3(kThreads), 0, s, c, parity);
+    else
+      hipLaunchKernelGGL((k_identify<kColSlices, false>), dim3(blocks), dim3(kThreads), 0, s, c,
+                         parity);
+  }
   return hipGetLastError();
 }
 
@@ -3002,8 +3035,12 @@ hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStre
                          b.lds_fft, s, b.ctxs, w.map[3], w.total, parity);
       break;
     default:
-      hipLaunchKernelGGL(k_identify_batch, dim3(b.grid_ident), dim3(kThreads), 0, s, b.ctxs,
-                         w.map[2], w.total, w.cnt + 2 * b.count, parity);
+      if (b.col_slices == 1)
+        hipLaunchKernelGGL(k_identify_batch<1>, dim3(b.grid_ident), dim3(kThreads), 0, s,
+                           b.ctxs, w.map[2], w.total, w.cnt + 2 * b.count, parity);
+      else
+        hipLaunchKernelGGL(k_identify_batch<kColSlices>, dim3(b.grid_ident), dim3(kThreads), 0,
+                           s, b.ctxs, w.map[2], w.total, w.cnt + 2 * b.count, parity);
       break;
   }
   return hipGetLastError();
