@@ -7,6 +7,13 @@
 // TCC_EA0_RDREQ_128B_sum it shows the size of the HBM read requests each
 // form produces.  Built by build() into tests/gather_probe.
 //
+// Then two measurements behind the gather-fold's design: plain random
+// gathers from arrays of 8 MiB to 1 GiB (does a smaller footprint, one
+// signal's x in the Infinity Cache, raise the line rate?), and the
+// transpose of a gather -- a contiguous read whose samples are written to
+// random places, alone and as a filter over a 4x larger contiguous read
+// (the first phase of a form that would stream x instead of gathering it).
+//
 // Every index is (a u + c) & (n - 1) < n, and buffer loads are also range-
 // checked by the hardware against the array's size.
 #include <hip/hip_runtime.h>
@@ -48,6 +55,34 @@ __global__ void __launch_bounds__(kThreads)
   out[t] = s;
 }
 
+// contiguous read of the first kPer * grid samples, each written to a
+// random place of out (n elements)
+__global__ void __launch_bounds__(kThreads)
+    k_scatter(const unsigned long long *in, uint32_t n, uint32_t a, uint32_t c,
+              unsigned long long *out) {
+  const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = t + j * gridDim.x * kThreads;
+    out[(a * i + c) & (n - 1)] = in[i];
+  }
+}
+
+// contiguous read of every sample of in (n elements); those whose permuted
+// index falls in the first quarter are written there
+__global__ void __launch_bounds__(kThreads)
+    k_filter_scatter(const unsigned long long *in, uint32_t n, uint32_t a, uint32_t c,
+                     unsigned long long *out) {
+  const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = t + j * gridDim.x * kThreads;
+    const uint32_t v = (a * i + c) & (n - 1);
+    const unsigned long long x = in[i];
+    if (v < n / 4) out[v] = x;
+  }
+}
+
 #define CHECK(x)                                                                   \
   do {                                                                             \
     const hipError_t e_ = (x);                                                     \
@@ -83,6 +118,29 @@ int Time(const char *name, const unsigned long long *p, uint32_t n, uint32_t a, 
   return 0;
 }
 
+// Best of 5 timed launches (after one warm-up) of kernel K over `blocks`.
+template <typename K, typename... Args>
+int Best(float *us, K kernel, uint32_t blocks, Args... args) {
+  hipEvent_t e0, e1;
+  CHECK(hipEventCreate(&e0));
+  CHECK(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int rep = 0; rep < 6; ++rep) {
+    CHECK(hipEventRecord(e0));
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kThreads), 0, 0, args...);
+    CHECK(hipGetLastError());
+    CHECK(hipEventRecord(e1));
+    CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0 && ms < best) best = ms;
+  }
+  *us = best * 1e3f;
+  CHECK(hipEventDestroy(e0));
+  CHECK(hipEventDestroy(e1));
+  return 0;
+}
+
 }  // namespace
 
 int main() {
@@ -105,6 +163,39 @@ int main() {
       return 1;
   }
   CHECK(hipFree(p));
+  CHECK(hipFree(out));
+
+  // plain random gathers by array size, 2^23 per launch
+  const uint32_t nmax = 1u << 27;  // 1 GiB
+  unsigned long long *big = nullptr, *dst = nullptr;
+  CHECK(hipMalloc(&big, size_t(nmax) * 8));
+  CHECK(hipMalloc(&dst, size_t(n) * 8));
+  CHECK(hipMemset(big, 1, size_t(nmax) * 8));
+  CHECK(hipMemset(dst, 0, size_t(n) * 8));
+  CHECK(hipMalloc(&out, size_t(gathers / kPer) * 8));
+  CHECK(hipDeviceSynchronize());
+  const uint32_t ar = 2654435761u, blocks = gathers / kPer / kThreads;
+  for (uint32_t logn = 20; logn <= 27; ++logn) {
+    float us = 0;
+    if (Best(&us, k_gather<0>, blocks, (const unsigned long long *)big, 1u << logn, ar, 12345u,
+             out))
+      return 1;
+    std::printf("size %5u MiB  random gathers %8.1f us  as 128-byte lines %7.1f GB/s\n",
+                (1u << logn) >> 17, us, gathers * 128.0 / us / 1e3);
+  }
+  // scatters: 2^23 samples read in order, written at random into n; then
+  // all n read in order, a quarter written at random
+  float us = 0;
+  if (Best(&us, k_scatter, blocks, (const unsigned long long *)big, n, ar, 12345u, dst)) return 1;
+  std::printf("scatter %u contiguous reads to random 8-byte writes %8.1f us  %6.1f G samples/s\n",
+              gathers, us, gathers / us / 1e3);
+  if (Best(&us, k_filter_scatter, n / kPer / kThreads, (const unsigned long long *)big, n, ar,
+           12345u, dst))
+    return 1;
+  std::printf("filter %u contiguous reads, %u random 8-byte writes %8.1f us  %6.1f G samples/s\n",
+              n, n / 4, us, (n / 4) / us / 1e3);
+  CHECK(hipFree(big));
+  CHECK(hipFree(dst));
   CHECK(hipFree(out));
   std::printf("done\n");
   return 0;
