@@ -50,11 +50,11 @@ def time_solves(M, solver, xd, steps, warmup):
 LINE = 128
 
 
-def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
+def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA, per_launch=None):
     """Algorithmic bytes of every k_bin_time launch of the last solve: LINE
     bytes per distinct LINE-byte line of x its gathers touch, plus the fold
     it writes (rows x B double2).  Also the reference's logical gather
-    bytes."""
+    bytes.  per_launch (a list) receives each launch's algorithmic bytes."""
     bins_log, _ = solver.log()
     tfs, qs = solver.transforms()
     line_bytes = logical = 0
@@ -62,6 +62,7 @@ def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
     seen = np.zeros(n // per_line, dtype=bool)
     for it, B in enumerate(bins_log):
         B = int(B)
+        before = line_bytes
         P = M.lib().mpsft_window_p(n, B, delta)
         p2 = (P - 1) // 2
         bits = M.num_bits(n, B)
@@ -75,6 +76,8 @@ def gather_bytes(M, solver, elem_bytes, n=N, delta=DELTA):
                 seen[((a * (u % n) + c) % n) // per_line] = True
             line_bytes += LINE * int(seen.sum()) + rows * B * 16
             logical += rows * P * elem_bytes
+        if per_launch is not None:
+            per_launch.append(line_bytes - before)
     return line_bytes, logical, len(bins_log)
 
 
@@ -143,7 +146,8 @@ def large_workload(M, steps, warmup):
     err = max(abs(got[key] - v) / abs(v) for key, v in truth.items() if key in got)
     _, ms = s.solve_profiled(xd, SOLVE_SEED)
     bins_log, _ = s.log()
-    line_bytes, logical, n_launch = gather_bytes(M, s, 8, n=n)
+    per_launch = []
+    line_bytes, logical, n_launch = gather_bytes(M, s, 8, n=n, per_launch=per_launch)
     # the FP64 form on the same complex64-rounded samples
     del xd
     s64 = M.Solver(n, M.solve_options(k, sig_eff, window_delta=DELTA), M.MPSFT_FP64)
@@ -170,7 +174,9 @@ def large_workload(M, steps, warmup):
                         "effective_GBs": round(line_bytes / t / 1e9, 1),
                         "frac_of_8TBs": round(line_bytes / t / 1e9 / HBM_PEAK_GBS, 4),
                         "logical_gather_GBs": round(logical / t / 1e9, 1),
-                        "launches": n_launch},
+                        "launches": n_launch,
+                        "bins_per_launch": [int(b) for b in bins_log],
+                        "algorithmic_bytes_per_launch": per_launch},
     }
 
 
