@@ -454,6 +454,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the timed form's batch (tile width x identify bins) before "
+                         "timing it, on solve seeds the timed steps do not use")
     ap.add_argument("--batch-count", type=int, default=64,
                     help="signals per batched solve (the timed form)")
     ap.add_argument("--cpu-solves", type=int, default=3)
@@ -537,11 +540,16 @@ def main():
 
     bm = M.Batch([N] * G, opts, M.MPSFT_MIXED)
     xdb = bm.upload(xs64)
+    tuned = None
+    if args.autotune:
+        log("autotuning the batch")
+        tuned = bm.autotune(xdb, seeds_for(-1000))
     log("timing the batched MIXED form")
     t_bm = time_batched(M, bm, xdb, seeds_for, args.steps, args.warmup)
     if args.batched_only:
         print(json.dumps({"us_per_solve": round(t_bm / G * 1e6, 2), "signals": G,
-                          "steps": args.steps, "warmup": args.warmup}), flush=True)
+                          "steps": args.steps, "warmup": args.warmup,
+                          "tile_residues": tuned}), flush=True)
         return
     # identical support against the FP64 single-signal form on the same
     # complex64-rounded input, for every signal of the last step's seeds

@@ -330,10 +330,12 @@ int mpsft_batch_solve(mpsft_batch *batch, const void *const *x_dev, const uint32
 int mpsft_batch_solve_profiled(mpsft_batch *batch, const void *const *x_dev,
                                const uint32_t *seeds, int32_t *iters, int32_t *status,
                                double *kernel_ms, int32_t *launches, void *stream);
-/* Times short batched solves of these signals at every gather-fold tile
- * width (0: the default width per B, or 1, 2, 4, 8 residues) and k_identify
- * bins per unit (8, 16, 32), and keeps the fastest pair for later solves;
- * *residues gets its tile width.  Results do not depend on either. */
+/* Times whole batched solves of these signals in a coordinate search --
+ * k_identify bins per unit (8, 16, 32), then the gather-fold tile width of
+ * B >= 256 and then that of smaller B (0: the default width per B, or 1,
+ * 2, 4, 8 residues) -- and keeps the fastest setting for later solves;
+ * *residues gets the tile width of B >= 256.  Results depend on none of
+ * them.  mpsft_solver_set_tile_residues sets one width for every B. */
 int mpsft_batch_autotune(mpsft_batch *batch, const void *const *x_dev, const uint32_t *seeds,
                          int32_t *residues, void *stream);
 /* Results of signal i of the last batched solve, as the solver calls. */

@@ -385,7 +385,7 @@ int Engine::EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t b
     if (!WindowParams(n_, B, delta, &bp)) return MPSFT_ERR_INVALID_ARG;
     if (bp.logM > kMaxLogMBig) return MPSFT_ERR_UNSUPPORTED;
     const int32_t nlam = NumLags(bits_override > 0 ? bits_override : bp.bits, bp.f);
-    bp.R = ResiduesPerTile(B, nlam, bp.f, forced_R_, tile_target_);
+    bp.R = ResiduesPerTile(B, nlam, bp.f, forced_R_[B >= kSmallTileB ? 0 : 1], tile_target_);
     bp.slices = SlicesFor(n_, mask_, bp);
     bp.slice_shift = bp.slices > 1 ? Log2(n_) - 3 : 0;
     bp.off = 0;
@@ -458,20 +458,24 @@ int Engine::SetThresholds(double sigma) {
   return MPSFT_OK;
 }
 
-// Fixes the k_bin_time tile width (residues per tile) of every B, or back
-// to the default heuristic for R <= 0.  Tiling changes only how the work
-// is split, not what is summed; the buffers are re-sized by Reserve.
-int Engine::SetTileResidues(int32_t R) {
+// Fixes the k_bin_time tile width (residues per tile) of every B >=
+// kSmallTileB and of every smaller B, each back to the default heuristic
+// for R <= 0.  Tiling changes only how the work is split, not what is
+// summed; the buffers are re-sized by Reserve.
+int Engine::SetTileResidues(int32_t R_wide, int32_t R_narrow) {
   if (status_ != MPSFT_OK) return status_;
-  if (R > 8 || (R > 0 && (R & (R - 1)))) return MPSFT_ERR_INVALID_ARG;
+  for (const int32_t R : {R_wide, R_narrow})
+    if (R > 8 || (R > 0 && (R & (R - 1)))) return MPSFT_ERR_INVALID_ARG;
   Drain();
   ++tab_epoch_;
-  forced_R_ = R > 0 ? R : 0;
+  forced_R_[0] = R_wide > 0 ? R_wide : 0;
+  forced_R_[1] = R_narrow > 0 ? R_narrow : 0;
   for (size_t B = 0; B < hbtab_.size(); ++B) {
     if (!built_[B]) continue;
     BParams &bp = hbtab_[B];
     const int32_t bits = tab_bits_override_ > 0 ? tab_bits_override_ : bp.bits;
-    bp.R = ResiduesPerTile(int32_t(B), NumLags(bits, bp.f), bp.f, forced_R_, tile_target_);
+    bp.R = ResiduesPerTile(int32_t(B), NumLags(bits, bp.f), bp.f,
+                           forced_R_[B >= size_t(kSmallTileB) ? 0 : 1], tile_target_);
   }
   if (d_btab_)
     HIP_OK(hipMemcpy(d_btab_, hbtab_.data(), hbtab_.size() * sizeof(BParams),

@@ -46,6 +46,7 @@ constexpr int kSlices = 8;         // XCDs of MI355X: partial gather-folds of an
 constexpr int kIdentKeys = 64;     // new keys a k_identify workgroup lists before appending
 constexpr int kPrepBlocks = 16;    // k_bin_time workgroups computing mode terms
 constexpr int kMaxBins = 32767;    // largest B (2B-1 <= 2^kMaxLogMBig)
+constexpr int kSmallTileB = 256;   // below it a B takes the narrow class's tile width
 constexpr int kTargetTiles = 512;  // k_bin_time tiles aimed for per trial
 constexpr int kColSlices = 4;      // k_identify holds trials*rows <= 256 entries per bin
 constexpr int kMaxTrials = 15;     // trials per iteration (odd)
@@ -228,7 +229,8 @@ class Engine {
   int precision() const { return precision_; }
   // k_bin_time tile width: residues per tile (1, 2, 4, 8), or 0 for the
   // default choice.
-  int SetTileResidues(int32_t R);
+  int SetTileResidues(int32_t R) { return SetTileResidues(R, R); }
+  int SetTileResidues(int32_t R_wide, int32_t R_narrow);
 
   // Recording: the next solves keep every iteration's identified bins, so
   // that Replay (the recovery map with those decisions fixed, linear in x)
@@ -355,7 +357,9 @@ class Engine {
   int32_t graph_late_iters_ = 2;            // iterations per late graph
   int64_t shape_gen_ = 0;
   size_t lds_fft_small_ = 0;  // k_bin_fft's LDS on its small path (largest reserved B)
-  int32_t forced_R_ = 0;  // k_bin_time tile width fixed by the autotuner
+  // k_bin_time tile width fixed by the autotuner (0: the default per B), for
+  // B >= kSmallTileB and for B < kSmallTileB (the late iterations' B)
+  int32_t forced_R_[2] = {0, 0};
   // k_bin_time tiles aimed for per trial by the default tile width: fewer
   // (wider tiles) when many signals share the launches of a batch
   int32_t tile_target_ = kTargetTiles;
