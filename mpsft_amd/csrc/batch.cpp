@@ -3,6 +3,8 @@
 // all of them.
 #include "batch.h"
 
+#include <cstring>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -99,6 +101,9 @@ Batch::~Batch() {
   (void)hipFree(d_adj_map_fft_);
   (void)hipFree(d_modes_out_);
   (void)hipFree(d_adj_steps_);
+  (void)hipFree(d_stage_);
+  if (p_stage_) (void)hipHostFree(p_stage_);
+  if (stage_ev_) (void)hipEventDestroy(stage_ev_);
   if (h_results_) (void)hipHostFree(h_results_);
   if (h_host_) (void)hipHostFree(const_cast<int32_t *>(h_host_));
 }
@@ -110,11 +115,18 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
   if (status_ != MPSFT_OK) return status_;
   if (!x || !seeds) return MPSFT_ERR_INVALID_ARG;
   seeds_.assign(seeds, seeds + eng_.size());
+  // the previous solve's staged uploads have been copied (its end drained
+  // the stream; the event covers a solve that stopped early)
+  if (stage_ev_ && hipEventSynchronize(stage_ev_) != hipSuccess) return MPSFT_ERR_HIP;
+  stage_used_ = 0;
+  uploads_.assign(eng_.size(), TfUpload{});
   std::vector<Ctx> ctxs(eng_.size());
   for (size_t i = 0; i < eng_.size(); ++i) {
     Engine &e = *eng_[i];
     e.rec_valid_ = false;
+    e.defer_upload_ = &uploads_[i];
     int rc = e.PrepareSolve(x[i], opts_[i], seeds[i], s, &args_[i]);
+    e.defer_upload_ = nullptr;
     if (rc != MPSFT_OK) return rc;
     if (e.record_ && (rc = e.EnsureRecord()) != MPSFT_OK) return rc;
     if (e.fft_big_) return MPSFT_ERR_UNSUPPORTED;
@@ -122,6 +134,8 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
     ctxs[i] = e.MakeCtx();
     ctxs[i].batch_done = d_counters_ + 1;
   }
+  const int urc = FlushUploads(s);
+  if (urc != MPSFT_OK) return urc;
   HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
   HIP_OK(hipMemcpyAsync(d_args_, args_.data(), args_.size() * sizeof(InitArgs),
                         hipMemcpyHostToDevice, s));
@@ -238,6 +252,61 @@ int Batch::Build(hipStream_t s) {
   return MPSFT_OK;
 }
 
+int Batch::FlushUploads(hipStream_t s) {
+  auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
+  int32_t m = 0;
+  size_t bytes = 0;
+  for (const TfUpload &u : uploads_)
+    if (u.ntf > 0 || u.nq > 0) {
+      ++m;
+      bytes += up16(size_t(u.ntf) * sizeof(DevTf)) + up16(size_t(u.nq) * sizeof(int64_t));
+    }
+  if (m == 0) return MPSFT_OK;
+  const size_t head = up16(size_t(m) * sizeof(TfScatterDesc));
+  bytes += head;
+  if (stage_used_ + bytes > stage_cap_) {
+    // a pending copy may still read the staged bytes: wait before reusing them
+    if (stage_ev_ && hipEventSynchronize(stage_ev_) != hipSuccess) return MPSFT_ERR_HIP;
+    stage_used_ = 0;
+    if (bytes > stage_cap_) {
+      const size_t cap = std::max(bytes, std::max<size_t>(2 * stage_cap_, 1 << 16));
+      (void)hipFree(d_stage_);
+      if (p_stage_) (void)hipHostFree(p_stage_);
+      d_stage_ = p_stage_ = nullptr;
+      stage_cap_ = 0;
+      if (hipMalloc(reinterpret_cast<void **>(&d_stage_), cap) != hipSuccess ||
+          hipHostMalloc(reinterpret_cast<void **>(&p_stage_), cap) != hipSuccess)
+        return MPSFT_ERR_NOMEM;
+      stage_cap_ = cap;
+    }
+  }
+  if (!stage_ev_ && hipEventCreate(&stage_ev_) != hipSuccess)
+    return MPSFT_ERR_HIP;
+  unsigned char *h = p_stage_ + stage_used_;
+  TfScatterDesc *desc = reinterpret_cast<TfScatterDesc *>(h);
+  size_t off = head;
+  int32_t k = 0;
+  for (const TfUpload &u : uploads_) {
+    if (u.ntf <= 0 && u.nq <= 0) continue;
+    TfScatterDesc &d = desc[k++];
+    d.tf_dst = u.tf_dst;
+    d.q_dst = u.q_dst;
+    d.ntf = u.ntf;
+    d.nq = u.nq;
+    d.tf_off = int64_t(off);
+    std::memcpy(h + off, u.tf_src, size_t(u.ntf) * sizeof(DevTf));
+    off += up16(size_t(u.ntf) * sizeof(DevTf));
+    d.q_off = int64_t(off);
+    std::memcpy(h + off, u.q_src, size_t(u.nq) * sizeof(int64_t));
+    off += up16(size_t(u.nq) * sizeof(int64_t));
+  }
+  HIP_OK(hipMemcpyAsync(d_stage_ + stage_used_, h, bytes, hipMemcpyHostToDevice, s));
+  HIP_OK(LaunchTfScatter(d_stage_ + stage_used_, m, s));
+  HIP_OK(hipEventRecord(stage_ev_, s));
+  stage_used_ += bytes;
+  return MPSFT_OK;
+}
+
 // Every signal's transforms and q's drawn for iterations [0, need); when an
 // engine's arrays must grow, the stream is drained first and the Ctx array
 // re-uploaded (the kernels read it).
@@ -250,12 +319,17 @@ int Batch::EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s) {
   for (auto &e : eng_) grow = grow || want > e->tf_cap_;
   if (grow && hipStreamSynchronize(s) != hipSuccess) return MPSFT_ERR_HIP;
   std::vector<Ctx> ctxs(eng_.size());
+  uploads_.assign(eng_.size(), TfUpload{});
   for (size_t i = 0; i < eng_.size(); ++i) {
+    eng_[i]->defer_upload_ = &uploads_[i];
     const int rc = eng_[i]->EnsureTransforms(seeds_[i], want, opts_[i].trials, s);
+    eng_[i]->defer_upload_ = nullptr;
     if (rc != MPSFT_OK) return rc;
     ctxs[i] = eng_[i]->MakeCtx();
     ctxs[i].batch_done = d_counters_ + 1;
   }
+  const int urc = FlushUploads(s);
+  if (urc != MPSFT_OK) return urc;
   if (grow && hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
                              hipMemcpyHostToDevice, s) != hipSuccess)
     return MPSFT_ERR_HIP;

@@ -58,6 +58,9 @@ class Batch {
   int RunProfiled(hipStream_t s, int32_t max_iter, double *kernel_ms, int32_t *launches);
   int Finish(int32_t *iters, int32_t *status, hipStream_t s);
   int EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s);
+  // Copies every engine's deferred transform upload (uploads_) in one
+  // staging copy and one k_tf_scatter launch.
+  int FlushUploads(hipStream_t s);
 
   int status_ = MPSFT_OK;
   int precision_;
@@ -89,6 +92,13 @@ class Batch {
   bool solved_ = false;  // h_results_ hold a finished batched solve
   AdjStep *d_adj_steps_ = nullptr;
   size_t adj_steps_cap_ = 0;
+  // The transform uploads of a solve: each engine's deferred upload, then
+  // staged (pinned host, device) and appended within a solve, so a pending
+  // copy's bytes are never rewritten; stage_ev_ follows the last copy.
+  std::vector<TfUpload> uploads_;
+  unsigned char *p_stage_ = nullptr, *d_stage_ = nullptr;
+  size_t stage_cap_ = 0, stage_used_ = 0;
+  hipEvent_t stage_ev_ = nullptr;
 };
 
 }  // namespace mpsft

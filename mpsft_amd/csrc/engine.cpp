@@ -604,6 +604,7 @@ int Engine::GrowTransforms(int32_t count, bool *grown) {
 // and its stop is reported by a kernel) or after the arrays grow (callers
 // drain the stream first).
 int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s) {
+  if (defer_upload_) *defer_upload_ = TfUpload{};
   bool grown = false;
   const int rc = GrowTransforms(count, &grown);
   if (rc != MPSFT_OK) return rc;
@@ -635,12 +636,17 @@ int Engine::EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipSt
   }
   const int32_t from = (fresh || grown) ? 0 : tf_count_;
   const size_t q0 = size_t(from) * trials_cap_, q1 = size_t(target) * trials_cap_;
-  std::copy(h_tfs_.begin() + from, h_tfs_.begin() + target, p_tfs_ + from);
-  std::copy(h_qs_.begin() + q0, h_qs_.begin() + q1, p_qs_ + q0);
-  HIP_OK(hipMemcpyAsync(d_tfs_ + from, p_tfs_ + from, size_t(target - from) * sizeof(DevTf),
-                        hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(d_qs_ + q0, p_qs_ + q0, (q1 - q0) * sizeof(int64_t),
-                        hipMemcpyHostToDevice, s));
+  if (defer_upload_) {
+    *defer_upload_ = TfUpload{h_tfs_.data() + from, d_tfs_ + from, int64_t(target - from),
+                              h_qs_.data() + q0, d_qs_ + q0, int64_t(q1 - q0)};
+  } else {
+    std::copy(h_tfs_.begin() + from, h_tfs_.begin() + target, p_tfs_ + from);
+    std::copy(h_qs_.begin() + q0, h_qs_.begin() + q1, p_qs_ + q0);
+    HIP_OK(hipMemcpyAsync(d_tfs_ + from, p_tfs_ + from, size_t(target - from) * sizeof(DevTf),
+                          hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(d_qs_ + q0, p_qs_ + q0, (q1 - q0) * sizeof(int64_t),
+                          hipMemcpyHostToDevice, s));
+  }
   tf_count_ = target;
   tf_valid_ = true;
   return MPSFT_OK;

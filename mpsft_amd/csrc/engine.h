@@ -115,6 +115,17 @@ struct IterParams {
   int32_t done;  // the loop stopped before this iteration
 };
 
+// New transforms and q's an engine drew but did not upload (a batched solve
+// gathers every signal's into one copy): host sources, device destinations.
+struct TfUpload {
+  const DevTf *tf_src;
+  DevTf *tf_dst;
+  int64_t ntf;
+  const int64_t *q_src;
+  int64_t *q_dst;
+  int64_t nq;
+};
+
 struct LoopCounters {
   int32_t it;     // iteration index (-1 before the first)
   int32_t stale;  // iterations that found nothing (src/demo1.cc:55-57)
@@ -256,6 +267,9 @@ class Engine {
   int SetThresholds(double sigma);
   int Reserve(int32_t trials, const std::vector<int32_t> &bs);
   int EnsureTransforms(uint32_t seed, int32_t count, int32_t trials, hipStream_t s);
+  // While set, EnsureTransforms records its upload here instead of copying
+  // (the host sources stay valid until the next draw).
+  TfUpload *defer_upload_ = nullptr;
   int GrowTransforms(int32_t count, bool *grown);
   Ctx MakeCtx() const;
   int LaunchIteration(const Ctx &c, int parity, hipStream_t s, bool big = true);

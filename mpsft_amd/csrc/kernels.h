@@ -72,6 +72,16 @@ struct BatchLaunch {
   int32_t *host;                        // host-mapped copy of counters
   int mixed;
 };
+// A staging area in device memory holding `count` descriptors, then their
+// payloads: k_tf_scatter copies each descriptor's transforms and q's to an
+// engine's arrays (one launch for every signal of a batch).
+struct TfScatterDesc {
+  DevTf *tf_dst;
+  int64_t *q_dst;
+  int64_t tf_off, q_off;  // payload offsets from the staging area, in bytes
+  int64_t ntf, nq;
+};
+hipError_t LaunchTfScatter(const unsigned char *staging, int32_t count, hipStream_t s);
 hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count,
                             int32_t *counters, hipStream_t s);
 hipError_t LaunchBatchIteration(const BatchLaunch &b, int parity, hipStream_t s);

@@ -2864,12 +2864,27 @@ __global__ void __launch_bounds__(kThreads)
                 int64_t(per) * blockDim.x);
 }
 
+// One workgroup per descriptor of a staging area (TfScatterDesc).
+__global__ void __launch_bounds__(kThreads) k_tf_scatter(const unsigned char *base) {
+  const TfScatterDesc d = reinterpret_cast<const TfScatterDesc *>(base)[blockIdx.x];
+  const DevTf *tf = reinterpret_cast<const DevTf *>(base + d.tf_off);
+  const int64_t *q = reinterpret_cast<const int64_t *>(base + d.q_off);
+  for (int64_t i = threadIdx.x; i < d.ntf; i += blockDim.x) d.tf_dst[i] = tf[i];
+  for (int64_t i = threadIdx.x; i < d.nq; i += blockDim.x) d.q_dst[i] = q[i];
+}
+
 int32_t Blocks(int64_t work) {
   const int64_t b = (work + kThreads - 1) / kThreads;
   return int32_t(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
 }  // namespace
+
+hipError_t LaunchTfScatter(const unsigned char *staging, int32_t count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tf_scatter, dim3(count), dim3(kThreads), 0, s, staging);
+  return hipGetLastError();
+}
 
 hipError_t LaunchBuildTwiddles(double2 *tw, hipStream_t s) {
   hipLaunchKernelGGL(k_build_twiddles, dim3(Blocks(kTwM / 2)), dim3(kThreads), 0, s, tw);
