@@ -23,6 +23,9 @@ namespace {
 // 8192: 145.9 us per solve), and bins per k_identify unit (default 16: four
 // per wavefront), each overridable for measurement by MPSFT_PACKED_GRID /
 // MPSFT_IDENT_BINS.
+// Iterations whose transforms a batched solve draws before its first launch.
+constexpr int32_t kBatchFirstDraws = 4;
+
 int32_t EnvInt(const char *name, int32_t def, int32_t lo, int32_t hi) {
   const char *e = std::getenv(name);
   if (!e) return def;
@@ -125,7 +128,10 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
     Engine &e = *eng_[i];
     e.rec_valid_ = false;
     e.defer_upload_ = &uploads_[i];
-    int rc = e.PrepareSolve(x[i], opts_[i], seeds[i], s, &args_[i]);
+    // the first two graphs' iterations only: Run draws the rest while the
+    // device works (every signal's first 32 drawn here kept the device idle
+    // between solves)
+    int rc = e.PrepareSolve(x[i], opts_[i], seeds[i], s, &args_[i], kBatchFirstDraws);
     e.defer_upload_ = nullptr;
     if (rc != MPSFT_OK) return rc;
     if (e.record_ && (rc = e.EnsureRecord()) != MPSFT_OK) return rc;

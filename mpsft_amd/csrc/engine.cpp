@@ -815,7 +815,7 @@ int Engine::ReadState(hipStream_t s) {
 // window tables of every reachable B, thresholds, buffers and the first
 // transform draws; *a gets the device-side settings for k_init.
 int Engine::PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t seed,
-                         hipStream_t s, InitArgs *a) {
+                         hipStream_t s, InitArgs *a, int32_t first_draws) {
   if (status_ != MPSFT_OK) return status_;
   if (!x || opt.trials <= 0 || opt.trials % 2 != 1 || opt.trials > kMaxTrials ||
       !(opt.window_threshold > 0) || opt.k < 0 || opt.min_bins <= 0 || opt.min_bins % 2 != 1 ||
@@ -849,8 +849,8 @@ int Engine::PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t
   const int32_t max_iter = opt.max_iter > 0 ? opt.max_iter : 100000;
   // A few graphs' worth up front; the launch loop draws ahead in doubling
   // batches while the device runs.
-  rc = EnsureTransforms(seed, std::min(max_iter, std::max(kFirstDraws, ChunkIters())), opt.trials,
-                        s);
+  const int32_t first = first_draws > 0 ? first_draws : std::max(kFirstDraws, ChunkIters());
+  rc = EnsureTransforms(seed, std::min(max_iter, first), opt.trials, s);
   if (rc != MPSFT_OK) return rc;
   *a = InitArgs{};
   a->x = x;

@@ -260,8 +260,10 @@ class Engine {
 
  private:
   friend class Batch;
+  // first_draws > 0: the iterations whose transforms are drawn before the
+  // first launch (default: a few graphs' worth).
   int PrepareSolve(const void *x, const mpsft_solve_options &opt, uint32_t seed, hipStream_t s,
-                   InitArgs *a);
+                   InitArgs *a, int32_t first_draws = 0);
   int SolveResult(const mpsft_solve_options &opt, int32_t *iters);
   int EnsureTables(const std::vector<int32_t> &bs, double delta, int32_t bits_override);
   int SetThresholds(double sigma);
