@@ -137,8 +137,7 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
     if (e.record_ && (rc = e.EnsureRecord()) != MPSFT_OK) return rc;
     if (e.fft_big_) return MPSFT_ERR_UNSUPPORTED;
     if ((e.precision_ == MPSFT_MIXED) != (precision_ == MPSFT_MIXED)) return MPSFT_ERR_INVALID_ARG;
-    ctxs[i] = e.MakeCtx();
-    ctxs[i].batch_done = d_counters_ + 1;
+    ctxs[i] = SignalCtx(e);
   }
   const int urc = FlushUploads(s);
   if (urc != MPSFT_OK) return urc;
@@ -258,6 +257,17 @@ int Batch::Build(hipStream_t s) {
   return MPSFT_OK;
 }
 
+// Signal i's Ctx in a batch: its engine's, counting its stop in the batch's
+// counters, and without the engine's host-mapped progress words, which only
+// a single-signal launch loop reads (the plan kernel would otherwise store
+// to host memory for every signal on every iteration).
+Ctx Batch::SignalCtx(const Engine &e) const {
+  Ctx c = e.MakeCtx();
+  c.batch_done = d_counters_ + 1;
+  c.progress = nullptr;
+  return c;
+}
+
 int Batch::FlushUploads(hipStream_t s) {
   auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
   int32_t m = 0;
@@ -331,8 +341,7 @@ int Batch::EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s) {
     const int rc = eng_[i]->EnsureTransforms(seeds_[i], want, opts_[i].trials, s);
     eng_[i]->defer_upload_ = nullptr;
     if (rc != MPSFT_OK) return rc;
-    ctxs[i] = eng_[i]->MakeCtx();
-    ctxs[i].batch_done = d_counters_ + 1;
+    ctxs[i] = SignalCtx(*eng_[i]);
   }
   const int urc = FlushUploads(s);
   if (urc != MPSFT_OK) return urc;
@@ -520,8 +529,7 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   if (rc0 != MPSFT_OK) return rc0;
   std::vector<Ctx> ctxs(K);
   for (int32_t i = 0; i < K; ++i) {
-    ctxs[i] = eng_[i]->MakeCtx();
-    ctxs[i].batch_done = d_counters_ + 1;
+    ctxs[i] = SignalCtx(*eng_[i]);
   }
   HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
   std::vector<AdjSig> sig(K);

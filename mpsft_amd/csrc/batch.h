@@ -61,6 +61,7 @@ class Batch {
   // Copies every engine's deferred transform upload (uploads_) in one
   // staging copy and one k_tf_scatter launch.
   int FlushUploads(hipStream_t s);
+  Ctx SignalCtx(const Engine &e) const;
 
   int status_ = MPSFT_OK;
   int precision_;
