@@ -2159,14 +2159,34 @@ __global__ void __launch_bounds__(kPlanThreads)
       s_rows[1][tid] = c.rows_cap;
     }
   }
-  for (int k = 0; k < kUnitKinds; ++k) part[k][tid] = sum[k];
-  __syncthreads();
-  for (int d = 1; d < kPlanThreads; d <<= 1) {  // inclusive scan of the sums
-    int32_t v[kUnitKinds];
-    for (int k = 0; k < kUnitKinds; ++k) v[k] = tid >= d ? part[k][tid - d] : 0;
+  // inclusive scan of the sums: within each wavefront by shuffles, then of
+  // the wavefronts' totals by the first wavefront (two barriers in all)
+  {
+    __shared__ int32_t wtot[kUnitKinds][kPlanThreads / 64];
+    const int lane = tid & 63, wv = tid >> 6;
+    int32_t inc[kUnitKinds];
+    for (int k = 0; k < kUnitKinds; ++k) {
+      int32_t v = sum[k];
+      for (int d = 1; d < 64; d <<= 1) {
+        const int32_t t = __shfl(v, lane - d);  // every lane takes part
+        if (lane >= d) v += t;
+      }
+      inc[k] = v;
+      if (lane == 63) wtot[k][wv] = v;
+    }
     __syncthreads();
-    for (int k = 0; k < kUnitKinds; ++k) part[k][tid] += v[k];
+    if (wv == 0) {
+      for (int k = 0; k < kUnitKinds; ++k) {
+        int32_t v = lane < kPlanThreads / 64 ? wtot[k][lane] : 0;
+        for (int d = 1; d < kPlanThreads / 64; d <<= 1) {
+          const int32_t t = __shfl(v, lane - d);
+          if (lane >= d) v += t;
+        }
+        if (lane < kPlanThreads / 64) wtot[k][lane] = v;
+      }
+    }
     __syncthreads();
+    for (int k = 0; k < kUnitKinds; ++k) part[k][tid] = inc[k] + (wv > 0 ? wtot[k][wv - 1] : 0);
   }
   int32_t base[kUnitKinds];
   for (int k = 0; k < kUnitKinds; ++k) base[k] = part[k][tid] - sum[k];
