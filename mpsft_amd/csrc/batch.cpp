@@ -18,14 +18,14 @@ namespace {
     if ((expr) != hipSuccess) return MPSFT_ERR_HIP;  \
   } while (0)
 
+// Iterations whose transforms a batched solve draws before its first launch.
+constexpr int32_t kBatchFirstDraws = 4;
+
 // Workgroups of each packed batched launch (default 8192: 32 per CU of
 // MI355X's 256; measured on 64 config-2 signals, 2048: 152.2, 4096: 148.9,
 // 8192: 145.9 us per solve), and bins per k_identify unit (default 16: four
 // per wavefront), each overridable for measurement by MPSFT_PACKED_GRID /
 // MPSFT_IDENT_BINS.
-// Iterations whose transforms a batched solve draws before its first launch.
-constexpr int32_t kBatchFirstDraws = 4;
-
 int32_t EnvInt(const char *name, int32_t def, int32_t lo, int32_t hi) {
   const char *e = std::getenv(name);
   if (!e) return def;
