@@ -535,7 +535,7 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   std::vector<AdjSig> sig(K);
   int32_t max_steps = 0;
   int64_t max_m = 1, max_n = 1, max_lags = 0;
-  int any_tab = 0, any_gen = 0;
+  int any_tab = 0, any_gen = 0, small_only = 1;
   for (int32_t i = 0; i < K; ++i) {
     Engine &e = *eng_[i];
     if (m[i] < 0 || (m[i] > 0 && (!keys[i] || !grad[i])) || !out[i]) return MPSFT_ERR_INVALID_ARG;
@@ -551,6 +551,7 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
     g.lags_stride = e.lags_stride_;
     max_lags = std::max(max_lags, e.lags_stride_);
     (g.lags ? any_tab : any_gen) = 1;
+    if (e.M_cap_ > (1 << kFftSmallLogM)) small_only = 0;
     g.rec = e.d_rec_;
     g.keys = keys[i];
     g.grad = static_cast<const double2 *>(grad[i]);
@@ -586,6 +587,7 @@ int Batch::Adjoint(const int64_t *const *keys, const void *const *grad, const in
   a.per_small = int32_t(std::min<int64_t>((max_m + kThreads - 1) / kThreads, 16));
   a.per_gather = int32_t(std::min<int64_t>(4096, (max_n + 1023) / 1024));
   a.per_lags = LagBlocks(max_lags);
+  a.fft_small_only = small_only;
   a.any_tab = any_tab;
   a.any_gen = any_gen;
   a.mixed = precision == MPSFT_MIXED ? 1 : 0;

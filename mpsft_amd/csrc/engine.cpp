@@ -1184,6 +1184,7 @@ int Engine::Adjoint(const int64_t *keys, const void *grad, int64_t m, void *out,
   a.per_small = 16;
   a.per_gather = int32_t(std::min<int64_t>(4096, (n_ + 1023) / 1024));
   a.per_lags = LagBlocks(lags_stride_);
+  a.fft_small_only = M_cap_ <= (1 << kFftSmallLogM) ? 1 : 0;
   a.any_tab = g.lags ? 1 : 0;
   a.any_gen = g.lags ? 0 : 1;
   a.mixed = precision == MPSFT_MIXED ? 1 : 0;

@@ -137,6 +137,7 @@ struct AdjLaunch {
   int32_t per_small, per_gather;  // blocks per signal: clear / seed, gather
   int32_t per_lags;               // blocks per (signal, step) of k_adj_lags
   int any_tab, any_gen;           // some signal has lag tables / some has none
+  int fft_small_only;             // every signal's bin FFTs are on the small path
   int mixed;                      // output precision
 };
 hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s);

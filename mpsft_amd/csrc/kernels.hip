@@ -2409,6 +2409,7 @@ __device__ __forceinline__ void AdjTermsBody(const Ctx &c, const IterParams &ip,
 // forward bin FFT on conj(Abar), output conjugated (the Bluestein matrix is
 // symmetric).  Block blk = trial * rows_cap + row; fbar holds
 // trials_cap x rows_cap x B_cap entries.
+template <int kPaths>
 __device__ __forceinline__ void AdjFftBody(const Ctx &c, const IterParams &ip, int32_t trials,
                                            double2 *fbar, int blk, double2 *lds) {
   const int32_t trial = blk / c.rows_cap;
@@ -2420,11 +2421,17 @@ __device__ __forceinline__ void AdjFftBody(const Ctx &c, const IterParams &ip, i
   const double2 rp = CisFrac(ModN(ip.tf.b * tau, n, mask), n);
   const int64_t off = (int64_t(trial) * c.rows_cap + row) * c.B_cap;
   double2 *a = lds;
-  const bool small = ip.logM <= kFftSmallLogM;
-  if (small)
+  // kFftSmallOnly: the launch's signals all have M <= 2^kFftSmallLogM, and
+  // the kernel carries the small path's registers only
+  const bool small = kPaths == kFftSmallOnly || ip.logM <= kFftSmallLogM;
+  if constexpr (kPaths == kFftSmallOnly) {
     BinFftSmall<true>(c, ip, c.A + off, rp, a);
-  else
-    BinFftLarge<true>(c, ip, c.A + off, rp, a);
+  } else {
+    if (small)
+      BinFftSmall<true>(c, ip, c.A + off, rp, a);
+    else
+      BinFftLarge<true>(c, ip, c.A + off, rp, a);
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < B; k += kFftThreads) fbar[off + k] = Conj(a[small ? k : Pad(k)]);
 }
@@ -2805,6 +2812,7 @@ __global__ void __launch_bounds__(kThreads)
 
 // Blocks below kPrepBlocks of each signal update lam (the terms), the
 // others transform one (trial, row) each.
+template <int kPaths>
 __global__ void __launch_bounds__(kFftThreads)
     k_adj_fft_terms(const Ctx *ctxs, const AdjSig *sig, const AdjStep *steps, const int2 *map) {
   HIP_DYNAMIC_SHARED(double2, lds)
@@ -2819,7 +2827,8 @@ __global__ void __launch_bounds__(kFftThreads)
     return;
   }
   const int64_t istride = int64_t(c.trials_cap) * c.rows_cap * c.B_cap;
-  AdjFftBody(c, st.ip, st.trials, g.fbar + int64_t(st.it) * istride, m.y - kPrepBlocks, lds);
+  AdjFftBody<kPaths>(c, st.ip, st.trials, g.fbar + int64_t(st.it) * istride, m.y - kPrepBlocks,
+                     lds);
 }
 
 // The lag tables of every (signal, step, trial) with a table (AdjSig.lags):
@@ -3131,8 +3140,12 @@ hipError_t LaunchAdj(const AdjLaunch &b, hipStream_t s) {
     const AdjStep *st = b.steps + int64_t(j) * b.count;
     hipLaunchKernelGGL(k_adj_bins, dim3(b.nb_bins), dim3(kThreads), 0, s, b.ctxs, b.sig, st,
                        b.map_bins, b.nblk_bins);
-    hipLaunchKernelGGL(k_adj_fft_terms, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft, s, b.ctxs,
-                       b.sig, st, b.map_fft);
+    if (b.fft_small_only)
+      hipLaunchKernelGGL(k_adj_fft_terms<kFftSmallOnly>, dim3(b.nb_fft), dim3(kFftThreads),
+                         b.lds_fft, s, b.ctxs, b.sig, st, b.map_fft);
+    else
+      hipLaunchKernelGGL(k_adj_fft_terms<kFftBoth>, dim3(b.nb_fft), dim3(kFftThreads), b.lds_fft,
+                         s, b.ctxs, b.sig, st, b.map_fft);
   }
   if (b.any_tab)
     hipLaunchKernelGGL(k_adj_lags, dim3(b.count * b.max_steps * b.per_lags), dim3(kThreads), 0,
