@@ -2,7 +2,7 @@
 bin FFTs (with plain, Kahan-compensated or double-float accumulation) against
 the FP64 form, measured on one real first iteration (n = 2^20, S = 1000
 log-uniform modes, sigma = 1e-7, B = 2001, complex64-rounded samples) and
-judged by the oracle's own decisions (Iterate's per-bin body, orc_identify).
+scored by the oracle's own decisions (Iterate's per-bin body, orc_identify).
 
 The fold is restated here in numpy from the device kernel's definition
 (fold[row][m] = sum_k x[a u + c] tap(t), u = q - p2 + (off(row) + k) B + rho,
@@ -140,7 +140,7 @@ def _bins(it, g, fold, fft32=False):
     return A
 
 
-def _judge(orc, it, A):
+def _decide(orc, it, A):
     """Decisions of Iterate's per-bin body on A against the FP64 form's."""
     mm = orc.ModeMap()
     _, tr = orc.identify(A[None], it["n"], it["opt"], it["tf"], np.array([it["q"]], np.int32), mm)
@@ -172,7 +172,7 @@ def test_precision_form_decisions(orc, iteration, form):
         A = _bins(iteration, g, _fold(g, "f64"), fft32=True)
     else:
         A = _bins(iteration, g, _fold(g, form))
-    r = _judge(orc, iteration, A)
+    r = _decide(orc, iteration, A)
     noise = iteration["sig"] / np.sqrt(2 * iteration["B"])
     print(f"[precision] {form}: bin error rms {r['rms']:.2e} (noise per bin {noise:.2e}), "
           f"decisions changed on {r['flips']} of {iteration['B']} bins, "
