@@ -139,11 +139,11 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
     if ((e.precision_ == MPSFT_MIXED) != (precision_ == MPSFT_MIXED)) return MPSFT_ERR_INVALID_ARG;
     ctxs[i] = SignalCtx(e);
   }
-  const int urc = FlushUploads(s);
+  // the transforms, the Ctx array and the loop's initial arguments in one
+  // staged copy and one scatter
+  const int urc = FlushUploads(s, Blob{ctxs.data(), d_ctxs_, ctxs.size() * sizeof(Ctx)},
+                               Blob{args_.data(), d_args_, args_.size() * sizeof(InitArgs)});
   if (urc != MPSFT_OK) return urc;
-  HIP_OK(hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx), hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(d_args_, args_.data(), args_.size() * sizeof(InitArgs),
-                        hipMemcpyHostToDevice, s));
   HIP_OK(hipStreamSynchronize(s));
   return Build(s);
 }
@@ -268,10 +268,13 @@ Ctx Batch::SignalCtx(const Engine &e) const {
   return c;
 }
 
-int Batch::FlushUploads(hipStream_t s) {
-  auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
-  int32_t m = 0;
-  size_t bytes = 0;
+int Batch::FlushUploads(hipStream_t s, const Blob &a, const Blob &b) {
+  static_assert(sizeof(DevTf) % 8 == 0 && sizeof(Ctx) % 8 == 0 && sizeof(InitArgs) % 8 == 0,
+                "staged payloads are copied in 8-byte words");
+  auto up16 = [](size_t n) { return (n + 15) & ~size_t(15); };
+  const bool blobs = a.bytes > 0 || b.bytes > 0;
+  int32_t m = blobs ? 1 : 0;
+  size_t bytes = up16(a.bytes) + up16(b.bytes);
   for (const TfUpload &u : uploads_)
     if (u.ntf > 0 || u.nq > 0) {
       ++m;
@@ -296,25 +299,29 @@ int Batch::FlushUploads(hipStream_t s) {
       stage_cap_ = cap;
     }
   }
-  if (!stage_ev_ && hipEventCreate(&stage_ev_) != hipSuccess)
-    return MPSFT_ERR_HIP;
+  if (!stage_ev_ && hipEventCreate(&stage_ev_) != hipSuccess) return MPSFT_ERR_HIP;
   unsigned char *h = p_stage_ + stage_used_;
   TfScatterDesc *desc = reinterpret_cast<TfScatterDesc *>(h);
   size_t off = head;
   int32_t k = 0;
+  // one payload: its bytes staged at off, its descriptor slot filled
+  auto put = [&](TfScatterDesc &d, int r, void *dst, const void *src, size_t n) {
+    d.dst[r] = dst;
+    d.off[r] = int64_t(off);
+    d.words[r] = int64_t(n / 8);
+    if (n) std::memcpy(h + off, src, n);
+    off += up16(n);
+  };
+  if (blobs) {
+    TfScatterDesc &d = desc[k++];
+    put(d, 0, a.dst, a.src, a.bytes);
+    put(d, 1, b.dst, b.src, b.bytes);
+  }
   for (const TfUpload &u : uploads_) {
     if (u.ntf <= 0 && u.nq <= 0) continue;
     TfScatterDesc &d = desc[k++];
-    d.tf_dst = u.tf_dst;
-    d.q_dst = u.q_dst;
-    d.ntf = u.ntf;
-    d.nq = u.nq;
-    d.tf_off = int64_t(off);
-    std::memcpy(h + off, u.tf_src, size_t(u.ntf) * sizeof(DevTf));
-    off += up16(size_t(u.ntf) * sizeof(DevTf));
-    d.q_off = int64_t(off);
-    std::memcpy(h + off, u.q_src, size_t(u.nq) * sizeof(int64_t));
-    off += up16(size_t(u.nq) * sizeof(int64_t));
+    put(d, 0, u.tf_dst, u.tf_src, size_t(u.ntf) * sizeof(DevTf));
+    put(d, 1, u.q_dst, u.q_src, size_t(u.nq) * sizeof(int64_t));
   }
   HIP_OK(hipMemcpyAsync(d_stage_ + stage_used_, h, bytes, hipMemcpyHostToDevice, s));
   HIP_OK(LaunchTfScatter(d_stage_ + stage_used_, m, s));
@@ -343,11 +350,9 @@ int Batch::EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s) {
     if (rc != MPSFT_OK) return rc;
     ctxs[i] = SignalCtx(*eng_[i]);
   }
-  const int urc = FlushUploads(s);
+  const int urc = FlushUploads(s, grow ? Blob{ctxs.data(), d_ctxs_, ctxs.size() * sizeof(Ctx)}
+                                      : Blob{});
   if (urc != MPSFT_OK) return urc;
-  if (grow && hipMemcpyAsync(d_ctxs_, ctxs.data(), ctxs.size() * sizeof(Ctx),
-                             hipMemcpyHostToDevice, s) != hipSuccess)
-    return MPSFT_ERR_HIP;
   return MPSFT_OK;
 }
 

@@ -14,6 +14,13 @@
 
 namespace mpsft {
 
+// A host array bound for device memory (a staged upload).
+struct Blob {
+  const void *src = nullptr;
+  void *dst = nullptr;
+  size_t bytes = 0;
+};
+
 class Batch {
  public:
   Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, int precision);
@@ -58,9 +65,10 @@ class Batch {
   int RunProfiled(hipStream_t s, int32_t max_iter, double *kernel_ms, int32_t *launches);
   int Finish(int32_t *iters, int32_t *status, hipStream_t s);
   int EnsureAhead(int32_t need, int32_t max_iter, hipStream_t s);
-  // Copies every engine's deferred transform upload (uploads_) in one
-  // staging copy and one k_tf_scatter launch.
-  int FlushUploads(hipStream_t s);
+  // Copies every engine's deferred transform upload (uploads_), and up to
+  // two host arrays (a, b) to device memory, in one staging copy and one
+  // k_tf_scatter launch.
+  int FlushUploads(hipStream_t s, const Blob &a = Blob{}, const Blob &b = Blob{});
   Ctx SignalCtx(const Engine &e) const;
 
   int status_ = MPSFT_OK;

@@ -73,13 +73,13 @@ struct BatchLaunch {
   int mixed;
 };
 // A staging area in device memory holding `count` descriptors, then their
-// payloads: k_tf_scatter copies each descriptor's transforms and q's to an
-// engine's arrays (one launch for every signal of a batch).
+// payloads: k_tf_scatter copies each descriptor's two payloads (an engine's
+// transforms and q's, or a batch's Ctx and InitArgs arrays) to their
+// destinations, in 8-byte words (one launch for every signal of a batch).
 struct TfScatterDesc {
-  DevTf *tf_dst;
-  int64_t *q_dst;
-  int64_t tf_off, q_off;  // payload offsets from the staging area, in bytes
-  int64_t ntf, nq;
+  void *dst[2];
+  int64_t off[2];    // payload offsets from the staging area, in bytes
+  int64_t words[2];  // payload sizes in 8-byte words
 };
 hipError_t LaunchTfScatter(const unsigned char *staging, int32_t count, hipStream_t s);
 hipError_t LaunchBatchReset(const Ctx *ctxs, const InitArgs *args, int32_t count,

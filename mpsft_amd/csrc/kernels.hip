@@ -2896,10 +2896,11 @@ __global__ void __launch_bounds__(kThreads)
 // One workgroup per descriptor of a staging area (TfScatterDesc).
 __global__ void __launch_bounds__(kThreads) k_tf_scatter(const unsigned char *base) {
   const TfScatterDesc d = reinterpret_cast<const TfScatterDesc *>(base)[blockIdx.x];
-  const DevTf *tf = reinterpret_cast<const DevTf *>(base + d.tf_off);
-  const int64_t *q = reinterpret_cast<const int64_t *>(base + d.q_off);
-  for (int64_t i = threadIdx.x; i < d.ntf; i += blockDim.x) d.tf_dst[i] = tf[i];
-  for (int64_t i = threadIdx.x; i < d.nq; i += blockDim.x) d.q_dst[i] = q[i];
+  for (int r = 0; r < 2; ++r) {
+    unsigned long long *dst = static_cast<unsigned long long *>(d.dst[r]);
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(base + d.off[r]);
+    for (int64_t i = threadIdx.x; i < d.words[r]; i += blockDim.x) dst[i] = src[i];
+  }
 }
 
 int32_t Blocks(int64_t work) {
