@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/mpsft.h"
+#include "mt.h"
 
 namespace mpsft {
 
@@ -344,8 +345,9 @@ class Engine {
   double2 *d_tmp_ = nullptr;     // cap double2 staging for mode I/O
   int64_t *d_tmpk_ = nullptr;    // cap int64 staging for mode I/O
 
-  // host RNG of the transform draws (src/base.cc:32-36)
-  std::mt19937 rng_;
+  // host RNG of the transform draws (src/base.cc:32-36): std::mt19937's
+  // stream, seeded lazily (a solve draws a few dozen words of the 624)
+  LazyMt19937 rng_;
   // RandomInt32 (src/base.cc:53): uniform_int_distribution<int32_t>() over
   // mt19937 as libstdc++ computed it before GCC 11: 32-bit outputs below
   // 2^31, the rest rejected.
