@@ -144,7 +144,6 @@ int Batch::Prepare(const void *const *x, const uint32_t *seeds, hipStream_t s) {
   const int urc = FlushUploads(s, Blob{ctxs.data(), d_ctxs_, ctxs.size() * sizeof(Ctx)},
                                Blob{args_.data(), d_args_, args_.size() * sizeof(InitArgs)});
   if (urc != MPSFT_OK) return urc;
-  HIP_OK(hipStreamSynchronize(s));
   return Build(s);
 }
 
@@ -154,6 +153,9 @@ int Batch::Build(hipStream_t s) {
   bool same = graph_ != nullptr && !graph_stale_;
   for (size_t i = 0; i < eng_.size() && same; ++i) same = built_gen_[i] == eng_[i]->shape_gen_;
   if (same) return MPSFT_OK;
+  // the maps below are rewritten by blocking copies: the queued work that
+  // may read them (an autotune pass's launches, the uploads) ends first
+  HIP_OK(hipStreamSynchronize(s));
   graph_stale_ = false;
   std::vector<int2> map_time, map_fft, map_ident;
   std::vector<int32_t> nblk_ident(eng_.size());
