@@ -66,6 +66,22 @@ def test_transform_make_matches_oracle(orc, kats):
     assert (tf.a * tf.a_inv) % (1 << 22) == 1
 
 
+def test_inverse_mod_power_of_two():
+    """The inverse of an odd multiplier modulo n = 2^k (Newton's iteration on
+    the host) equals Python's modular inverse for k = 1..62 (five steps are
+    needed above 2^48); other n (extended Euclid) alongside."""
+    rng = np.random.default_rng(5)
+    for k in range(1, 63):
+        n = 1 << k
+        for a in [int(v) | 1 for v in rng.integers(0, n, 40, dtype=np.uint64)] + [1, n - 1]:
+            tf = mpsft_amd.transform_make(n, a, 0, 0)
+            assert tf.a_inv == pow(a, -1, n), (n, a)
+    for n in (3, 1109, 1048573, 3 * (1 << 20)):
+        for a in rng.integers(1, n, 20):
+            if np.gcd(int(a), n) == 1:
+                assert mpsft_amd.transform_make(n, int(a), 0, 0).a_inv == pow(int(a), -1, n)
+
+
 def test_argument_errors_need_no_gpu():
     L = mpsft_amd.lib()
     tf = mpsft_amd.Transform()
