@@ -21,11 +21,14 @@ namespace {
 // Iterations whose transforms a batched solve draws before its first launch.
 constexpr int32_t kBatchFirstDraws = 4;
 
-// Workgroups of each packed batched launch (default 8192: 32 per CU of
-// MI355X's 256; measured on 64 config-2 signals, 2048: 152.2, 4096: 148.9,
-// 8192: 145.9 us per solve), and bins per k_identify unit (default 16: four
-// per wavefront), each overridable for measurement by MPSFT_PACKED_GRID /
-// MPSFT_IDENT_BINS.
+// Workgroups of each packed batched launch (default 24576: 96 per CU of
+// MI355X's 256, so that a launch of up to that many units gives each
+// workgroup one unit and leaves the balancing to the dispatcher; measured
+// on 64 signals of n = 2^22, S = 1000: 4096: 118.0, 8192: 116.7-116.9,
+// 16384: 116.3-116.6, 24576: 115.9, 32768: 116.0, 65536: 116.4 us per
+// solve; on 256 uneven signals 8192 and 24576 within 1%), and bins per
+// k_identify unit (default 16: four per wavefront), each overridable for
+// measurement by MPSFT_PACKED_GRID / MPSFT_IDENT_BINS.
 int32_t EnvInt(const char *name, int32_t def, int32_t lo, int32_t hi) {
   const char *e = std::getenv(name);
   if (!e) return def;
@@ -207,9 +210,9 @@ int Batch::Build(hipStream_t s) {
   bl_.nb_time = int32_t(map_time.size());
   bl_.nb_fft = int32_t(map_fft.size());
   bl_.nb_ident = int32_t(map_ident.size());
-  // The packed launches: at most kPackedGrid workgroups each (8 per CU),
+  // The packed launches: at most `grid` workgroups each (24576 by default),
   // each visiting units blockIdx.x, blockIdx.x + gridDim.x, ...
-  const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 8192, 64, 1 << 20);
+  const int32_t grid = EnvInt("MPSFT_PACKED_GRID", 24576, 64, 1 << 20);
   bl_.ident_bins = ident_bins_;
   int64_t trials_rows = 1;
   for (auto &e : eng_) trials_rows = std::max<int64_t>(trials_rows, int64_t(e->trials_cap_) * e->rows_cap_);
