@@ -446,6 +446,17 @@ class Solver:
         return tf, qs[:m * self.opt.trials].reshape(m, self.opt.trials)
 
 
+def _u32_array(values, count):
+    """`count` integers as a uint32 array, each taken modulo 2^32 (ctypes
+    stores an int's low 32 bits); other numbers go through int()."""
+    out = (C.c_uint32 * count)()
+    try:
+        out[:] = values
+    except TypeError:
+        out[:] = [int(v) & 0xFFFFFFFF for v in values]
+    return out
+
+
 class Batch:
     """The batched form of Demo1::Run (no reference counterpart): many
     independent signals, each with its own n, options and seed, recovered
@@ -486,9 +497,7 @@ class Batch:
             ptrs = (C.c_void_p * self.count)(*[_ptr(d).value for d in xds])
             if all(isinstance(d, DeviceArray) for d in xds):
                 self._ptr_cache = (key, ptrs, list(xds))
-        sd = (C.c_uint32 * self.count)()
-        sd[:] = [int(v) & 0xFFFFFFFF for v in seeds]
-        return ptrs, sd
+        return ptrs, _u32_array(seeds, self.count)
 
     def solve(self, xds, seeds, stream=None):
         """Returns (iterations, statuses) per signal; raises on a failure
@@ -499,7 +508,7 @@ class Batch:
         rc = lib().mpsft_batch_solve(self.h, ptrs, sd, it, stv, stream)
         if rc not in (0, 6, 7):  # per-signal capacity / iteration-cap statuses are reported
             check(rc, "mpsft_batch_solve")
-        return list(it), list(stv)
+        return it[:], stv[:]
 
     def solve_profiled(self, xds, seeds, stream=None):
         """The batched solve one iteration at a time with event-timed

@@ -145,3 +145,15 @@ def test_window_delta_from_taps(orc, n, bins, delta):
                 np.where(np.arange(wt.size) == wt.size // 3, wt * 1.01, wt)):
         with pytest.raises(mpsft_amd.MpsftError):
             mpsft_amd.window_delta(n, bins, bad)
+
+
+def test_batch_seed_array_takes_low_32_bits():
+    """Batch solve seeds reach the C-ABI as uint32: every kind of integer
+    (Python, numpy, negative, wider than 32 bits) and integral floats keep
+    their low 32 bits; a wrong count is refused."""
+    from mpsft_amd.api import _u32_array
+    vals = [5, (1 << 40) + 5, -1, np.int64(7), np.uint64((1 << 63) + 9), 2.0]
+    assert list(_u32_array(vals, len(vals))) == [5, 5, 0xFFFFFFFF, 7, 9, 2]
+    assert list(_u32_array(np.arange(4, dtype=np.int64), 4)) == [0, 1, 2, 3]
+    with pytest.raises(ValueError):
+        _u32_array([1, 2, 3], 4)
