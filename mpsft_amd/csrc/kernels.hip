@@ -2099,9 +2099,10 @@ __global__ void __launch_bounds__(kThreads)
 
 // ---- the batched iteration: plan, then three launches over packed units
 //
-// k_batch_plan starts every signal's next iteration (NextIteration and
-// PublishIteration, as k_bin_time's first workgroup does for one signal)
-// and packs the work of the three launches that follow: the units of
+// k_batch_signal starts every signal's next iteration (NextIteration and
+// PublishIteration, as k_bin_time's first workgroup does for one signal),
+// one wavefront per signal, and k_batch_plan (one workgroup) packs the
+// work of the three launches that follow: the units of
 // signal i occupy [off[i], off[i] + cnt[i]) of each launch's map, so a
 // launch's workgroups visit only units that exist this iteration (signals
 // that stopped, and the tiles of a B smaller than the largest, cost
@@ -2119,44 +2120,55 @@ __device__ __forceinline__ int32_t PrepUnits(int64_t terms) {
   return (u + 7) & ~7;
 }
 
+// Signal blockIdx.x's loop control, on the first lane of a wavefront of
+// its own (every signal's chain of dependent loads runs at once, spread
+// over the CUs): NextIteration and PublishLoop into slot p, then the
+// signal's unit counts and rows for k_batch_plan.
+__global__ void __launch_bounds__(64)
+    k_batch_signal(const Ctx *ctxs, int32_t count, int p, BatchWork w, int32_t ident_bins) {
+  const int32_t i = blockIdx.x;
+  if (threadIdx.x != 0 || i >= count) return;
+  const Ctx &c = ctxs[i];
+  // the iteration's parameters are built in place, in slot p
+  IterParams &ip = c.st->P[p];
+  LoopCounters lc;
+  NextIteration(c, p, &ip, &lc);
+  PublishLoop(c, p, ip, lc);
+  int32_t u[kUnitKinds] = {0, 0, 0, 0};
+  if (!ip.done) {
+    const int32_t trials = c.st->trials;
+    // a multiple of 8 units per signal: every signal's units start on XCD 0
+    u[0] = (PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
+            trials * ((ip.B + ip.R - 1) / ip.R) * ip.slices + 7) & ~7;
+    u[ip.logM <= kFftSmallLogM ? 1 : 3] = trials * ip.rows;
+    u[2] = (ip.B + ident_bins - 1) / ident_bins;
+  }
+  for (int k = 0; k < kUnitKinds; ++k) w.cnt[k * count + i] = u[k];
+  w.rows[i] = ip.rows;
+}
+
 __global__ void __launch_bounds__(kPlanThreads)
-    k_batch_plan(const Ctx *ctxs, int32_t count, int p, BatchWork w, int32_t ident_bins,
-                 int32_t *counters, int32_t *host) {
+    k_batch_plan(const Ctx *ctxs, int32_t count, BatchWork w, int32_t *counters, int32_t *host) {
   __shared__ int32_t part[kUnitKinds][kPlanThreads];
   // with at most kPlanThreads signals (one per thread): each signal's unit
   // counts, rows and rows_cap, for the map fill below
   __shared__ int32_t s_cnt[kUnitKinds][kPlanThreads];
   __shared__ int32_t s_rows[2][kPlanThreads];
   const int tid = threadIdx.x;
-  // signals [i0, i1) belong to this thread
+  // signals [i0, i1) belong to this thread; k_batch_signal wrote their
+  // counts and rows
   const int32_t per = (count + kPlanThreads - 1) / kPlanThreads;
   const int32_t i0 = min(count, tid * per), i1 = min(count, i0 + per);
   int32_t sum[kUnitKinds] = {0, 0, 0, 0};
   for (int32_t i = i0; i < i1; ++i) {
-    const Ctx &c = ctxs[i];
-    // the iteration's parameters are built in place, in slot p
-    IterParams &ip = c.st->P[p];
-    LoopCounters lc;
-    NextIteration(c, p, &ip, &lc);
-    PublishLoop(c, p, ip, lc);
-    int32_t u[kUnitKinds] = {0, 0, 0, 0};
-    if (!ip.done) {
-      const int32_t trials = c.st->trials;
-      // a multiple of 8 units per signal: every signal's units start on XCD 0
-      u[0] = (PrepUnits(int64_t(min(c.st->count, c.cap)) * trials) +
-              trials * ((ip.B + ip.R - 1) / ip.R) * ip.slices + 7) & ~7;
-      u[ip.logM <= kFftSmallLogM ? 1 : 3] = trials * ip.rows;
-      u[2] = (ip.B + ident_bins - 1) / ident_bins;
-    }
     for (int k = 0; k < kUnitKinds; ++k) {
-      w.cnt[k * count + i] = u[k];
-      sum[k] += u[k];
-      if (per == 1) s_cnt[k][tid] = u[k];
+      const int32_t u = w.cnt[k * count + i];
+      sum[k] += u;
+      if (per == 1) s_cnt[k][tid] = u;
     }
-    w.rows[i] = ip.rows;
     if (per == 1) {
-      s_rows[0][tid] = ip.rows;
-      s_rows[1][tid] = c.rows_cap;
+      s_rows[0][tid] = w.rows[i];
+      s_rows[1][tid] = ctxs[i].rows_cap;
     }
   }
   // inclusive scan of the sums: within each wavefront by shuffles, then of
@@ -3062,8 +3074,10 @@ hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStre
   const BatchWork &w = b.work;
   switch (stage) {
     case 0:
-      hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, s, b.ctxs, b.count, parity,
-                         w, b.ident_bins, b.counters, b.host);
+      hipLaunchKernelGGL(k_batch_signal, dim3(b.count), dim3(64), 0, s, b.ctxs, b.count, parity, w,
+                         b.ident_bins);
+      hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(kPlanThreads), 0, s, b.ctxs, b.count, w,
+                         b.counters, b.host);
       break;
     case 1:
       if (b.mixed)
