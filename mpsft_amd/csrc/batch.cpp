@@ -82,19 +82,21 @@ Batch::Batch(int32_t count, const int64_t *n, const mpsft_solve_options *opts, i
       AllocDev(&d_counters_, 2) != MPSFT_OK || AllocDev(&d_results_, count) != MPSFT_OK ||
       hipHostMalloc(reinterpret_cast<void **>(&h_results_), count * sizeof(BatchResultDev)) !=
           hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&hp), 2 * sizeof(int32_t),
+      hipHostMalloc(reinterpret_cast<void **>(&hp), 4 * sizeof(int32_t),
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer(reinterpret_cast<void **>(&d_host_), hp, 0) != hipSuccess) {
     status_ = MPSFT_ERR_NOMEM;
     return;
   }
   h_host_ = hp;
-  hp[0] = hp[1] = 0;
+  hp[0] = hp[1] = hp[3] = 0;
+  hp[2] = 1;
 }
 
 Batch::~Batch() {
   (void)hipDeviceSynchronize();
   if (graph_) (void)hipGraphExecDestroy(graph_);
+  if (graph_small_) (void)hipGraphExecDestroy(graph_small_);
   (void)hipFree(d_ctxs_);
   (void)hipFree(d_args_);
   (void)hipFree(d_map_);
@@ -236,27 +238,35 @@ int Batch::Build(hipStream_t s) {
   bl_.host = d_host_;
   bl_.mixed = precision_ == MPSFT_MIXED ? 1 : 0;
 
-  if (graph_) {
-    (void)hipGraphExecDestroy(graph_);
-    graph_ = nullptr;
-  }
-  hipStream_t cs;
-  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+  for (hipGraphExec_t *ge : {&graph_, &graph_small_})
+    if (*ge) {
+      (void)hipGraphExecDestroy(*ge);
+      *ge = nullptr;
+    }
+  // graph_ with every launch, graph_small_ without the large-path bin FFT
+  bl_.large_fft = 1;
+  for (int large = 1; large >= 0; --large) {
+    BatchLaunch b = bl_;
+    b.large_fft = large;
+    hipStream_t cs;
+    HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return MPSFT_ERR_HIP;
+    }
+    const hipError_t e0 = LaunchBatchIteration(b, 0, cs);
+    const hipError_t e1 = LaunchBatchIteration(b, 1, cs);
+    hipGraph_t g;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
-    return MPSFT_ERR_HIP;
-  }
-  const hipError_t e0 = LaunchBatchIteration(bl_, 0, cs);
-  const hipError_t e1 = LaunchBatchIteration(bl_, 1, cs);
-  hipGraph_t g;
-  const hipError_t ec = hipStreamEndCapture(cs, &g);
-  (void)hipStreamDestroy(cs);
-  if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) return MPSFT_ERR_HIP;
-  const hipError_t ei = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (ei != hipSuccess) {
-    graph_ = nullptr;
-    return MPSFT_ERR_HIP;
+    if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) return MPSFT_ERR_HIP;
+    hipGraphExec_t *ge = large ? &graph_ : &graph_small_;
+    const hipError_t ei = hipGraphInstantiate(ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      *ge = nullptr;
+      return MPSFT_ERR_HIP;
+    }
   }
   (void)s;
   return MPSFT_OK;
@@ -382,7 +392,10 @@ int Batch::Run(hipStream_t s, int32_t max_iter) {
     if (launched - started < lookahead) {
       const int rc = EnsureAhead(std::min(max_iter, launched + 3), max_iter, s);
       if (rc != MPSFT_OK) return fail(rc);
-      if (hipGraphLaunch(graph_, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
+      // without the large-path bin FFT once a started iteration had none
+      // (a signal's B never grows, so none come back this solve)
+      const hipGraphExec_t g = h_host_[2] == 0 && graph_small_ ? graph_small_ : graph_;
+      if (hipGraphLaunch(g, s) != hipSuccess) return fail(MPSFT_ERR_HIP);
       launched += 2;
       continue;
     }
@@ -408,6 +421,7 @@ int Batch::Solve(const void *const *x, const uint32_t *seeds, int32_t *iters, in
   for (auto &a : args_) max_iter = std::max(max_iter, a.max_iter);
   h_host_[0] = 0;
   h_host_[1] = 0;
+  h_host_[2] = 1;  // large-path bin-FFT units: assumed until a plan reports none
   HIP_OK(LaunchBatchReset(d_ctxs_, d_args_, count(), d_counters_, s));
   rc = Run(s, max_iter);
   if (rc != MPSFT_OK) return rc;
@@ -424,6 +438,7 @@ int Batch::SolveProfiled(const void *const *x, const uint32_t *seeds, int32_t *i
   for (auto &a : args_) max_iter = std::max(max_iter, a.max_iter);
   h_host_[0] = 0;
   h_host_[1] = 0;
+  h_host_[2] = 1;  // large-path bin-FFT units: assumed until a plan reports none
   HIP_OK(LaunchBatchReset(d_ctxs_, d_args_, count(), d_counters_, s));
   rc = RunProfiled(s, max_iter, kernel_ms, launches);
   if (rc != MPSFT_OK) return rc;

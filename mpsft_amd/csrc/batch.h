@@ -92,6 +92,9 @@ class Batch {
   int32_t *d_host_ = nullptr;
   BatchLaunch bl_{};
   hipGraphExec_t graph_ = nullptr;
+  // the same two iterations without the large-path bin-FFT launch, for the
+  // rest of a solve once an iteration has no large-path units
+  hipGraphExec_t graph_small_ = nullptr;
   bool graph_stale_ = false;   // launch parameters changed since the graph's capture
   int32_t ident_bins_ = 16;    // k_identify bins per unit (autotuned)
   AdjSig *d_adj_sig_ = nullptr;

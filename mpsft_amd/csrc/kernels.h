@@ -69,8 +69,10 @@ struct BatchLaunch {
   size_t lds_time, lds_fft;             // dynamic LDS (max over signals)
   size_t lds_fft_small;                 // the small bin-FFT path's (max over signals)
   int32_t *counters;                    // device {iterations started, signals stopped}
-  int32_t *host;                        // host-mapped copy of counters
+  int32_t *host;                        // host-mapped: the counters, then the iteration's
+                                        // large-path bin-FFT units
   int mixed;
+  int large_fft = 1;                    // 0: the large-path bin-FFT launch is left out
 };
 // A staging area in device memory holding `count` descriptors, then their
 // payloads: k_tf_scatter copies each descriptor's two payloads (an engine's

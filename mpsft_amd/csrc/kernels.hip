@@ -2219,6 +2219,10 @@ __global__ void __launch_bounds__(kPlanThreads)
     // publishes after the previous one has finished).
     const int32_t v = atomicAdd(&counters[0], 1) + 1;
     const int32_t d = __hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the large-path bin-FFT units of this iteration: a signal's B never
+    // grows during a solve, so once none are left none come back
+    __hip_atomic_store(&host[2], part[3][kPlanThreads - 1], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[1], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&host[0], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -3090,8 +3094,9 @@ hipError_t LaunchBatchStage(const BatchLaunch &b, int parity, int stage, hipStre
     case 2:  // one kernel per bin-FFT path, each over its own units
       hipLaunchKernelGGL(k_bin_fft_batch<kFftSmallOnly>, dim3(b.grid_fft), dim3(kFftThreads),
                          b.lds_fft_small, s, b.ctxs, w.map[1], w.total, parity);
-      hipLaunchKernelGGL(k_bin_fft_batch<kFftLargeOnly>, dim3(b.grid_fft), dim3(kFftThreads),
-                         b.lds_fft, s, b.ctxs, w.map[3], w.total, parity);
+      if (b.large_fft)
+        hipLaunchKernelGGL(k_bin_fft_batch<kFftLargeOnly>, dim3(b.grid_fft), dim3(kFftThreads),
+                           b.lds_fft, s, b.ctxs, w.map[3], w.total, parity);
       break;
     default:
       if (b.col_slices == 1)
